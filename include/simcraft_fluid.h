@@ -1,0 +1,184 @@
+/*
+ * simcraft_fluid.h - C ABI of libsimcraft_fluid.so, an MI355X (gfx950) implementation of
+ * SimCraft FluidSim's FLIP/PIC substep.
+ *
+ * Plain C: pointers, sizes and status codes only. All arrays crossing the ABI use the
+ * reference's dense layout (Spatify LinearHashXYZ, Spatify/include/Spatify/hash.h:25):
+ *   idx = k + nz' * (j + ny' * i)     (k fastest, x slowest)
+ * where (nx', ny', nz') are the dimensions of the array in question:
+ *   cell-centred arrays           nx     x ny     x nz
+ *   u face grid (axis 0)          (nx+1) x ny     x nz
+ *   v face grid (axis 1)          nx     x (ny+1) x nz
+ *   w face grid (axis 2)          nx     x ny     x (nz+1)
+ * Particles are AoS xyz doubles (std::vector<Vec3d> in the reference,
+ * FluidSim/include/FluidSim/cpu/fluid-simulator.h:200-202).
+ *
+ * A context owns every device buffer and one HIP stream. Host arrays are copied at the
+ * call and never retained. A context is not re-entrant; contexts are independent.
+ * No entry point aborts the process: failures return a status and fs_last_error()
+ * holds the message.
+ */
+#ifndef SIMCRAFT_FLUID_H_
+#define SIMCRAFT_FLUID_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The reference's failure channels are an undefined ERROR() macro, asserts and
+ * a std::cerr warning (FluidSim/src/cpu/fluid-simulator.cc:169-172); these replace them. */
+typedef enum fs_status {
+  FS_OK = 0,
+  FS_NOT_CONVERGED = 1, /* PCG hit its iteration cap (CgSolver3D max_iterations) */
+  FS_BREAKDOWN = 2,     /* s.z == 0 in PCG (asserted at project-solver.cc:330) */
+  FS_NONFINITE = 3,     /* NaN/Inf residual */
+  FS_BAD_ARG = 4,
+  FS_DEVICE_ERROR = 5,  /* HIP error, or a device-side wait that exceeded its bound */
+  FS_UNSUPPORTED = 6,   /* e.g. APIC, Multigrid: declared but unimplemented in the reference */
+  FS_NO_DEVICE = 7
+} fs_status;
+
+/* Mirrors fluid::AdvectionMethod (FluidSim/include/FluidSim/fluid-simulator.h:14-18). */
+enum { FS_ADVECT_PIC = 0, FS_ADVECT_FLIP = 1, FS_ADVECT_APIC = 2 };
+/* Mirrors fluid::ProjectSolver (fluid-simulator.h:20-23). */
+enum { FS_PROJECT_FVM = 0, FS_PROJECT_FDM = 1 };
+/* Mirrors fluid::PreconditionerMethod (fluid-simulator.h:30-33). */
+enum { FS_PRECOND_MULTIGRID = 0, FS_PRECOND_MIC = 1 };
+/* Mirrors fluid::CompressedSolverMethod (fluid-simulator.h:35-37). */
+enum { FS_SOLVER_CG = 0 };
+/* Mirrors fluid::ReconstructorMethod (fluid-simulator.h:39-41). */
+enum { FS_RECON_NAIVE = 0 };
+
+/* Arithmetic of the whole step. */
+enum {
+  FS_PRECISION_FP64 = 0,  /* reference arithmetic: FP64 everywhere */
+  FS_PRECISION_MIXED = 1  /* FP32 particles/grids/stencil/PCG vectors; FP64 dot products,
+                             norms and an outer FP64 iterative refinement of the pressure */
+};
+
+/* PCG stopping rule. */
+enum {
+  FS_TOL_ABS_LINF = 0, /* reference: ||r||_inf < tol (project-solver.cc:334-335, project-solver.h:129) */
+  FS_TOL_REL_L2 = 1    /* ||r||_2 <= tol * ||rhs||_2 */
+};
+
+/* Dot products inside PCG on the device always use one fixed reduction tree, so results are
+ * deterministic from run to run. */
+
+typedef struct fs_config {
+  int32_t nx, ny, nz;             /* resolution (cells) */
+  double size_x, size_y, size_z;  /* domain extent; origin is (0,0,0) */
+  int32_t precision;              /* FS_PRECISION_* */
+  int32_t advection;              /* FS_ADVECT_* (setAdvector) */
+  double flip_alpha;              /* FLIP blend, advect-solver.h:129 (0.97) */
+  double mic_tau;                 /* MIC(0) tuning, project-solver.h:83 (0.97) */
+  double mic_sigma;               /* MIC(0) safety, project-solver.h:84 (0.25) */
+  double cg_tolerance;            /* project-solver.h:129 (1e-6) */
+  int32_t cg_max_iterations;      /* project-solver.h:130 (300) */
+  int32_t tol_mode;               /* FS_TOL_* */
+  double gravity;                 /* applyForce, fluid-simulator.cc:16 (9.8, along -y) */
+  int32_t sdf_extrapolate_iters;  /* fluid-simulator.cc:151 (10) */
+  int32_t sdf_smooth_iters;       /* fluid-simulator.cc:152 (5) */
+  int32_t vel_extrapolate_iters;  /* fluid-simulator.cc:160-162 (10) */
+  double recon_radius_factor;     /* fluid-simulator.cc:147 (0.8 * h) */
+  int32_t advect_on_cleared_grid; /* 1 = reference order: clear() before advect (fluid-simulator.cc:141-143) */
+  double cfl_factor;              /* fluid-simulator.cc:202 (5.0) */
+  /* mixed precision only */
+  double inner_tolerance;         /* relative ||r||_2 reduction of each FP32 inner solve */
+  int32_t max_refinements;        /* outer FP64 refinement passes */
+  int32_t check_interval;         /* PCG iterations between host convergence reads */
+  int32_t profile_stages;         /* 1 = record per-stage device times into fs_stats */
+  int32_t reserved[7];
+} fs_config;
+
+typedef struct fs_stats {
+  int32_t status;         /* fs_status of the step */
+  int32_t cg_iterations;  /* PCG iterations, counted as CgSolver3D::solve counts them (project-solver.cc:326-344);
+                             mixed precision: total inner iterations */
+  int32_t refinements;    /* outer FP64 passes (mixed precision), else 0 */
+  int32_t n_active;       /* active cells in the last pressure solve */
+  double residual_inf;    /* final ||r||_inf of the pressure system (FP64) */
+  double residual_rel;    /* final ||r||_2 / ||rhs||_2 (FP64) */
+  double rhs_norm2;       /* ||rhs||_2 */
+  int32_t substeps;       /* substeps taken by fs_step_frame */
+  int32_t pad0;
+  double ms_advect, ms_reconstruct, ms_p2g, ms_extrapolate, ms_build, ms_pcg, ms_project, ms_g2p;
+  double ms_total;
+} fs_stats;
+
+typedef struct fs_ctx fs_ctx;
+
+/* Reference defaults (each field cites its source above). */
+void fs_config_default(fs_config* cfg);
+
+/* Construct / destroy. Replaces cpu::FluidSimulator(n, size, resolution)
+ * (FluidSim/include/FluidSim/cpu/fluid-simulator.h:28-60) plus FluidSimulator::setBackend
+ * (FluidSim/src/fluid-simulator.cc:4-9); the particle count is set by fs_set_particles. */
+int fs_create(const fs_config* cfg, fs_ctx** out);
+void fs_destroy(fs_ctx* ctx);
+const char* fs_last_error(const fs_ctx* ctx);
+/* Number of visible HIP devices (0 when none); never initialises more than the runtime. */
+int fs_device_count(void);
+
+/* ---- Backend level: FluidComputeBackend (fluid-simulator.h:43-53) and cpu::FluidSimulator ---- */
+/* setCollider(mesh) (cpu/fluid-simulator.h:62-71) takes a mesh and builds the SDF with
+ * manifold2SDF; this entry takes the cell-centred collider SDF (nx*ny*nz doubles) directly. */
+int fs_set_collider_sdf(fs_ctx* ctx, const double* sdf);
+/* setInitialFluid (cpu/fluid-simulator.h:73-78) draws unseeded random particles; this entry
+ * takes explicit positions (and velocities; NULL = zero). Positions must lie in the domain. */
+int fs_set_particles(fs_ctx* ctx, const double* pos_xyz, const double* vel_xyz, int64_t n);
+int fs_get_particles(fs_ctx* ctx, double* pos_xyz, double* vel_xyz);
+int64_t fs_num_particles(const fs_ctx* ctx);
+int fs_set_advector(fs_ctx* ctx, int32_t method);                       /* cpu/fluid-simulator.h:80-98 */
+int fs_set_projector(fs_ctx* ctx, int32_t method);                      /* cpu/fluid-simulator.h:100-107 */
+int fs_set_compressed_solver(fs_ctx* ctx, int32_t solver, int32_t precond); /* :109-113 */
+int fs_set_reconstructor(fs_ctx* ctx, int32_t method);                  /* :168-179 */
+/* cpu::FluidSimulator::substep(dt) (FluidSim/src/cpu/fluid-simulator.cc:140-182). */
+int fs_substep(fs_ctx* ctx, double dt, fs_stats* stats);
+/* cpu::FluidSimulator::step(Frame&) (fluid-simulator.cc:205-220): CFL-limited substeps
+ * until frame_dt is covered. */
+int fs_step_frame(fs_ctx* ctx, double frame_dt, fs_stats* stats);
+/* CFL() (fluid-simulator.cc:184-203). */
+int fs_cfl(fs_ctx* ctx, double* dt_out);
+
+/* ---- Operator level: the stages of substep, each on the context's device state ---- */
+int fs_clear(fs_ctx* ctx);                                   /* fluid-simulator.cc:20-37 */
+int fs_advect(fs_ctx* ctx, double dt);                       /* advect-solver.cc:170-187 (+handleCollision :100-137) */
+int fs_reconstruct(fs_ctx* ctx, double radius);              /* sdf.h:210-225, ns-util.h:46-109 */
+int fs_extrapolate_fluid_sdf(fs_ctx* ctx, int32_t iters);    /* cpu/fluid-simulator.h:123-166 */
+int fs_smooth_fluid_surface(fs_ctx* ctx, int32_t iters);     /* fluid-simulator.cc:39-74 */
+int fs_p2g(fs_ctx* ctx, double dt);                          /* advect-solver.cc:188-271 (PIC :18-98) */
+int fs_apply_dirichlet_boundary(fs_ctx* ctx);                /* fluid-simulator.cc:113-138 */
+int fs_extrapolate_velocity(fs_ctx* ctx, int32_t iters);     /* cpu/fluid-simulator.h:204-253, 3 grids */
+int fs_apply_force(fs_ctx* ctx, double dt);                  /* fluid-simulator.cc:13-18 */
+int fs_build_system(fs_ctx* ctx, double dt);                 /* project-solver.cc:59-228 */
+int fs_solve_pressure(fs_ctx* ctx, fs_stats* stats);         /* CgSolver3D::solve, project-solver.cc:308-346 */
+int fs_project(fs_ctx* ctx, double dt);                      /* project-solver.cc:348-435 */
+int fs_apply_collider(fs_ctx* ctx);                          /* fluid-simulator.cc:76-111 */
+int fs_g2p(fs_ctx* ctx, double dt);                          /* advect-solver.cc:157-169 (PIC :7-17) */
+
+/* applyCompressedMatrix (project-solver.cc:16-39) on the assembled system: y = A x over active
+ * cells (inactive entries of y are left untouched). x, y: dense nx*ny*nz. */
+int fs_spmv(fs_ctx* ctx, const double* x, double* y);
+/* ModifiedIncompleteCholesky3D::precond (project-solver.cc:230-306): z = M^-1 r over active
+ * cells of the assembled system. */
+int fs_mic_apply(fs_ctx* ctx, const double* r, double* z);
+
+/* ---- State access (dense reference layout) ----
+ * Names: "u","v","w","ubuf","vbuf","wbuf","u_valid","v_valid","w_valid","u_valid_buf",
+ * "v_valid_buf","w_valid_buf","uw","vw","ww","u_last","v_last","w_last","fluid_sdf",
+ * "fluid_sdf_buf","sdf_valid","sdf_valid_buf","collider_sdf","u_weights","v_weights",
+ * "w_weights","adiag","a_left","a_right","a_up","a_down","a_front","a_back","rhs",
+ * "active","pg","mic_e".
+ * dtype: 0 = float64, 1 = uint8. Floating arrays are always exchanged as float64. */
+int fs_array_info(const fs_ctx* ctx, const char* name, int64_t dims[3], int32_t* dtype);
+int fs_get_array(fs_ctx* ctx, const char* name, void* out);
+int fs_set_array(fs_ctx* ctx, const char* name, const void* in);
+int fs_synchronize(fs_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SIMCRAFT_FLUID_H_ */

@@ -1,0 +1,266 @@
+"""ctypes binding of the oracle (oracle/libfs_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, by __graft_entry__.smoke() as the checker and by
+bench.py's cpu_baseline leg. The methods mirror simcraft_amd.FluidSim so that a parity test
+drives both through the same calls.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from simcraft_amd._capi import fs_config, fs_stats, default_config  # noqa: F401  (structs only)
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libfs_oracle.so"
+DOT_SERIAL, DOT_TREE = 0, 1
+P2G_SORTED, P2G_INDEX = 0, 1
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_SIGS = {
+    "fso_create": (_P, [C.POINTER(fs_config)]),
+    "fso_destroy": (None, [_P]),
+    "fso_set_dot_mode": (C.c_int, [_P, C.c_int]),
+    "fso_set_p2g_order": (C.c_int, [_P, C.c_int]),
+    "fso_set_particles": (C.c_int, [_P, _D, _D, C.c_int64]),
+    "fso_get_particles": (C.c_int, [_P, _D, _D]),
+    "fso_num_particles": (C.c_int64, [_P]),
+    "fso_set_collider_sdf": (C.c_int, [_P, _D]),
+    "fso_clear": (C.c_int, [_P]),
+    "fso_advect": (C.c_int, [_P, C.c_double]),
+    "fso_reconstruct": (C.c_int, [_P, C.c_double]),
+    "fso_extrapolate_fluid_sdf": (C.c_int, [_P, C.c_int]),
+    "fso_smooth_fluid_surface": (C.c_int, [_P, C.c_int]),
+    "fso_p2g": (C.c_int, [_P, C.c_double]),
+    "fso_apply_dirichlet_boundary": (C.c_int, [_P]),
+    "fso_extrapolate_velocity": (C.c_int, [_P, C.c_int]),
+    "fso_apply_force": (C.c_int, [_P, C.c_double]),
+    "fso_build_system": (C.c_int, [_P, C.c_double]),
+    "fso_solve_pressure": (C.c_int, [_P, C.POINTER(fs_stats)]),
+    "fso_project": (C.c_int, [_P, C.c_double]),
+    "fso_apply_collider": (C.c_int, [_P]),
+    "fso_g2p": (C.c_int, [_P, C.c_double]),
+    "fso_substep": (C.c_int, [_P, C.c_double, C.POINTER(fs_stats)]),
+    "fso_step_frame": (C.c_int, [_P, C.c_double, C.POINTER(fs_stats)]),
+    "fso_cfl": (C.c_int, [_P, _D]),
+    "fso_spmv": (C.c_int, [_P, _D, _D]),
+    "fso_mic_apply": (C.c_int, [_P, _D, _D]),
+    "fso_array_info": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "fso_get_array": (C.c_int, [_P, C.c_char_p, _P]),
+    "fso_set_array": (C.c_int, [_P, C.c_char_p, _P]),
+    "fso_fraction_inside": (C.c_double, [C.c_double] * 4),
+    "fso_morton3d": (C.c_int, [C.c_int] * 3),
+    "fso_bspline": (C.c_double, [C.c_double]),
+    "fso_sdf_eval": (C.c_double, [_P, _D]),
+    "fso_sdf_grad": (None, [_P, _D, _D]),
+    "fso_sample_velocity": (None, [_P, _D, _D]),
+    "fso_nearest": (None, [_P, C.c_int, _D, C.POINTER(C.c_int)]),
+    "fso_last_iterations": (C.c_int, [_P]),
+}
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        lib = C.CDLL(str(LIB_PATH), mode=os.RTLD_LOCAL)
+        for n, (res, args) in _SIGS.items():
+            f = getattr(lib, n)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def fraction_inside(lu, ru, rd, ld) -> float:
+    return load().fso_fraction_inside(lu, ru, rd, ld)
+
+
+def morton3d(x, y, z) -> int:
+    return load().fso_morton3d(x, y, z)
+
+
+def bspline(r) -> float:
+    return load().fso_bspline(r)
+
+
+class OracleSim:
+    """CPU FP64 restatement of cpu::FluidSimulator with the same methods as FluidSim."""
+
+    def __init__(self, cfg: fs_config | None = None, dot_mode: int = DOT_TREE, p2g_order: int = P2G_SORTED,
+                 **overrides):
+        self._lib = load()
+        self.cfg = cfg if cfg is not None else default_config()
+        for k, v in overrides.items():
+            setattr(self.cfg, k, v)
+        self._ctx = self._lib.fso_create(C.byref(self.cfg))
+        if not self._ctx:
+            raise ValueError("bad oracle config")
+        self._lib.fso_set_dot_mode(self._ctx, dot_mode)
+        self._lib.fso_set_p2g_order(self._ctx, p2g_order)
+        self.shape = (self.cfg.nx, self.cfg.ny, self.cfg.nz)
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.fso_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_dot_mode(self, mode: int):
+        self._lib.fso_set_dot_mode(self._ctx, mode)
+
+    def set_collider_sdf(self, sdf):
+        a = np.ascontiguousarray(sdf, dtype=np.float64)
+        assert a.shape == self.shape
+        self._lib.fso_set_collider_sdf(self._ctx, _dp(a))
+
+    def set_particles(self, pos, vel=None):
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        self._lib.fso_set_particles(self._ctx, _dp(p), _dp(v) if v is not None else None, len(p))
+
+    def num_particles(self):
+        return int(self._lib.fso_num_particles(self._ctx))
+
+    def particles(self):
+        n = self.num_particles()
+        p, v = np.empty((n, 3)), np.empty((n, 3))
+        self._lib.fso_get_particles(self._ctx, _dp(p), _dp(v))
+        return p, v
+
+    def positions(self):
+        return self.particles()[0]
+
+    def substep(self, dt):
+        st = fs_stats()
+        self._lib.fso_substep(self._ctx, dt, C.byref(st))
+        return st.as_dict()
+
+    def step_frame(self, frame_dt=1.0 / 60.0):
+        st = fs_stats()
+        self._lib.fso_step_frame(self._ctx, frame_dt, C.byref(st))
+        return st.as_dict()
+
+    def cfl(self):
+        out = C.c_double()
+        self._lib.fso_cfl(self._ctx, C.byref(out))
+        return out.value
+
+    def clear(self):
+        self._lib.fso_clear(self._ctx)
+
+    def advect(self, dt):
+        self._lib.fso_advect(self._ctx, dt)
+
+    def reconstruct(self, radius):
+        self._lib.fso_reconstruct(self._ctx, radius)
+
+    def extrapolate_fluid_sdf(self, iters):
+        self._lib.fso_extrapolate_fluid_sdf(self._ctx, iters)
+
+    def smooth_fluid_surface(self, iters):
+        self._lib.fso_smooth_fluid_surface(self._ctx, iters)
+
+    def p2g(self, dt=0.0):
+        self._lib.fso_p2g(self._ctx, dt)
+
+    def apply_dirichlet_boundary(self):
+        self._lib.fso_apply_dirichlet_boundary(self._ctx)
+
+    def extrapolate_velocity(self, iters):
+        self._lib.fso_extrapolate_velocity(self._ctx, iters)
+
+    def apply_force(self, dt):
+        self._lib.fso_apply_force(self._ctx, dt)
+
+    def build_system(self, dt):
+        self._lib.fso_build_system(self._ctx, dt)
+
+    def solve_pressure(self):
+        st = fs_stats()
+        self._lib.fso_solve_pressure(self._ctx, C.byref(st))
+        return st.as_dict()
+
+    def project(self, dt):
+        self._lib.fso_project(self._ctx, dt)
+
+    def apply_collider(self):
+        self._lib.fso_apply_collider(self._ctx)
+
+    def g2p(self, dt=0.0):
+        self._lib.fso_g2p(self._ctx, dt)
+
+    def spmv(self, x, y=None):
+        xa = np.ascontiguousarray(x, dtype=np.float64)
+        ya = np.zeros(self.shape) if y is None else np.array(y, dtype=np.float64, copy=True)
+        self._lib.fso_spmv(self._ctx, _dp(xa), _dp(ya))
+        return ya
+
+    def mic_apply(self, r, z=None):
+        ra = np.ascontiguousarray(r, dtype=np.float64)
+        za = np.zeros(self.shape) if z is None else np.array(z, dtype=np.float64, copy=True)
+        self._lib.fso_mic_apply(self._ctx, _dp(ra), _dp(za))
+        return za
+
+    def array_info(self, name):
+        dims = (C.c_int64 * 3)()
+        dt = C.c_int32()
+        if self._lib.fso_array_info(self._ctx, name.encode(), dims, C.byref(dt)) != 0:
+            raise KeyError(name)
+        return (dims[0], dims[1], dims[2]), (np.dtype(np.uint8) if dt.value == 1 else np.dtype(np.float64))
+
+    def get(self, name):
+        shape, dtype = self.array_info(name)
+        out = np.empty(shape, dtype=dtype)
+        self._lib.fso_get_array(self._ctx, name.encode(), out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def set(self, name, value):
+        shape, dtype = self.array_info(name)
+        a = np.ascontiguousarray(value, dtype=dtype)
+        assert a.shape == shape, (name, a.shape, shape)
+        self._lib.fso_set_array(self._ctx, name.encode(), a.ctypes.data_as(C.c_void_p))
+
+    # point queries used by known-answer tests
+    def sdf_eval(self, p):
+        pa = np.ascontiguousarray(p, dtype=np.float64)
+        return self._lib.fso_sdf_eval(self._ctx, _dp(pa))
+
+    def sdf_grad(self, p):
+        pa = np.ascontiguousarray(p, dtype=np.float64)
+        out = np.empty(3)
+        self._lib.fso_sdf_grad(self._ctx, _dp(pa), _dp(out))
+        return out
+
+    def sample_velocity(self, p):
+        pa = np.ascontiguousarray(p, dtype=np.float64)
+        out = np.empty(3)
+        self._lib.fso_sample_velocity(self._ctx, _dp(pa), _dp(out))
+        return out
+
+    def nearest(self, axis, p):
+        pa = np.ascontiguousarray(p, dtype=np.float64)
+        out = (C.c_int * 3)()
+        self._lib.fso_nearest(self._ctx, axis, _dp(pa), out)
+        return tuple(out)

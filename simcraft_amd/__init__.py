@@ -1,0 +1,223 @@
+"""simcraft_amd - SimCraft FluidSim's FLIP/PIC substep on MI355X (HIP, gfx950).
+
+Python host mirror of the reference's simulator interface (fluid::cpu::FluidSimulator and its
+solver plug-ins, FluidSim/include/FluidSim/cpu/fluid-simulator.h:25-272) over the C ABI of
+libsimcraft_fluid.so. All compute runs in HIP kernels; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import warnings
+
+import numpy as np
+
+from . import _capi
+from ._capi import (ADVECT_FLIP, ADVECT_PIC, PRECISION_FP64, PRECISION_MIXED, PRECOND_MIC, PROJECT_FVM, RECON_NAIVE,
+                    SOLVER_CG, TOL_ABS_LINF, TOL_REL_L2, fs_config, fs_stats)
+
+__all__ = ["FluidSim", "FluidSimError", "default_config", "device_count", "fs_config", "fs_stats"]
+
+_DP = C.POINTER(C.c_double)
+
+
+class FluidSimError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{_capi.STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+def default_config(**overrides) -> fs_config:
+    """fs_config with the reference defaults (fs_config_default) and the given overrides."""
+    return _capi.default_config(**overrides)
+
+
+def device_count() -> int:
+    return int(_capi.load().fs_device_count())
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_DP)
+
+
+class FluidSim:
+    """One simulation context on the current HIP device.
+
+    Mirrors cpu::FluidSimulator: construct with a resolution/size config, set the collider
+    and the initial fluid, choose the solvers, then step. Operator-level methods run single
+    stages of substep() on the device-resident state (used for parity tests).
+    """
+
+    def __init__(self, cfg: fs_config | None = None, **overrides):
+        self._lib = _capi.load()
+        self.cfg = cfg if cfg is not None else default_config()
+        for k, v in overrides.items():
+            setattr(self.cfg, k, v)
+        ptr = C.c_void_p()
+        st = self._lib.fs_create(C.byref(self.cfg), C.byref(ptr))
+        if st != _capi.FS_OK:
+            raise FluidSimError(st, (self._lib.fs_last_error(None) or b"").decode())
+        self._ctx = ptr
+        self.shape = (self.cfg.nx, self.cfg.ny, self.cfg.nz)
+
+    # -- lifetime -------------------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_ctx", None):
+            self._lib.fs_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int, allow=(_capi.FS_OK,)) -> int:
+        if st in allow:
+            return st
+        msg = (self._lib.fs_last_error(self._ctx) or b"").decode()
+        raise FluidSimError(st, msg)
+
+    # -- backend level (FluidComputeBackend) --------------------------------------------------
+    def set_collider_sdf(self, sdf: np.ndarray) -> None:
+        a = np.ascontiguousarray(sdf, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"collider SDF must have shape {self.shape}")
+        self._check(self._lib.fs_set_collider_sdf(self._ctx, _dptr(a)))
+
+    def set_particles(self, pos: np.ndarray, vel: np.ndarray | None = None) -> None:
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        if v is not None and v.shape != p.shape:
+            raise ValueError("velocities must match positions")
+        self._check(self._lib.fs_set_particles(self._ctx, _dptr(p), _dptr(v) if v is not None else None, len(p)))
+
+    def num_particles(self) -> int:
+        return int(self._lib.fs_num_particles(self._ctx))
+
+    def particles(self) -> tuple[np.ndarray, np.ndarray]:
+        n = self.num_particles()
+        p = np.empty((n, 3))
+        v = np.empty((n, 3))
+        self._check(self._lib.fs_get_particles(self._ctx, _dptr(p), _dptr(v)))
+        return p, v
+
+    def positions(self) -> np.ndarray:
+        return self.particles()[0]
+
+    def set_advector(self, method: int = ADVECT_FLIP) -> None:
+        self._check(self._lib.fs_set_advector(self._ctx, method))
+
+    def set_projector(self, method: int = PROJECT_FVM) -> None:
+        self._check(self._lib.fs_set_projector(self._ctx, method))
+
+    def set_compressed_solver(self, solver: int = SOLVER_CG, precond: int = PRECOND_MIC) -> None:
+        self._check(self._lib.fs_set_compressed_solver(self._ctx, solver, precond))
+
+    def set_reconstructor(self, method: int = RECON_NAIVE) -> None:
+        self._check(self._lib.fs_set_reconstructor(self._ctx, method))
+
+    def substep(self, dt: float) -> dict:
+        st = fs_stats()
+        s = self._check(self._lib.fs_substep(self._ctx, dt, C.byref(st)),
+                        allow=(_capi.FS_OK, _capi.FS_NOT_CONVERGED, _capi.FS_BREAKDOWN, _capi.FS_NONFINITE))
+        if s != _capi.FS_OK:
+            warnings.warn(f"pressure solve ended with {_capi.STATUS_NAMES[s]} (residual {st.residual_inf:g})")
+        return st.as_dict()
+
+    def step_frame(self, frame_dt: float = 1.0 / 60.0) -> dict:
+        st = fs_stats()
+        self._check(self._lib.fs_step_frame(self._ctx, frame_dt, C.byref(st)),
+                    allow=(_capi.FS_OK, _capi.FS_NOT_CONVERGED))
+        return st.as_dict()
+
+    def cfl(self) -> float:
+        out = C.c_double()
+        self._check(self._lib.fs_cfl(self._ctx, C.byref(out)))
+        return out.value
+
+    # -- operator level ------------------------------------------------------------------------
+    def clear(self):
+        self._check(self._lib.fs_clear(self._ctx))
+
+    def advect(self, dt: float):
+        self._check(self._lib.fs_advect(self._ctx, dt))
+
+    def reconstruct(self, radius: float):
+        self._check(self._lib.fs_reconstruct(self._ctx, radius))
+
+    def extrapolate_fluid_sdf(self, iters: int):
+        self._check(self._lib.fs_extrapolate_fluid_sdf(self._ctx, iters))
+
+    def smooth_fluid_surface(self, iters: int):
+        self._check(self._lib.fs_smooth_fluid_surface(self._ctx, iters))
+
+    def p2g(self, dt: float = 0.0):
+        self._check(self._lib.fs_p2g(self._ctx, dt))
+
+    def apply_dirichlet_boundary(self):
+        self._check(self._lib.fs_apply_dirichlet_boundary(self._ctx))
+
+    def extrapolate_velocity(self, iters: int):
+        self._check(self._lib.fs_extrapolate_velocity(self._ctx, iters))
+
+    def apply_force(self, dt: float):
+        self._check(self._lib.fs_apply_force(self._ctx, dt))
+
+    def build_system(self, dt: float):
+        self._check(self._lib.fs_build_system(self._ctx, dt))
+
+    def solve_pressure(self) -> dict:
+        st = fs_stats()
+        self._check(self._lib.fs_solve_pressure(self._ctx, C.byref(st)),
+                    allow=(_capi.FS_OK, _capi.FS_NOT_CONVERGED, _capi.FS_BREAKDOWN, _capi.FS_NONFINITE))
+        return st.as_dict()
+
+    def project(self, dt: float):
+        self._check(self._lib.fs_project(self._ctx, dt))
+
+    def apply_collider(self):
+        self._check(self._lib.fs_apply_collider(self._ctx))
+
+    def g2p(self, dt: float = 0.0):
+        self._check(self._lib.fs_g2p(self._ctx, dt))
+
+    def spmv(self, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:
+        xa = np.ascontiguousarray(x, dtype=np.float64)
+        ya = np.zeros(self.shape) if y is None else np.array(y, dtype=np.float64, copy=True)
+        self._check(self._lib.fs_spmv(self._ctx, _dptr(xa), _dptr(ya)))
+        return ya
+
+    def mic_apply(self, r: np.ndarray, z: np.ndarray | None = None) -> np.ndarray:
+        ra = np.ascontiguousarray(r, dtype=np.float64)
+        za = np.zeros(self.shape) if z is None else np.array(z, dtype=np.float64, copy=True)
+        self._check(self._lib.fs_mic_apply(self._ctx, _dptr(ra), _dptr(za)))
+        return za
+
+    # -- named state ---------------------------------------------------------------------------
+    def array_info(self, name: str) -> tuple[tuple[int, int, int], np.dtype]:
+        dims = (C.c_int64 * 3)()
+        dt = C.c_int32()
+        self._check(self._lib.fs_array_info(self._ctx, name.encode(), dims, C.byref(dt)))
+        return (dims[0], dims[1], dims[2]), (np.dtype(np.uint8) if dt.value == 1 else np.dtype(np.float64))
+
+    def get(self, name: str) -> np.ndarray:
+        shape, dtype = self.array_info(name)
+        out = np.empty(shape, dtype=dtype)
+        self._check(self._lib.fs_get_array(self._ctx, name.encode(), out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set(self, name: str, value: np.ndarray) -> None:
+        shape, dtype = self.array_info(name)
+        a = np.ascontiguousarray(value, dtype=dtype)
+        if a.shape != shape:
+            raise ValueError(f"{name} must have shape {shape}")
+        self._check(self._lib.fs_set_array(self._ctx, name.encode(), a.ctypes.data_as(C.c_void_p)))
+
+    def synchronize(self) -> None:
+        self._check(self._lib.fs_synchronize(self._ctx))

@@ -1,0 +1,151 @@
+// fs_ctx.h - the context behind the C ABI: device state of one FLIP simulation.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/simcraft_fluid.h"
+#include "fs_device.h"
+
+// Status-returning HIP check used by host code of the library.
+#define FS_HIP(ctx, call)                                                          \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      (ctx)->set_error(std::string(#call) + ": " + hipGetErrorString(e_));         \
+      return FS_DEVICE_ERROR;                                                      \
+    }                                                                              \
+  } while (0)
+
+#define FS_TRY(expr)          \
+  do {                        \
+    int s_ = (expr);          \
+    if (s_ != FS_OK) return s_; \
+  } while (0)
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Device scalars of one pressure solve. Written by the last block of a reduction kernel,
+// read by the next kernels and copied to the host at convergence checks.
+struct PcgScalars {
+  double sigma;      // z.r of the current iteration
+  double alpha;      // sigma / (s.As)
+  double beta;
+  double sdotz;
+  double residual;   // ||r||_inf
+  double rnorm2sq;   // ||r||_2^2 (fixed tree)
+  double bnorm2sq;   // ||rhs||_2^2
+  double tol_abs;    // absolute threshold applied to ||r||_inf or ||r||_2^2 depending on mode
+  int32_t converged;
+  int32_t conv_iter;
+  int32_t status;
+  int32_t wait_timeouts;
+  uint32_t done_ctr[8];  // last-block counters, one per kernel kind (reset by the last block)
+};
+
+struct PcgState {
+  int nbi = 0, nbj = 0, S = 0, nlive = 0;
+  int64_t ne = 0;                // elements nlive*S*64
+  int nchunks = 0;               // ceil(ne/4096)
+  std::vector<int> live_host;    // brick -> live index or -1
+  DBuf live_map;                 // int[nbi*nbj]
+  DBuf brick_ij;                 // int2[nlive]
+  DBuf fwd_order, bwd_order;     // int[nlive]: dependency order (anti-diagonal) and its reverse
+  DBuf brick_flag;               // int[nbi*nbj]
+  // packed system (element layout)
+  DBuf adiag, al, ad, ab, nbm, rhs;
+  DBuf e, cl, cd, cb, dr, du, df;  // MIC factor and sweep coefficients
+  DBuf tflags, tnum;             // factor sweep: flag word and 6 tau numerators per element
+  DBuf x, r, s, t, q, z;         // PCG vectors
+  DBuf partials;                 // double[max(nchunks, nlive)]
+  DBuf partials2;                // second partial array (max reductions)
+  DBuf scal;                     // PcgScalars
+  DBuf progress;                 // uint64[3*nlive]: factor/fwd/bwd progress per brick
+  DBuf tickets;                  // uint64[3]
+  uint64_t ticket_base[3] = {0, 0, 0};
+  uint64_t epoch[3] = {0, 0, 0};
+  PcgScalars* host_scal = nullptr;  // pinned
+  bool built = false;
+  // mixed-precision copies
+  DBuf f_adiag, f_al, f_ad, f_ab, f_e, f_cl, f_cd, f_cb, f_dr, f_du, f_df;
+  DBuf f_r, f_s, f_t, f_q, f_z, f_x;
+};
+
+struct fs_ctx {
+  fs_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int nx = 0, ny = 0, nz = 0;
+  double h[3] = {0, 0, 0};
+  bool mixed = false;
+  size_t rsz = 8;  // bytes per real of grids/particles (8 FP64, 4 mixed)
+  int n_face[3] = {0, 0, 0};
+  int fdim[3][3];  // face grid dims
+  int n_cell = 0;
+
+  // particles (SoA)
+  int64_t np = 0;
+  DBuf pos[3], vel[3];
+  DBuf staging;  // AoS upload/download
+  // face grids: current, ping-pong buffer, valid flags, P2G weight sums, FLIP copy, solid weights
+  DBuf g[3], gbuf[3], gvalid[3], gvalidbuf[3], gw[3], glast[3], gweights[3];
+  // cell-centred
+  DBuf fsdf, fsdfbuf, sdfvalid, sdfvalidbuf, collider, pg;
+  DBuf adiag, an[6], rhs, active;
+  // binning
+  DBuf keys, slots, perm, counts, offs, scan_tmp;
+  size_t scan_tmp_bytes = 0;
+  size_t bins_cap = 0;
+  DBuf red;  // small reduction scratch
+  PcgState pcg;
+  hipEvent_t ev[12];
+  bool ev_ok = false;
+
+  void set_error(const std::string& s) { err = s; }
+};
+
+// Allocation helpers
+int fs_alloc(fs_ctx* c, DBuf& b, size_t bytes);
+void fs_free(DBuf& b);
+int fs_zero(fs_ctx* c, DBuf& b);
+
+// Stages implemented in the .hip files.
+int fs_stage_clear(fs_ctx* c);
+int fs_stage_advect(fs_ctx* c, double dt);
+int fs_stage_reconstruct(fs_ctx* c, double radius);
+int fs_stage_extrapolate_sdf(fs_ctx* c, int iters);
+int fs_stage_smooth(fs_ctx* c, int iters);
+int fs_stage_p2g(fs_ctx* c);
+int fs_stage_dirichlet(fs_ctx* c);
+int fs_stage_extrapolate_velocity(fs_ctx* c, int iters);
+int fs_stage_force(fs_ctx* c, double dt);
+int fs_stage_build(fs_ctx* c, double dt);
+int fs_stage_solve(fs_ctx* c, fs_stats* st);
+int fs_stage_project(fs_ctx* c, double dt);
+int fs_stage_collider(fs_ctx* c);
+int fs_stage_g2p(fs_ctx* c);
+int fs_stage_cfl(fs_ctx* c, double* out);
+int fs_pcg_spmv_dense(fs_ctx* c, const double* x, double* y);
+int fs_pcg_mic_dense(fs_ctx* c, const double* r, double* z);
+int fs_pcg_export_e(fs_ctx* c, double* out);
+void fs_pcg_release(fs_ctx* c);
+int fs_particles_upload(fs_ctx* c, const double* pos, const double* vel);
+int fs_particles_download(fs_ctx* c, double* pos, double* vel);
+
+// Conversions between the FP64 ABI and device storage of a given real size.
+int fs_upload_real(fs_ctx* c, DBuf& dst, const double* src, size_t n);
+int fs_download_real(fs_ctx* c, const DBuf& src, double* dst, size_t n);
+
+inline unsigned fs_blocks(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }

@@ -1,0 +1,1085 @@
+// fs_grid.hip - HIP kernels for every non-PCG stage of the FLIP substep, gfx950.
+//
+// Layout: dense arrays, k fastest (Spatify LinearHashXYZ, hash.h:25); particles SoA.
+// Each kernel restates one SimCraft function (cited) with the same per-element operation
+// order, so the FP64 build is bit-identical to a correctly rounded CPU evaluation.
+// P2G is a deterministic gather over particles binned by their nearest node (counting sort,
+// particle index ascending inside a bin) instead of the reference's racy TBB scatter.
+#include <hipcub/hipcub.hpp>
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "fs_ctx.h"
+
+using namespace fsd;
+
+namespace {
+
+constexpr int TPB = 256;
+
+template <typename T>
+struct GridArgs {
+  int nx, ny, nz;
+  T h[3];
+  T ext[3];
+};
+
+template <typename T>
+GridArgs<T> grid_args(const fs_ctx* c) {
+  GridArgs<T> a;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  for (int d = 0; d < 3; d++) a.h[d] = (T)c->h[d];
+  // HybridAdvectionSolver3D width/height/depth = resolution * h (cpu/fluid-simulator.h:88-95)
+  a.ext[0] = (T)(c->nx * c->h[0]);
+  a.ext[1] = (T)(c->ny * c->h[1]);
+  a.ext[2] = (T)(c->nz * c->h[2]);
+  return a;
+}
+
+__device__ inline void unlin3(int id, int ny, int nz, int& i, int& j, int& k) {
+  k = id % nz;
+  int t = id / nz;
+  j = t % ny;
+  i = t / ny;
+}
+
+// 4x4x4 stencil of one face grid around TGrid::nearest: first node index and 1D kernel
+// factors. The node weight CubicKernel::weight(h, p - pn) is (wx*wy)*wz.
+template <typename T>
+struct Stencil {
+  int b[3];
+  T w[3][4];
+};
+
+template <typename T>
+__device__ inline void make_stencil(const T p[3], int axis, const T h[3], const T hs[3], Stencil<T>& st) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    T off = (c == axis) ? T(0) : T(0.5);
+    int nb = nearest1(p[c], off, h[c]);
+    st.b[c] = nb - 1;
+#pragma unroll
+    for (int o = 0; o < 4; o++) {
+      T pn = coord1(nb - 1 + o, off, h[c]);
+      st.w[c][o] = bspline(fabs(p[c] - pn) / hs[c]);
+    }
+  }
+}
+
+// One grid of sampleVelocity (FluidSim/src/cpu/util.cc:40-85): i outer, k inner,
+// out-of-range nodes skipped; two grids sampled with the same weights (FLIP: current, last).
+template <typename T, bool TWO>
+__device__ inline void sample_grid(const Stencil<T>& st, const T* __restrict__ G, const T* __restrict__ G2, int gx,
+                                   int gy, int gz, T& acc, T& acc2, T& ws) {
+  acc = T(0);
+  acc2 = T(0);
+  ws = T(0);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int x = st.b[0] + i;
+    if (x < 0 || x >= gx) continue;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      int y = st.b[1] + j;
+      if (y < 0 || y >= gy) continue;
+      T wxy = st.w[0][i] * st.w[1][j];
+      int base = gz * (y + gy * x);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        int z = st.b[2] + k;
+        if (z < 0 || z >= gz) continue;
+        T w = wxy * st.w[2][k];
+        acc = acc + G[base + z] * w;
+        if (TWO) acc2 = acc2 + G2[base + z] * w;
+        ws = ws + w;
+      }
+    }
+  }
+}
+
+template <typename T>
+struct Grids3 {
+  const T* g[3];
+  int d[3][3];
+};
+
+// sampleVelocity over the three face grids.
+template <typename T>
+__device__ inline V3<T> sample_velocity(const GridArgs<T>& ga, const Grids3<T>& G, T px, T py, T pz) {
+  T p[3] = {px, py, pz};
+  T hs[3] = {ga.h[0], ga.h[0], ga.h[0]};  // scalar h.x (util.cc:47)
+  T out[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    Stencil<T> st;
+    make_stencil(p, a, ga.h, hs, st);
+    T acc, acc2, ws;
+    sample_grid<T, false>(st, G.g[a], nullptr, G.d[a][0], G.d[a][1], G.d[a][2], acc, acc2, ws);
+    out[a] = ws > T(0) ? acc / ws : T(0);
+  }
+  return {out[0], out[1], out[2]};
+}
+
+// HybridAdvectionSolver3D::handleCollision (FluidSim/src/cpu/advect-solver.cc:100-137)
+template <typename T>
+__device__ inline void handle_collision(const GridArgs<T>& ga, const T* __restrict__ col, V3<T>& p, V3<T>& v) {
+  T d = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p.x, p.y, p.z);
+  if (d < T(0)) {
+    V3<T> n = gnormalize(sdf_grad(col, ga.nx, ga.ny, ga.nz, ga.h, p.x, p.y, p.z));
+    p.x = p.x - d * n.x;
+    p.y = p.y - d * n.y;
+    p.z = p.z - d * n.z;
+    T dv = v.x * n.x + v.y * n.y + v.z * n.z;
+    T m = smin(dv, T(0));
+    v.x = v.x - m * n.x;
+    v.y = v.y - m * n.y;
+    v.z = v.z - m * n.z;
+  }
+  if (p.x < T(0)) {
+    p.x = T(0);
+    if (v.x < T(0)) v.x = T(0);
+  }
+  if (p.x > ga.ext[0]) {
+    p.x = ga.ext[0];
+    if (v.x > T(0)) v.x = T(0);
+  }
+  if (p.y < T(0)) {
+    p.y = T(0);
+    if (v.y < T(0)) v.y = T(0);
+  }
+  if (p.y > ga.ext[1]) {
+    p.y = ga.ext[1];
+    if (v.y > T(0)) v.y = T(0);
+  }
+  if (p.z < T(0)) {
+    p.z = T(0);
+    if (v.z < T(0)) v.z = T(0);
+  }
+  if (p.z > ga.ext[2]) {
+    p.z = ga.ext[2];
+    if (v.z > T(0)) v.z = T(0);
+  }
+}
+
+// FlipAdvectionSolver3D::advect (advect-solver.cc:170-187): Ralston RK3 + collision.
+template <typename T>
+__global__ void k_advect(int n, GridArgs<T> ga, Grids3<T> G, const T* __restrict__ col, T* __restrict__ px,
+                         T* __restrict__ py, T* __restrict__ pz, T* __restrict__ vx, T* __restrict__ vy,
+                         T* __restrict__ vz, T dt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  V3<T> p = {px[i], py[i], pz[i]};
+  V3<T> k1 = {vx[i], vy[i], vz[i]};
+  T c1 = T(0.5) * dt;
+  V3<T> p1 = {p.x + c1 * k1.x, p.y + c1 * k1.y, p.z + c1 * k1.z};
+  V3<T> k2 = sample_velocity(ga, G, p1.x, p1.y, p1.z);
+  T c2 = T(0.75) * dt;
+  V3<T> p2 = {p.x + c2 * k2.x, p.y + c2 * k2.y, p.z + c2 * k2.z};
+  V3<T> k3 = sample_velocity(ga, G, p2.x, p2.y, p2.z);
+  T a1 = T(2.0 / 9.0) * dt, a2 = T(1.0 / 3.0) * dt, a3 = T(4.0 / 9.0) * dt;
+  p.x = p.x + a1 * k1.x + a2 * k2.x + a3 * k3.x;
+  p.y = p.y + a1 * k1.y + a2 * k2.y + a3 * k3.y;
+  p.z = p.z + a1 * k1.z + a2 * k2.z + a3 * k3.z;
+  V3<T> v = k1;
+  handle_collision(ga, col, p, v);
+  px[i] = p.x;
+  py[i] = p.y;
+  pz[i] = p.z;
+  vx[i] = v.x;
+  vy[i] = v.y;
+  vz[i] = v.z;
+}
+
+// FlipAdvectionSolver3D::solveG2P / PicAdvector3D::solveG2P (advect-solver.cc:7-17, 157-169).
+template <typename T, bool FLIP>
+__global__ void k_g2p(int n, GridArgs<T> ga, Grids3<T> G, Grids3<T> L, const T* __restrict__ px,
+                      const T* __restrict__ py, const T* __restrict__ pz, T* __restrict__ vx, T* __restrict__ vy,
+                      T* __restrict__ vz, T alpha) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T p[3] = {px[i], py[i], pz[i]};
+  T hs[3] = {ga.h[0], ga.h[0], ga.h[0]};
+  T* vout[3] = {vx, vy, vz};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    Stencil<T> st;
+    make_stencil(p, a, ga.h, hs, st);
+    T acc, acc2, ws;
+    sample_grid<T, FLIP>(st, G.g[a], L.g[a], G.d[a][0], G.d[a][1], G.d[a][2], acc, acc2, ws);
+    T pic = ws > T(0) ? acc / ws : T(0);
+    if (FLIP) {
+      T old = ws > T(0) ? acc2 / ws : T(0);
+      T fl = vout[a][i] + pic - old;
+      vout[a][i] = alpha * fl + (T(1) - alpha) * pic;
+    } else {
+      vout[a][i] = pic;
+    }
+  }
+}
+
+// ---- counting sort of particles into bins (stable: index ascending inside a bin) ----
+template <typename T>
+__global__ void k_keys_nearest(int n, int axis, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
+                               const T* __restrict__ pz, int* __restrict__ keys) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // bin grid covers nearest in [0, n_c] on every axis for positions inside the domain
+  int b0 = nearest1(px[i], axis == 0 ? T(0) : T(0.5), ga.h[0]);
+  int b1 = nearest1(py[i], axis == 1 ? T(0) : T(0.5), ga.h[1]);
+  int b2 = nearest1(pz[i], axis == 2 ? T(0) : T(0.5), ga.h[2]);
+  b0 = fclamp(b0, 0, ga.nx);
+  b1 = fclamp(b1, 0, ga.ny);
+  b2 = fclamp(b2, 0, ga.nz);
+  keys[i] = lin3(b0, b1, b2, ga.ny + 1, ga.nz + 1);
+}
+
+// ParticleNeighbourSearcher::update key (Spatify/include/Spatify/ns-util.h:50-62)
+template <typename T>
+__global__ void k_keys_morton(int n, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
+                              const T* __restrict__ pz, int* __restrict__ keys) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int ix = (int)floor(px[i] / ga.h[0]);
+  if (ix < 0) ix = 0;
+  if (ix >= ga.nx) ix = ga.nx - 1;
+  int iy = (int)floor(py[i] / ga.h[1]);
+  if (iy < 0) iy = 0;
+  if (iy >= ga.ny) iy = ga.ny - 1;
+  int iz = (int)floor(pz[i] / ga.h[2]);
+  if (iz < 0) iz = 0;
+  if (iz >= ga.nz) iz = ga.nz - 1;
+  keys[i] = morton3D(ix, iy, iz);
+}
+
+__global__ void k_count(int n, const int* __restrict__ keys, int* __restrict__ counts, int* __restrict__ slots) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  slots[i] = atomicAdd(&counts[keys[i]], 1);
+}
+
+__global__ void k_scatter(int n, const int* __restrict__ keys, const int* __restrict__ slots,
+                          const int* __restrict__ offs, int* __restrict__ perm) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  perm[offs[keys[i]] + slots[i]] = i;
+}
+
+__global__ void k_sort_bins(int nb, const int* __restrict__ offs, int* __restrict__ perm) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  int s = offs[b], e = offs[b + 1];
+  for (int i = s + 1; i < e; i++) {
+    int v = perm[i];
+    int j = i - 1;
+    while (j >= s && perm[j] > v) {
+      perm[j + 1] = perm[j];
+      j--;
+    }
+    perm[j + 1] = v;
+  }
+}
+
+// P2G gather for one face grid (FlipAdvectionSolver3D::solveP2G, advect-solver.cc:188-271):
+// node n sums v*w and w over particles whose nearest node b satisfies n-b in {-1..2}^3,
+// visiting bins b in lexicographic order and particles by index; then ug /= uw or 0.
+template <typename T>
+__global__ void k_p2g_gather(int axis, GridArgs<T> ga, int gx, int gy, int gz, const int* __restrict__ offs,
+                             const int* __restrict__ perm, const T* __restrict__ px, const T* __restrict__ py,
+                             const T* __restrict__ pz, const T* __restrict__ vc, T* __restrict__ G,
+                             T* __restrict__ Wt, uint8_t* __restrict__ valid) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= gx * gy * gz) return;
+  int x, y, z;
+  unlin3(id, gy, gz, x, y, z);
+  const int bx = ga.nx + 1, by = ga.ny + 1, bz = ga.nz + 1;
+  T o0 = axis == 0 ? T(0) : T(0.5), o1 = axis == 1 ? T(0) : T(0.5), o2 = axis == 2 ? T(0) : T(0.5);
+  T pnx = coord1(x, o0, ga.h[0]), pny = coord1(y, o1, ga.h[1]), pnz = coord1(z, o2, ga.h[2]);
+  T acc = T(0), ws = T(0);
+  int x0 = max(x - 2, 0), x1 = min(x + 1, bx - 1);
+  int y0 = max(y - 2, 0), y1 = min(y + 1, by - 1);
+  int z0 = max(z - 2, 0), z1 = min(z + 1, bz - 1);
+  for (int b0 = x0; b0 <= x1; b0++)
+    for (int b1 = y0; b1 <= y1; b1++) {
+      int row = bz * (b1 + by * b0);
+      int s = offs[row + z0], e = offs[row + z1 + 1];  // bins of one z-run are contiguous
+      for (int t = s; t < e; t++) {
+        int p = perm[t];
+        T w = bspline(fabs(px[p] - pnx) / ga.h[0]) * bspline(fabs(py[p] - pny) / ga.h[1]) *
+              bspline(fabs(pz[p] - pnz) / ga.h[2]);
+        acc = acc + vc[p] * w;
+        ws = ws + w;
+      }
+    }
+  if (ws > T(0)) {
+    G[id] = acc / ws;
+    valid[id] = 1;
+  } else {
+    G[id] = T(0);
+    valid[id] = 0;
+  }
+  Wt[id] = ws;
+}
+
+// NaiveReconstructor::reconstruct (FluidSim/include/FluidSim/cpu/sdf.h:210-225) with
+// forNeighbours (ns-util.h:80-109). Each bin's last particle is skipped (inclusive end used
+// as exclusive bound). Since sqrt and x-radius are monotone, min(|x_c-x_p|-r) over the
+// included particles equals sqrt(min |x_c-x_p|^2) - r; the inclusion test |d| <= 2r is
+// d^2 <= s_max with s_max the largest double whose rounded sqrt is <= 2r.
+template <typename T>
+__global__ void k_reconstruct(GridArgs<T> ga, const int* __restrict__ offs, const int* __restrict__ perm,
+                              const T* __restrict__ px, const T* __restrict__ py, const T* __restrict__ pz,
+                              T radius, T r2, T s_max, T* __restrict__ sdf, uint8_t* __restrict__ valid) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= ga.nx * ga.ny * ga.nz) return;
+  int i, j, k;
+  unlin3(id, ga.ny, ga.nz, i, j, k);
+  T cx = coord1(i, T(0.5), ga.h[0]), cy = coord1(j, T(0.5), ga.h[1]), cz = coord1(k, T(0.5), ga.h[2]);
+  int x_min = max(0, (int)floor((cx - r2) / ga.h[0]));
+  int x_max = min(ga.nx - 1, (int)ceil((cx + r2) / ga.h[0]));
+  int y_min = max(0, (int)floor((cy - r2) / ga.h[1]));
+  int y_max = min(ga.ny - 1, (int)ceil((cy + r2) / ga.h[1]));
+  int z_min = max(0, (int)floor((cz - r2) / ga.h[2]));
+  int z_max = min(ga.nz - 1, (int)ceil((cz + r2) / ga.h[2]));
+  T dmin = T(0);
+  bool found = false;
+  for (int a = x_min; a <= x_max; a++)
+    for (int b = y_min; b <= y_max; b++)
+      for (int c = z_min; c <= z_max; c++) {
+        int m = morton3D(a, b, c);
+        int s = offs[m], e = offs[m + 1] - 1;
+        for (int t = s; t < e; t++) {
+          int p = perm[t];
+          T dx = cx - px[p], dy = cy - py[p], dz = cz - pz[p];
+          T d2 = dx * dx + dy * dy + dz * dz;
+          if (d2 <= s_max) {
+            if (!found || d2 < dmin) dmin = d2;
+            found = true;
+          }
+        }
+      }
+  T out = T(1e9);
+  if (found) {
+    T v = sqrt(dmin) - radius;
+    out = smin(out, v);
+  }
+  sdf[id] = out;
+  valid[id] = found ? 1 : 0;
+}
+
+// extrapolateFluidSdf iteration (cpu/fluid-simulator.h:124-164): valid cells are not copied.
+template <typename T>
+__global__ void k_extrap_sdf(int nx, int ny, int nz, const T* __restrict__ cur, T* __restrict__ buf,
+                             const uint8_t* __restrict__ val, uint8_t* __restrict__ vbuf) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nx * ny * nz) return;
+  int i, j, k;
+  unlin3(id, ny, nz, i, j, k);
+  if (val[id]) {
+    vbuf[id] = 1;
+    return;
+  }
+  T sum = T(0);
+  int count = 0;
+  const int sx = ny * nz, sy = nz;
+  if (i > 0 && val[id - sx]) { sum += cur[id - sx]; count++; }
+  if (i < nx - 1 && val[id + sx]) { sum += cur[id + sx]; count++; }
+  if (j > 0 && val[id - sy]) { sum += cur[id - sy]; count++; }
+  if (j < ny - 1 && val[id + sy]) { sum += cur[id + sy]; count++; }
+  if (k > 0 && val[id - 1]) { sum += cur[id - 1]; count++; }
+  if (k < nz - 1 && val[id + 1]) { sum += cur[id + 1]; count++; }
+  if (count > 0) {
+    buf[id] = sum / (T)count;
+    vbuf[id] = 1;
+  } else {
+    vbuf[id] = 0;
+  }
+}
+
+// smoothFluidSurface iteration (FluidSim/src/cpu/fluid-simulator.cc:40-71): min(avg6, self).
+template <typename T>
+__global__ void k_smooth(int nx, int ny, int nz, const T* __restrict__ cur, T* __restrict__ buf) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nx * ny * nz) return;
+  int i, j, k;
+  unlin3(id, ny, nz, i, j, k);
+  T sum = T(0);
+  int count = 0;
+  const int sx = ny * nz, sy = nz;
+  if (i > 0) { sum += cur[id - sx]; count++; }
+  if (i < nx - 1) { sum += cur[id + sx]; count++; }
+  if (j > 0) { sum += cur[id - sy]; count++; }
+  if (j < ny - 1) { sum += cur[id + sy]; count++; }
+  if (k > 0) { sum += cur[id - 1]; count++; }
+  if (k < nz - 1) { sum += cur[id + 1]; count++; }
+  T b = sum / (T)count;
+  T c0 = cur[id];
+  if (c0 < b) b = c0;
+  buf[id] = b;
+}
+
+// applyDirichletBoundary (fluid-simulator.cc:113-138) for face grid `axis`.
+template <typename T>
+__global__ void k_dirichlet(int axis, int gx, int gy, int gz, T* __restrict__ G, uint8_t* __restrict__ valid) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= gx * gy * gz) return;
+  int i, j, k;
+  unlin3(id, gy, gz, i, j, k);
+  int c = axis == 0 ? i : (axis == 1 ? j : k);
+  int d = axis == 0 ? gx : (axis == 1 ? gy : gz);
+  if (c == 0 || c == d - 1) {
+    G[id] = T(0);
+    valid[id] = 1;
+  }
+}
+
+// extrapolate<GridType> iteration (cpu/fluid-simulator.h:209-251).
+template <typename T>
+__global__ void k_extrap_vel(int gx, int gy, int gz, const T* __restrict__ g, T* __restrict__ gbuf,
+                             const uint8_t* __restrict__ val, uint8_t* __restrict__ vbuf) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= gx * gy * gz) return;
+  int i, j, k;
+  unlin3(id, gy, gz, i, j, k);
+  if (val[id]) {
+    vbuf[id] = 1;
+    return;
+  }
+  T sum = T(0);
+  int count = 0;
+  const int sx = gy * gz, sy = gz;
+  if (i > 0 && val[id - sx]) { sum += g[id - sx]; count++; }
+  if (i < gx - 1 && val[id + sx]) { sum += g[id + sx]; count++; }
+  if (j > 0 && val[id - sy]) { sum += g[id - sy]; count++; }
+  if (j < gy - 1 && val[id + sy]) { sum += g[id + sy]; count++; }
+  if (k > 0 && val[id - 1]) { sum += g[id - 1]; count++; }
+  if (k < gz - 1 && val[id + 1]) { sum += g[id + 1]; count++; }
+  if (count > 0) {
+    gbuf[id] = sum / (T)count;
+    vbuf[id] = 1;
+  } else {
+    gbuf[id] = T(0);
+    vbuf[id] = 0;
+  }
+}
+
+// applyForce (fluid-simulator.cc:13-18)
+template <typename T>
+__global__ void k_force(int gx, int gy, int gz, T* __restrict__ V, const uint8_t* __restrict__ valid, T gdt) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= gx * gy * gz) return;
+  int i, j, k;
+  unlin3(id, gy, gz, i, j, k);
+  if (valid[id] && j > 0 && j < gy - 1) V[id] = V[id] - gdt;
+}
+
+// Face solid weights of FvmSolver3D::buildSystem (project-solver.cc:77-127): faces between
+// two air cells keep 0; otherwise 1 - fractionInside of the collider SDF at the 4 corners.
+template <typename T>
+__global__ void k_face_weights(int axis, GridArgs<T> ga, int gx, int gy, int gz, const T* __restrict__ fs,
+                               const T* __restrict__ col, T* __restrict__ W) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= gx * gy * gz) return;
+  int i, j, k;
+  unlin3(id, gy, gz, i, j, k);
+  int c = axis == 0 ? i : (axis == 1 ? j : k);
+  int d = axis == 0 ? gx : (axis == 1 ? gy : gz);
+  if (c == 0 || c == d - 1) {
+    W[id] = T(0);
+    return;
+  }
+  int ny = ga.ny, nz = ga.nz;
+  int im = i - (axis == 0), jm = j - (axis == 1), km = k - (axis == 2);
+  if (fs[lin3(im, jm, km, ny, nz)] > T(0) && fs[lin3(i, j, k, ny, nz)] > T(0)) {
+    W[id] = T(0);
+    return;
+  }
+  T hh = ga.h[0];
+  T p0 = coord1(i, axis == 0 ? T(0) : T(0.5), ga.h[0]);
+  T p1 = coord1(j, axis == 1 ? T(0) : T(0.5), ga.h[1]);
+  T p2 = coord1(k, axis == 2 ? T(0) : T(0.5), ga.h[2]);
+  // corner offsets Vec3d(a,b,c) * 0.5 * h, in the argument order of each call
+  T s0, s1, s2, s3;
+  const T hp = T(1) * T(0.5) * hh, hm = T(-1) * T(0.5) * hh, z0 = T(0) * T(0.5) * hh;
+  if (axis == 0) {  // bu(0,1,-1) bd(0,-1,-1) fd(0,-1,1) fu(0,1,1)
+    s0 = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p0 + z0, p1 + hp, p2 + hm);
+    s1 = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p0 + z0, p1 + hm, p2 + hm);
+    s2 = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p0 + z0, p1 + hm, p2 + hp);
+    s3 = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p0 + z0, p1 + hp, p2 + hp);
+  } else if (axis == 1) {  // lb(-1,0,-1) rb(1,0,-1) rf(1,0,1) lf(-1,0,1)
+    s0 = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p0 + hm, p1 + z0, p2 + hm);
+    s1 = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p0 + hp, p1 + z0, p2 + hm);
+    s2 = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p0 + hp, p1 + z0, p2 + hp);
+    s3 = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p0 + hm, p1 + z0, p2 + hp);
+  } else {  // ld(-1,-1,0) lu(-1,1,0) ru(1,1,0) rd(1,-1,0)
+    s0 = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p0 + hm, p1 + hm, p2 + z0);
+    s1 = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p0 + hm, p1 + hp, p2 + z0);
+    s2 = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p0 + hp, p1 + hp, p2 + z0);
+    s3 = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p0 + hp, p1 + hm, p2 + z0);
+  }
+  W[id] = T(1) - fraction_inside(s0, s1, s2, s3);
+}
+
+// Cell loop of FvmSolver3D::buildSystem (project-solver.cc:129-226): active marking and the
+// 7-point ghost-fluid Poisson row. Neighbour order left, right, down, up, back, front.
+template <typename T>
+__global__ void k_assemble(GridArgs<T> ga, T dt, const T* __restrict__ fs, const T* __restrict__ uW,
+                           const T* __restrict__ vW, const T* __restrict__ wW, const T* __restrict__ U,
+                           const T* __restrict__ V, const T* __restrict__ Wv, T* __restrict__ Adiag,
+                           T* __restrict__ aL, T* __restrict__ aR, T* __restrict__ aU, T* __restrict__ aD,
+                           T* __restrict__ aF, T* __restrict__ aB, T* __restrict__ rhs, uint8_t* __restrict__ active) {
+  const int nx = ga.nx, ny = ga.ny, nz = ga.nz;
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nx * ny * nz) return;
+  int i, j, k;
+  unlin3(id, ny, nz, i, j, k);
+  const int uL = lin3(i, j, k, ny, nz), uRi = lin3(i + 1, j, k, ny, nz);
+  const int vD = lin3(i, j, k, ny + 1, nz), vUi = lin3(i, j + 1, k, ny + 1, nz);
+  const int wB = lin3(i, j, k, ny, nz + 1), wFi = lin3(i, j, k + 1, ny, nz + 1);
+  T ad = T(0), rh = T(0), l = T(0), r = T(0), u = T(0), d = T(0), f = T(0), b = T(0);
+  bool act = false;
+  if (!(uW[uL] == T(0) && uW[uRi] == T(0) && vW[vD] == T(0) && vW[vUi] == T(0) && wW[wB] == T(0) && wW[wFi] == T(0)) &&
+      !(fs[id] > T(0))) {
+    act = true;
+    T sd = fs[id];
+    T factor = dt / ga.h[0];
+    const int sx = ny * nz, sy = nz;
+    if (i > 0) {
+      T fn = fs[id - sx];
+      if (fn > T(0)) {
+        T theta = smin(fn / (fn - sd), T(0.99));
+        ad += uW[uL] * factor / (T(1) - theta);
+      } else {
+        ad += uW[uL] * factor;
+        l -= uW[uL] * factor;
+      }
+      rh += uW[uL] * U[uL];
+    }
+    if (i < nx - 1) {
+      T fn = fs[id + sx];
+      if (fn > T(0)) {
+        T theta = smax(sd / (sd - fn), T(0.01));
+        ad += uW[uRi] * factor / theta;
+      } else {
+        ad += uW[uRi] * factor;
+        r -= uW[uRi] * factor;
+      }
+      rh -= uW[uRi] * U[uRi];
+    }
+    if (j > 0) {
+      T fn = fs[id - sy];
+      if (fn > T(0)) {
+        T theta = smin(fn / (fn - sd), T(0.99));
+        ad += vW[vD] * factor / (T(1) - theta);
+      } else {
+        ad += vW[vD] * factor;
+        d -= vW[vD] * factor;
+      }
+      rh += vW[vD] * V[vD];
+    }
+    if (j < nx - 1) {  // the reference bounds "up" by fluidSdf.width() (project-solver.cc:186)
+      T fn = fs[id + sy];
+      if (fn > T(0)) {
+        T theta = smax(sd / (sd - fn), T(0.01));
+        ad += vW[vUi] * factor / theta;
+      } else {
+        ad += vW[vUi] * factor;
+        u -= vW[vUi] * factor;
+      }
+      rh -= vW[vUi] * V[vUi];
+    }
+    if (k > 0) {
+      T fn = fs[id - 1];
+      if (fn > T(0)) {
+        T theta = smin(fn / (fn - sd), T(0.99));
+        ad += wW[wB] * factor / (T(1) - theta);
+      } else {
+        ad += wW[wB] * factor;
+        b -= wW[wB] * factor;
+      }
+      rh += wW[wB] * Wv[wB];
+    }
+    if (k < nz - 1) {
+      T fn = fs[id + 1];
+      if (fn > T(0)) {
+        T theta = smax(sd / (sd - fn), T(0.01));
+        ad += wW[wFi] * factor / theta;
+      } else {
+        ad += wW[wFi] * factor;
+        f -= wW[wFi] * factor;
+      }
+      rh -= wW[wFi] * Wv[wFi];
+    }
+  }
+  Adiag[id] = ad;
+  aL[id] = l;
+  aR[id] = r;
+  aU[id] = u;
+  aD[id] = d;
+  aF[id] = f;
+  aB[id] = b;
+  rhs[id] = rh;
+  active[id] = act ? 1 : 0;
+}
+
+// FvmSolver3D::project (project-solver.cc:348-421) for face grid `axis`.
+template <typename T>
+__global__ void k_project(int axis, GridArgs<T> ga, int gx, int gy, int gz, T dt, const T* __restrict__ fs,
+                          const T* __restrict__ pg, const T* __restrict__ Wt, T* __restrict__ G) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= gx * gy * gz) return;
+  if (Wt[id] <= T(0)) return;
+  int i, j, k;
+  unlin3(id, gy, gz, i, j, k);
+  int c = axis == 0 ? i : (axis == 1 ? j : k);
+  int d = axis == 0 ? gx : (axis == 1 ? gy : gz);
+  if (c == 0 || c == d - 1) {
+    G[id] = T(0);
+    return;
+  }
+  int ny = ga.ny, nz = ga.nz;
+  int cm = lin3(i - (axis == 0), j - (axis == 1), k - (axis == 2), ny, nz);
+  int cp = lin3(i, j, k, ny, nz);
+  T hh = ga.h[0];
+  T s0 = fs[cm], s1 = fs[cp];
+  if (s0 >= T(0) && s1 >= T(0)) return;
+  if (s0 < T(0) && s1 < T(0)) {
+    G[id] = G[id] - (pg[cp] - pg[cm]) * dt / hh;
+    return;
+  }
+  if (s0 < T(0)) {
+    T theta = smax(s0 / (s0 - s1), T(0.01));
+    G[id] = G[id] + pg[cm] * dt / hh / theta;
+  } else {
+    T theta = smin(s0 / (s0 - s1), T(0.99));
+    G[id] = G[id] - pg[cp] * dt / hh / (T(1) - theta);
+  }
+}
+
+// applyCollider (fluid-simulator.cc:76-111) for face grid `axis`.
+template <typename T>
+__global__ void k_collider(int axis, GridArgs<T> ga, int gx, int gy, int gz, const T* __restrict__ col,
+                           T* __restrict__ G) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= gx * gy * gz) return;
+  int i, j, k;
+  unlin3(id, gy, gz, i, j, k);
+  int c = axis == 0 ? i : (axis == 1 ? j : k);
+  int d = axis == 0 ? gx : (axis == 1 ? gy : gz);
+  if (c == 0 || c == d - 1) {
+    G[id] = T(0);
+    return;
+  }
+  T p0 = coord1(i, axis == 0 ? T(0) : T(0.5), ga.h[0]);
+  T p1 = coord1(j, axis == 1 ? T(0) : T(0.5), ga.h[1]);
+  T p2 = coord1(k, axis == 2 ? T(0) : T(0.5), ga.h[2]);
+  if (sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p0, p1, p2) < T(0)) {
+    V3<T> n = gnormalize(sdf_grad(col, ga.nx, ga.ny, ga.nz, ga.h, p0, p1, p2));
+    T nc = axis == 0 ? n.x : (axis == 1 ? n.y : n.z);
+    G[id] = G[id] - G[id] * nc;
+  }
+}
+
+// max |u| over a face grid (CFL, fluid-simulator.cc:184-203): fl(h/|u|) is monotone in |u|,
+// so min over faces of h/|u| is h / max|u|.
+template <typename T>
+__global__ void k_maxabs(int n, const T* __restrict__ g, unsigned long long* __restrict__ out) {
+  __shared__ double sm[TPB];
+  double m = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double a = fabs((double)g[i]);
+    m = a > m ? a : m;
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = sm[threadIdx.x + s] > sm[threadIdx.x] ? sm[threadIdx.x + s] : sm[threadIdx.x];
+    __syncthreads();
+  }
+  // non-negative doubles order like their bit patterns
+  if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(sm[0]));
+}
+
+template <typename T>
+__global__ void k_to_double(int64_t n, const T* __restrict__ a, double* __restrict__ b) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+template <typename T>
+__global__ void k_from_double(int64_t n, const double* __restrict__ a, T* __restrict__ b) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (T)a[i];
+}
+
+template <typename T>
+__global__ void k_split_aos(int64_t n, const double* __restrict__ aos, T* __restrict__ x, T* __restrict__ y,
+                            T* __restrict__ z) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = (T)aos[3 * i];
+  y[i] = (T)aos[3 * i + 1];
+  z[i] = (T)aos[3 * i + 2];
+}
+template <typename T>
+__global__ void k_join_aos(int64_t n, const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z,
+                           double* __restrict__ aos) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  aos[3 * i] = (double)x[i];
+  aos[3 * i + 1] = (double)y[i];
+  aos[3 * i + 2] = (double)z[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// host side
+
+template <typename T>
+Grids3<T> grids_of(const fs_ctx* c, const DBuf* bufs) {
+  Grids3<T> G;
+  for (int a = 0; a < 3; a++) {
+    G.g[a] = bufs[a].as<T>();
+    for (int d = 0; d < 3; d++) G.d[a][d] = c->fdim[a][d];
+  }
+  return G;
+}
+
+int ensure_bins(fs_ctx* c, size_t nbins) {
+  if (nbins <= c->bins_cap && c->counts.p) return FS_OK;
+  fs_free(c->counts);
+  fs_free(c->offs);
+  fs_free(c->scan_tmp);
+  FS_TRY(fs_alloc(c, c->counts, (nbins + 1) * sizeof(int)));
+  FS_TRY(fs_alloc(c, c->offs, (nbins + 1) * sizeof(int)));
+  size_t tmp = 0;
+  FS_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (int*)nullptr, (int*)nullptr, (int)(nbins + 1), c->stream));
+  FS_TRY(fs_alloc(c, c->scan_tmp, tmp + 256));
+  c->scan_tmp_bytes = tmp + 256;
+  c->bins_cap = nbins;
+  return FS_OK;
+}
+
+// keys must already be in c->keys; leaves offs[nbins+1] and perm.
+int counting_sort(fs_ctx* c, int nbins) {
+  FS_TRY(ensure_bins(c, (size_t)nbins));
+  int n = (int)c->np;
+  FS_HIP(c, hipMemsetAsync(c->counts.p, 0, (size_t)(nbins + 1) * sizeof(int), c->stream));
+  if (n > 0) k_count<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->counts.as<int>(), c->slots.as<int>());
+  size_t tmp = c->scan_tmp_bytes;
+  FS_HIP(c, hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.p, tmp, c->counts.as<int>(), c->offs.as<int>(), nbins + 1,
+                                             c->stream));
+  if (n > 0) {
+    k_scatter<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->slots.as<int>(), c->offs.as<int>(),
+                                                       c->perm.as<int>());
+    k_sort_bins<<<fs_blocks(nbins, TPB), TPB, 0, c->stream>>>(nbins, c->offs.as<int>(), c->perm.as<int>());
+  }
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+double smax_for(double r2) {
+  double s = r2 * r2;
+  while (std::sqrt(std::nextafter(s, INFINITY)) <= r2) s = std::nextafter(s, INFINITY);
+  while (s > 0 && std::sqrt(s) > r2) s = std::nextafter(s, -INFINITY);
+  return s;
+}
+float smax_for_f(float r2) {
+  float s = r2 * r2;
+  while (std::sqrt(std::nextafter(s, INFINITY)) <= r2) s = std::nextafter(s, INFINITY);
+  while (s > 0 && std::sqrt(s) > r2) s = std::nextafter(s, -INFINITY);
+  return s;
+}
+
+template <typename T>
+int advect_t(fs_ctx* c, double dt) {
+  int n = (int)c->np;
+  if (n == 0) return FS_OK;
+  GridArgs<T> ga = grid_args<T>(c);
+  k_advect<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, grids_of<T>(c, c->g), c->collider.as<T>(),
+                                                        c->pos[0].as<T>(), c->pos[1].as<T>(), c->pos[2].as<T>(),
+                                                        c->vel[0].as<T>(), c->vel[1].as<T>(), c->vel[2].as<T>(), (T)dt);
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+template <typename T>
+int reconstruct_t(fs_ctx* c, double radius) {
+  GridArgs<T> ga = grid_args<T>(c);
+  int n = (int)c->np;
+  int nb = morton3D(c->nx - 1, c->ny - 1, c->nz - 1) + 1;
+  if (n > 0)
+    k_keys_morton<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
+                                                               c->pos[2].as<T>(), c->keys.as<int>());
+  FS_TRY(counting_sort(c, nb));
+  T rad = (T)radius;
+  T r2 = T(2) * rad;
+  T smax;
+  if (sizeof(T) == 8) smax = (T)smax_for((double)r2);
+  else smax = (T)smax_for_f((float)r2);
+  k_reconstruct<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(
+      ga, c->offs.as<int>(), c->perm.as<int>(), c->pos[0].as<T>(), c->pos[1].as<T>(), c->pos[2].as<T>(), rad, r2,
+      smax, c->fsdf.as<T>(), c->sdfvalid.as<uint8_t>());
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+template <typename T>
+int extrap_sdf_t(fs_ctx* c, int iters) {
+  for (int it = 0; it < iters; it++) {
+    k_extrap_sdf<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->nx, c->ny, c->nz, c->fsdf.as<T>(),
+                                                                      c->fsdfbuf.as<T>(), c->sdfvalid.as<uint8_t>(),
+                                                                      c->sdfvalidbuf.as<uint8_t>());
+    std::swap(c->fsdf, c->fsdfbuf);
+    std::swap(c->sdfvalid, c->sdfvalidbuf);
+  }
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+template <typename T>
+int smooth_t(fs_ctx* c, int iters) {
+  for (int it = 0; it < iters; it++) {
+    k_smooth<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->nx, c->ny, c->nz, c->fsdf.as<T>(),
+                                                                  c->fsdfbuf.as<T>());
+    std::swap(c->fsdf, c->fsdfbuf);
+  }
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+template <typename T>
+int p2g_t(fs_ctx* c) {
+  GridArgs<T> ga = grid_args<T>(c);
+  int n = (int)c->np;
+  int nb = (c->nx + 1) * (c->ny + 1) * (c->nz + 1);
+  for (int a = 0; a < 3; a++) {
+    if (n > 0)
+      k_keys_nearest<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, a, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
+                                                                  c->pos[2].as<T>(), c->keys.as<int>());
+    FS_TRY(counting_sort(c, nb));
+    int gx = c->fdim[a][0], gy = c->fdim[a][1], gz = c->fdim[a][2];
+    k_p2g_gather<T><<<fs_blocks(c->n_face[a], TPB), TPB, 0, c->stream>>>(
+        a, ga, gx, gy, gz, c->offs.as<int>(), c->perm.as<int>(), c->pos[0].as<T>(), c->pos[1].as<T>(),
+        c->pos[2].as<T>(), c->vel[a].as<T>(), c->g[a].as<T>(), c->gw[a].as<T>(), c->gvalid[a].as<uint8_t>());
+    FS_HIP(c, hipGetLastError());
+    if (c->cfg.advection == FS_ADVECT_FLIP)
+      FS_HIP(c, hipMemcpyAsync(c->glast[a].p, c->g[a].p, (size_t)c->n_face[a] * sizeof(T), hipMemcpyDeviceToDevice,
+                               c->stream));
+  }
+  return FS_OK;
+}
+
+template <typename T>
+int dirichlet_t(fs_ctx* c) {
+  for (int a = 0; a < 3; a++)
+    k_dirichlet<T><<<fs_blocks(c->n_face[a], TPB), TPB, 0, c->stream>>>(a, c->fdim[a][0], c->fdim[a][1], c->fdim[a][2],
+                                                                        c->g[a].as<T>(), c->gvalid[a].as<uint8_t>());
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+template <typename T>
+int extrap_vel_t(fs_ctx* c, int iters) {
+  for (int a = 0; a < 3; a++)
+    for (int it = 0; it < iters; it++) {
+      k_extrap_vel<T><<<fs_blocks(c->n_face[a], TPB), TPB, 0, c->stream>>>(
+          c->fdim[a][0], c->fdim[a][1], c->fdim[a][2], c->g[a].as<T>(), c->gbuf[a].as<T>(),
+          c->gvalid[a].as<uint8_t>(), c->gvalidbuf[a].as<uint8_t>());
+      std::swap(c->g[a], c->gbuf[a]);
+      std::swap(c->gvalid[a], c->gvalidbuf[a]);
+    }
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+template <typename T>
+int force_t(fs_ctx* c, double dt) {
+  T gdt = (T)(c->cfg.gravity * dt);
+  k_force<T><<<fs_blocks(c->n_face[1], TPB), TPB, 0, c->stream>>>(c->fdim[1][0], c->fdim[1][1], c->fdim[1][2],
+                                                                  c->g[1].as<T>(), c->gvalid[1].as<uint8_t>(), gdt);
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+template <typename T>
+int build_t(fs_ctx* c, double dt) {
+  GridArgs<T> ga = grid_args<T>(c);
+  for (int a = 0; a < 3; a++)
+    k_face_weights<T><<<fs_blocks(c->n_face[a], TPB), TPB, 0, c->stream>>>(
+        a, ga, c->fdim[a][0], c->fdim[a][1], c->fdim[a][2], c->fsdf.as<T>(), c->collider.as<T>(),
+        c->gweights[a].as<T>());
+  // Aneighbour order Left, Right, Up, Down, Front, Back (project-solver.cc:7-14)
+  k_assemble<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(
+      ga, (T)dt, c->fsdf.as<T>(), c->gweights[0].as<T>(), c->gweights[1].as<T>(), c->gweights[2].as<T>(),
+      c->g[0].as<T>(), c->g[1].as<T>(), c->g[2].as<T>(), c->adiag.as<T>(), c->an[0].as<T>(), c->an[1].as<T>(),
+      c->an[2].as<T>(), c->an[3].as<T>(), c->an[4].as<T>(), c->an[5].as<T>(), c->rhs.as<T>(),
+      c->active.as<uint8_t>());
+  FS_HIP(c, hipGetLastError());
+  c->pcg.built = false;
+  return FS_OK;
+}
+
+template <typename T>
+int project_t(fs_ctx* c, double dt) {
+  GridArgs<T> ga = grid_args<T>(c);
+  for (int a = 0; a < 3; a++)
+    k_project<T><<<fs_blocks(c->n_face[a], TPB), TPB, 0, c->stream>>>(a, ga, c->fdim[a][0], c->fdim[a][1],
+                                                                      c->fdim[a][2], (T)dt, c->fsdf.as<T>(),
+                                                                      c->pg.as<T>(), c->gweights[a].as<T>(),
+                                                                      c->g[a].as<T>());
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+template <typename T>
+int collider_t(fs_ctx* c) {
+  GridArgs<T> ga = grid_args<T>(c);
+  for (int a = 0; a < 3; a++)
+    k_collider<T><<<fs_blocks(c->n_face[a], TPB), TPB, 0, c->stream>>>(a, ga, c->fdim[a][0], c->fdim[a][1],
+                                                                       c->fdim[a][2], c->collider.as<T>(),
+                                                                       c->g[a].as<T>());
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+template <typename T>
+int g2p_t(fs_ctx* c) {
+  int n = (int)c->np;
+  if (n == 0) return FS_OK;
+  GridArgs<T> ga = grid_args<T>(c);
+  if (c->cfg.advection == FS_ADVECT_FLIP)
+    k_g2p<T, true><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(
+        n, ga, grids_of<T>(c, c->g), grids_of<T>(c, c->glast), c->pos[0].as<T>(), c->pos[1].as<T>(),
+        c->pos[2].as<T>(), c->vel[0].as<T>(), c->vel[1].as<T>(), c->vel[2].as<T>(), (T)c->cfg.flip_alpha);
+  else
+    k_g2p<T, false><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(
+        n, ga, grids_of<T>(c, c->g), grids_of<T>(c, c->g), c->pos[0].as<T>(), c->pos[1].as<T>(), c->pos[2].as<T>(),
+        c->vel[0].as<T>(), c->vel[1].as<T>(), c->vel[2].as<T>(), (T)c->cfg.flip_alpha);
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+template <typename T>
+int cfl_t(fs_ctx* c, double* out) {
+  FS_HIP(c, hipMemsetAsync(c->red.p, 0, sizeof(unsigned long long), c->stream));
+  for (int a = 0; a < 3; a++)
+    k_maxabs<T><<<256, TPB, 0, c->stream>>>(c->n_face[a], c->g[a].as<T>(), c->red.as<unsigned long long>());
+  unsigned long long bits = 0;
+  FS_HIP(c, hipMemcpyAsync(&bits, c->red.p, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+  FS_HIP(c, hipStreamSynchronize(c->stream));
+  double m;
+  std::memcpy(&m, &bits, sizeof(m));
+  double hh = c->h[0];
+  double cfl = hh / 1e-6;
+  if (m > 0.0) {
+    double v = hh / m;
+    cfl = v < cfl ? v : cfl;
+  }
+  *out = c->cfg.cfl_factor * (1e-3 < cfl ? cfl : 1e-3);
+  return FS_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+#define FS_DISPATCH(c, fn, ...) ((c)->mixed ? fn<float>(c, ##__VA_ARGS__) : fn<double>(c, ##__VA_ARGS__))
+
+int fs_stage_clear(fs_ctx* c) {
+  // clear() (fluid-simulator.cc:20-37)
+  for (int a = 0; a < 3; a++) {
+    FS_TRY(fs_zero(c, c->gvalid[a]));
+    FS_TRY(fs_zero(c, c->gvalidbuf[a]));
+    FS_TRY(fs_zero(c, c->g[a]));
+    FS_TRY(fs_zero(c, c->gbuf[a]));
+    FS_TRY(fs_zero(c, c->gw[a]));
+  }
+  FS_TRY(fs_zero(c, c->pg));
+  return FS_OK;
+}
+int fs_stage_advect(fs_ctx* c, double dt) { return FS_DISPATCH(c, advect_t, dt); }
+int fs_stage_reconstruct(fs_ctx* c, double radius) { return FS_DISPATCH(c, reconstruct_t, radius); }
+int fs_stage_extrapolate_sdf(fs_ctx* c, int iters) { return FS_DISPATCH(c, extrap_sdf_t, iters); }
+int fs_stage_smooth(fs_ctx* c, int iters) { return FS_DISPATCH(c, smooth_t, iters); }
+int fs_stage_p2g(fs_ctx* c) { return FS_DISPATCH(c, p2g_t); }
+int fs_stage_dirichlet(fs_ctx* c) { return FS_DISPATCH(c, dirichlet_t); }
+int fs_stage_extrapolate_velocity(fs_ctx* c, int iters) { return FS_DISPATCH(c, extrap_vel_t, iters); }
+int fs_stage_force(fs_ctx* c, double dt) { return FS_DISPATCH(c, force_t, dt); }
+int fs_stage_build(fs_ctx* c, double dt) { return FS_DISPATCH(c, build_t, dt); }
+int fs_stage_project(fs_ctx* c, double dt) { return FS_DISPATCH(c, project_t, dt); }
+int fs_stage_collider(fs_ctx* c) { return FS_DISPATCH(c, collider_t); }
+int fs_stage_g2p(fs_ctx* c) { return FS_DISPATCH(c, g2p_t); }
+int fs_stage_cfl(fs_ctx* c, double* out) { return FS_DISPATCH(c, cfl_t, out); }
+
+int fs_upload_real(fs_ctx* c, DBuf& dst, const double* src, size_t n) {
+  if (!c->mixed) {
+    FS_HIP(c, hipMemcpyAsync(dst.p, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return FS_OK;
+  }
+  DBuf tmp;
+  FS_TRY(fs_alloc(c, tmp, n * sizeof(double)));
+  FS_HIP(c, hipMemcpyAsync(tmp.p, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  k_from_double<float><<<fs_blocks((int64_t)n, TPB), TPB, 0, c->stream>>>((int64_t)n, tmp.as<double>(), dst.as<float>());
+  FS_HIP(c, hipStreamSynchronize(c->stream));
+  fs_free(tmp);
+  return FS_OK;
+}
+
+int fs_download_real(fs_ctx* c, const DBuf& src, double* dst, size_t n) {
+  if (!c->mixed) {
+    FS_HIP(c, hipMemcpyAsync(dst, src.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    return FS_OK;
+  }
+  DBuf tmp;
+  FS_TRY(fs_alloc(c, tmp, n * sizeof(double)));
+  k_to_double<float><<<fs_blocks((int64_t)n, TPB), TPB, 0, c->stream>>>((int64_t)n, src.as<float>(), tmp.as<double>());
+  FS_HIP(c, hipMemcpyAsync(dst, tmp.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  FS_HIP(c, hipStreamSynchronize(c->stream));
+  fs_free(tmp);
+  return FS_OK;
+}
+
+int fs_particles_upload(fs_ctx* c, const double* pos, const double* vel) {
+  int64_t n = c->np;
+  if (n == 0) return FS_OK;
+  for (int which = 0; which < 2; which++) {
+    const double* src = which == 0 ? pos : vel;
+    DBuf* dst = which == 0 ? c->pos : c->vel;
+    if (!src) {
+      for (int d = 0; d < 3; d++) FS_TRY(fs_zero(c, dst[d]));
+      continue;
+    }
+    FS_HIP(c, hipMemcpyAsync(c->staging.p, src, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (c->mixed)
+      k_split_aos<float><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->staging.as<double>(), dst[0].as<float>(),
+                                                                   dst[1].as<float>(), dst[2].as<float>());
+    else
+      k_split_aos<double><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->staging.as<double>(), dst[0].as<double>(),
+                                                                    dst[1].as<double>(), dst[2].as<double>());
+    FS_HIP(c, hipGetLastError());
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return FS_OK;
+}
+
+int fs_particles_download(fs_ctx* c, double* pos, double* vel) {
+  int64_t n = c->np;
+  if (n == 0) return FS_OK;
+  for (int which = 0; which < 2; which++) {
+    double* dst = which == 0 ? pos : vel;
+    DBuf* src = which == 0 ? c->pos : c->vel;
+    if (!dst) continue;
+    if (c->mixed)
+      k_join_aos<float><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, src[0].as<float>(), src[1].as<float>(),
+                                                                  src[2].as<float>(), c->staging.as<double>());
+    else
+      k_join_aos<double><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, src[0].as<double>(), src[1].as<double>(),
+                                                                   src[2].as<double>(), c->staging.as<double>());
+    FS_HIP(c, hipGetLastError());
+    FS_HIP(c, hipMemcpyAsync(dst, c->staging.p, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return FS_OK;
+}

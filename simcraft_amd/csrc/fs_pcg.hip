@@ -1,0 +1,970 @@
+// fs_pcg.hip - MIC(0)-preconditioned CG pressure solve for gfx950
+// (CgSolver3D::solve, FluidSim/src/cpu/project-solver.cc:308-346, with
+// ModifiedIncompleteCholesky3D::precond :230-306 and applyCompressedMatrix :16-39).
+//
+// Layout ("column bricks"): the active cells are covered by 8x8 columns in (i,j) that span
+// all k. A live brick (one with an active cell) owns S = nz+14 slots of 64 lanes; cell
+// (i,j,k) sits in lane a*8+b (a=i%8, b=j%8) of slot k+a+b. The skew a+b turns the triangular
+// sweeps into one wavefront per brick: at step s every lane of the wave handles slot s, and
+// its (i-1) and (j-1) neighbours were finished by lanes a-1 / b-1 at step s-1, so they arrive
+// by a lane shuffle. Bricks hand their boundary rows to the next brick through global memory
+// with release/acquire progress counters; bricks are scheduled by ticket in dependency order,
+// so every wait is on a brick that is already running. Each slot reads/writes one contiguous
+// 512-byte row per array.
+//
+// Exactness: every cell does the reference's arithmetic in the reference's order. The MIC
+// sweep coefficients are the products the reference forms first (AL(c)*E(i-1) and
+// AL(i+1)*E(c)); inactive neighbours contribute +0 * +0. Dot products use one fixed tree.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fs_ctx.h"
+
+namespace {
+
+constexpr int CH = 4096;  // elements per chunk = 256 threads x 16
+constexpr int NB_SELF = 1, NB_L = 2, NB_D = 4, NB_B = 8, NB_R = 16, NB_U = 32, NB_F = 64;
+// factor-sweep flags
+constexpr int TF_DO = 1, TF_GL = 2, TF_L1 = 4, TF_L2 = 8, TF_GD = 16, TF_D1 = 32, TF_D2 = 64, TF_GB = 128,
+              TF_B1 = 256, TF_B2 = 512;
+constexpr int PUB = 4;                  // slots between progress publications
+constexpr long SPIN_LIMIT = 1000000;    // bounded wait (~1 s), then the solve reports a device error
+enum { STOP_RUN = 0, STOP_CONVERGED = 1, STOP_ABORT = 2 };
+enum { DC_SPMV = 0, DC_FWD = 1, DC_BWD = 2, DC_INIT = 3, DC_FACTOR = 4, DC_RES = 5 };
+
+struct Lay {
+  int nx, ny, nz, nbi, nbj, S, nlive;
+  int64_t ne;
+  const int* live;   // brick (I*nbj+J) -> live index or -1
+  const int2* bij;   // live index -> (I, J)
+};
+
+__device__ inline int64_t eidx(const Lay& L, int B, int s, int lane) { return ((int64_t)B * L.S + s) * 64 + lane; }
+
+// neighbour element of a valid cell; dir 0:-x 1:+x 2:-y 3:+y. -1 when outside the grid or in a
+// brick that holds no active cell.
+__device__ inline int64_t nb_elem(const Lay& L, int B, int s, int a, int b, int dir) {
+  int2 ij = L.bij[B];
+  if (dir == 0) {
+    if (a > 0) return eidx(L, B, s - 1, (a - 1) * 8 + b);
+    if (ij.x == 0) return -1;
+    int B2 = L.live[(ij.x - 1) * L.nbj + ij.y];
+    return B2 < 0 ? -1 : eidx(L, B2, s + 7, 56 + b);
+  } else if (dir == 1) {
+    if (8 * ij.x + a + 1 >= L.nx) return -1;
+    if (a < 7) return eidx(L, B, s + 1, (a + 1) * 8 + b);
+    int B2 = L.live[(ij.x + 1) * L.nbj + ij.y];
+    return B2 < 0 ? -1 : eidx(L, B2, s - 7, b);
+  } else if (dir == 2) {
+    if (b > 0) return eidx(L, B, s - 1, a * 8 + b - 1);
+    if (ij.y == 0) return -1;
+    int B2 = L.live[ij.x * L.nbj + ij.y - 1];
+    return B2 < 0 ? -1 : eidx(L, B2, s + 7, a * 8 + 7);
+  } else {
+    if (8 * ij.y + b + 1 >= L.ny) return -1;
+    if (b < 7) return eidx(L, B, s + 1, a * 8 + b + 1);
+    int B2 = L.live[ij.x * L.nbj + ij.y + 1];
+    return B2 < 0 ? -1 : eidx(L, B2, s - 7, a * 8);
+  }
+}
+
+__device__ inline bool decode(const Lay& L, int64_t e, int& B, int& s, int& a, int& b, int& i, int& j, int& k) {
+  int lane = (int)(e & 63);
+  int64_t t = e >> 6;
+  s = (int)(t % L.S);
+  B = (int)(t / L.S);
+  a = lane >> 3;
+  b = lane & 7;
+  int2 ij = L.bij[B];
+  i = 8 * ij.x + a;
+  j = 8 * ij.y + b;
+  k = s - a - b;
+  return i < L.nx && j < L.ny && k >= 0 && k < L.nz;
+}
+
+// ---- fixed-tree reductions -------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    double o = __shfl_xor(v, off, 64);
+    v = (v < o) ? o : v;
+  }
+  return v;
+}
+// 256-thread block: per-thread values -> (w0 + w1) + (w2 + w3) of wave butterflies.
+__device__ inline double block_sum256(double v, double* sm) {
+  double w = wave_sum(v);
+  int wid = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sm[wid] = w;
+  __syncthreads();
+  double r = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+  __syncthreads();
+  return r;
+}
+__device__ inline double block_max256(double v, double* sm) {
+  double w = wave_max(v);
+  int wid = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sm[wid] = w;
+  __syncthreads();
+  double r = sm[0];
+  for (int q = 1; q < 4; q++) r = (r < sm[q]) ? sm[q] : r;
+  __syncthreads();
+  return r;
+}
+// Fixed tree over n partials with a 256-thread block: thread t sums p[t + 256m] for m
+// ascending, then block_sum256.
+__device__ inline double tree256(const double* p, int n, double* sm) {
+  double a = 0.0;
+  for (int m = threadIdx.x; m < n; m += 256) a += p[m];
+  return block_sum256(a, sm);
+}
+// The same tree evaluated by one 64-lane wave (lane l plays threads l, l+64, l+128, l+192).
+__device__ inline double tree256_wave(const double* p, int n) {
+  double w[4];
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    double a = 0.0;
+    for (int m = threadIdx.x + 64 * v; m < n; m += 256) a += p[m];
+    w[v] = wave_sum(a);
+  }
+  return (w[0] + w[1]) + (w[2] + w[3]);
+}
+__device__ inline double max_wave(const double* p, int n) {
+  double m = 0.0;
+  for (int q = threadIdx.x; q < n; q += 64) m = (m < p[q]) ? p[q] : m;
+  return wave_max(m);
+}
+
+// Last-block election: returns true in exactly one block (all threads), after every block
+// has published its partial.
+__device__ inline bool last_block(uint32_t* ctr, uint32_t nblocks, int nthreads) {
+  __shared__ int is_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    uint32_t old = atomicAdd(ctr, 1u);
+    is_last = (old == nblocks - 1) ? 1 : 0;
+    if (is_last) {
+      __threadfence();
+      atomicExch(ctr, 0u);
+    }
+  }
+  __syncthreads();
+  (void)nthreads;
+  if (is_last) __threadfence();
+  return is_last != 0;
+}
+
+// ---- bounded progress wait -------------------------------------------------------------------
+__device__ inline uint64_t wait_progress(const uint64_t* prog, uint64_t target, uint64_t seen, PcgScalars* sc) {
+  if (seen >= target) return seen;
+  uint64_t v = __hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  long spins = 0;
+  while (v < target) {
+    __builtin_amdgcn_s_sleep(1);
+    v = __hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if ((++spins & 1023) == 0) {
+      int to = __hip_atomic_load(&sc->wait_timeouts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (to != 0 || spins > SPIN_LIMIT) {
+        if (threadIdx.x == 0) atomicAdd(&sc->wait_timeouts, 1);
+        return target;  // give up: results are invalid and the solve reports FS_DEVICE_ERROR
+      }
+    }
+  }
+  return v;
+}
+__device__ inline void publish(uint64_t* prog, uint64_t v) {
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(prog, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int take_ticket(uint64_t* ctr, uint64_t base, const int* order, int n) {
+  __shared__ int sB;
+  if (threadIdx.x == 0) {
+    uint64_t t = atomicAdd((unsigned long long*)ctr, 1ull) - base;
+    // a ticket outside [0, n) can only follow a failed launch; clamp it so every access
+    // stays in bounds (the dependency waits then time out and the solve reports an error)
+    sB = order[t < (uint64_t)n ? t : (uint64_t)(n - 1)];
+  }
+  __syncthreads();
+  return sB;
+}
+
+// ---- setup kernels ------------------------------------------------------------------------------
+__global__ void k_brick_flags(int nx, int ny, int nz, const uint8_t* __restrict__ active, int* __restrict__ flag,
+                              int nbj) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= nx * ny) return;
+  int i = row / ny, j = row % ny;
+  const uint8_t* a = active + (size_t)row * nz;
+  int any = 0;
+  for (int k = 0; k < nz; k++) any |= a[k];
+  if (any) atomicOr(&flag[(i / 8) * nbj + (j / 8)], 1);
+}
+
+// Pack the dense system into the brick layout; neighbour-activity masks; factor flags and
+// tau numerators (tau*A(c))*A(diagonal neighbour) of ModifiedIncompleteCholesky3D::precond.
+__global__ void k_pack(Lay L, double tau, const double* __restrict__ Adiag, const double* __restrict__ AL,
+                       const double* __restrict__ AD, const double* __restrict__ AB, const double* __restrict__ rhs,
+                       const uint8_t* __restrict__ act, double* __restrict__ padiag, double* __restrict__ pal,
+                       double* __restrict__ pad, double* __restrict__ pab, double* __restrict__ prhs,
+                       uint8_t* __restrict__ nbm, uint16_t* __restrict__ tfl, double* __restrict__ tnum,
+                       double* __restrict__ x, double* __restrict__ q, double* __restrict__ z) {
+  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= L.ne) return;
+  int B, s, a, b, i, j, k;
+  bool ok = decode(L, e, B, s, a, b, i, j, k);
+  double vd = 0, vl = 0, vdn = 0, vb = 0, vr = 0;
+  uint8_t m = 0;
+  uint16_t tf = 0;
+  double t6[6] = {0, 0, 0, 0, 0, 0};
+  if (ok) {
+    const int ny = L.ny, nz = L.nz, nx = L.nx;
+    const int sx = ny * nz, sy = nz;
+    int c = k + nz * (j + ny * i);
+    vd = Adiag[c];
+    vl = AL[c];
+    vdn = AD[c];
+    vb = AB[c];
+    vr = rhs[c];
+    if (act[c]) {
+      m = NB_SELF;
+      if (i > 0 && act[c - sx]) m |= NB_L;
+      if (j > 0 && act[c - sy]) m |= NB_D;
+      if (k > 0 && act[c - 1]) m |= NB_B;
+      if (i < nx - 1 && act[c + sx]) m |= NB_R;
+      if (j < ny - 1 && act[c + sy]) m |= NB_U;
+      if (k < nz - 1 && act[c + 1]) m |= NB_F;
+      if (vd != 0.0) {
+        tf = TF_DO;
+        if (i > 0 && act[c - sx] && Adiag[c - sx] != 0.0) {
+          tf |= TF_GL;
+          if (j < ny - 1 && act[c - sx + sy]) {
+            tf |= TF_L1;
+            t6[0] = tau * vl * AL[c - sx + sy];
+          }
+          if (k < nz - 1 && act[c - sx + 1]) {
+            tf |= TF_L2;
+            t6[1] = tau * vl * AL[c - sx + 1];
+          }
+        }
+        if (j > 0 && act[c - sy] && Adiag[c - sy] != 0.0) {
+          tf |= TF_GD;
+          if (i < nx - 1 && act[c + sx - sy]) {
+            tf |= TF_D1;
+            t6[2] = tau * vdn * AD[c + sx - sy];
+          }
+          if (k < nz - 1 && act[c - sy + 1]) {
+            tf |= TF_D2;
+            t6[3] = tau * vdn * AD[c - sy + 1];
+          }
+        }
+        if (k > 0 && act[c - 1] && Adiag[c - 1] != 0.0) {
+          tf |= TF_GB;
+          if (i < nx - 1 && act[c + sx - 1]) {
+            tf |= TF_B1;
+            t6[4] = tau * vb * AB[c + sx - 1];
+          }
+          if (j < ny - 1 && act[c + sy - 1]) {
+            tf |= TF_B2;
+            t6[5] = tau * vb * AB[c + sy - 1];
+          }
+        }
+      }
+    }
+  }
+  padiag[e] = vd;
+  pal[e] = vl;
+  pad[e] = vdn;
+  pab[e] = vb;
+  prhs[e] = vr;
+  nbm[e] = m;
+  tfl[e] = tf;
+#pragma unroll
+  for (int u = 0; u < 6; u++) tnum[u * L.ne + e] = t6[u];
+  x[e] = 0.0;
+  q[e] = 0.0;
+  z[e] = 0.0;
+}
+
+// E factor sweep (project-solver.cc:237-274), one wave per brick, tickets in dependency order.
+__global__ __launch_bounds__(64) void k_factor_sweep(Lay L, double sigma, const double* __restrict__ adiag,
+                                                     const double* __restrict__ al, const double* __restrict__ ad,
+                                                     const double* __restrict__ ab, const uint16_t* __restrict__ tfl,
+                                                     const double* __restrict__ tnum, double* __restrict__ E,
+                                                     uint64_t* ticket, uint64_t tbase, const int* __restrict__ order,
+                                                     uint64_t* progress, uint64_t pbase, PcgScalars* sc) {
+  const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
+  const int B = take_ticket(ticket, tbase, order, L.nlive);
+  const int2 ij = L.bij[B];
+  const int BL = ij.x > 0 ? L.live[(ij.x - 1) * L.nbj + ij.y] : -1;
+  const int BD = ij.y > 0 ? L.live[ij.x * L.nbj + ij.y - 1] : -1;
+  uint64_t seenL = 0, seenD = 0;
+  double eprev = 0.0;
+  for (int s = 0; s < L.S; s++) {
+    const int64_t e = eidx(L, B, s, lane);
+    const int need = min(s + 8, L.S);
+    double EL = __shfl_up(eprev, 8, 64);
+    double ED = __shfl_up(eprev, 1, 64);
+    if (BL >= 0) seenL = wait_progress(progress + BL, pbase + need, seenL, sc);
+    if (BD >= 0) seenD = wait_progress(progress + BD, pbase + need, seenD, sc);
+    if (a == 0) EL = (BL >= 0 && s + 7 < L.S) ? E[eidx(L, BL, s + 7, 56 + b)] : 0.0;
+    if (b == 0) ED = (BD >= 0 && s + 7 < L.S) ? E[eidx(L, BD, s + 7, a * 8 + 7)] : 0.0;
+    const int tf = tfl[e];
+    double ev = 0.0;
+    if (tf & TF_DO) {
+      const double dg = adiag[e];
+      double x = dg;
+      if (tf & TF_GL) {
+        double t = al[e] * EL;
+        x -= t * t;
+        if (tf & TF_L1) x -= tnum[e] / (EL * EL);
+        if (tf & TF_L2) x -= tnum[L.ne + e] / (EL * EL);
+      }
+      if (tf & TF_GD) {
+        double t = ad[e] * ED;
+        x -= t * t;
+        if (tf & TF_D1) x -= tnum[2 * L.ne + e] / (ED * ED);
+        if (tf & TF_D2) x -= tnum[3 * L.ne + e] / (ED * ED);
+      }
+      if (tf & TF_GB) {
+        double t = ab[e] * eprev;
+        x -= t * t;
+        if (tf & TF_B1) x -= tnum[4 * L.ne + e] / (eprev * eprev);
+        if (tf & TF_B2) x -= tnum[5 * L.ne + e] / (eprev * eprev);
+      }
+      if (x < sigma * dg) x = dg;
+      ev = 1.0 / sqrt(x);
+    }
+    E[e] = ev;
+    eprev = ev;
+    if ((s + 1) % PUB == 0 || s == L.S - 1) publish(progress + B, pbase + s + 1);
+  }
+}
+
+// Sweep coefficients: forward cL = AL(c)*E(i-1) etc., backward dR = AL(i+1)*E(c) etc.
+__global__ void k_coef(Lay L, const double* __restrict__ al, const double* __restrict__ ad,
+                       const double* __restrict__ ab, const uint8_t* __restrict__ nbm, const double* __restrict__ E,
+                       double* __restrict__ cl, double* __restrict__ cd, double* __restrict__ cb,
+                       double* __restrict__ dr, double* __restrict__ du, double* __restrict__ df) {
+  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= L.ne) return;
+  int B, s, a, b, i, j, k;
+  decode(L, e, B, s, a, b, i, j, k);
+  const int m = nbm[e];
+  double vl = 0, vd = 0, vb = 0, vr = 0, vu = 0, vf = 0;
+  if (m & NB_SELF) {
+    const double ec = E[e];
+    if (m & NB_L) vl = al[e] * E[nb_elem(L, B, s, a, b, 0)];
+    if (m & NB_D) vd = ad[e] * E[nb_elem(L, B, s, a, b, 2)];
+    if (m & NB_B) vb = ab[e] * E[e - 64];
+    if (m & NB_R) vr = al[nb_elem(L, B, s, a, b, 1)] * ec;
+    if (m & NB_U) vu = ad[nb_elem(L, B, s, a, b, 3)] * ec;
+    if (m & NB_F) vf = ab[e + 64] * ec;
+  }
+  cl[e] = vl;
+  cd[e] = vd;
+  cb[e] = vb;
+  dr[e] = vr;
+  du[e] = vu;
+  df[e] = vf;
+}
+
+// r = rhs, ||r||_inf and ||r||_2^2 (fixed tree); the last block decides "naturally converged".
+__global__ __launch_bounds__(256) void k_init(Lay L, const double* __restrict__ rhs, double* __restrict__ r,
+                                              double* __restrict__ part, double* __restrict__ part2, int nchunks,
+                                              PcgScalars* sc, int tol_mode, double tol) {
+  __shared__ double sm[4];
+  double mx = 0.0, ss = 0.0;
+  for (int m = 0; m < 16; m++) {
+    int64_t e = (int64_t)blockIdx.x * CH + m * 256 + threadIdx.x;
+    double v = 0.0;
+    if (e < L.ne) {
+      v = rhs[e];
+      r[e] = v;
+    }
+    double av = fabs(v);
+    mx = (mx < av) ? av : mx;
+    ss += v * v;
+  }
+  double bm = block_max256(mx, sm);
+  double bs = block_sum256(ss, sm);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = bs;
+    part2[blockIdx.x] = bm;
+  }
+  if (last_block(&sc->done_ctr[DC_INIT], gridDim.x, 256)) {
+    double tot = tree256(part, nchunks, sm);
+    double mxx = 0.0;
+    for (int q = 0; q < nchunks; q++) mxx = (mxx < part2[q]) ? part2[q] : mxx;
+    if (threadIdx.x == 0) {
+      sc->bnorm2sq = tot;
+      sc->rnorm2sq = tot;
+      sc->residual = mxx;
+      bool conv = tol_mode == FS_TOL_REL_L2 ? (sqrt(tot) <= tol * sqrt(tot)) : (mxx < tol);
+      sc->converged = conv ? STOP_CONVERGED : STOP_RUN;
+      sc->conv_iter = 0;
+      sc->status = FS_OK;
+      sc->wait_timeouts = 0;
+    }
+  }
+}
+
+// z = A s with the fixed-tree s.z (applyCompressedMatrix + dotProduct(s, z)); last block
+// forms alpha = sigma / s.z.
+template <bool PCG>
+__global__ __launch_bounds__(256) void k_spmv(Lay L, const double* __restrict__ x, const double* __restrict__ adiag,
+                                              const double* __restrict__ al, const double* __restrict__ ad,
+                                              const double* __restrict__ ab, const uint8_t* __restrict__ nbm,
+                                              double* __restrict__ y, double* __restrict__ part, int nchunks,
+                                              PcgScalars* sc, int iter) {
+  __shared__ double sm[4];
+  if (PCG && sc->converged != STOP_RUN) return;
+  double acc = 0.0;
+  for (int m = 0; m < 16; m++) {
+    int64_t e = (int64_t)blockIdx.x * CH + m * 256 + threadIdx.x;
+    if (e >= L.ne) continue;
+    const int f = nbm[e];
+    double v = 0.0;
+    const double xc = x[e];
+    if (f & NB_SELF) {
+      int B, s, a, b, i, j, k;
+      decode(L, e, B, s, a, b, i, j, k);
+      v = adiag[e] * xc;
+      if (f & NB_L) {
+        int64_t n = nb_elem(L, B, s, a, b, 0);
+        v = v + al[e] * x[n];
+      }
+      if (f & NB_D) {
+        int64_t n = nb_elem(L, B, s, a, b, 2);
+        v = v + ad[e] * x[n];
+      }
+      if (f & NB_B) v = v + ab[e] * x[e - 64];
+      if (f & NB_R) {
+        int64_t n = nb_elem(L, B, s, a, b, 1);
+        v = v + al[n] * x[n];
+      }
+      if (f & NB_U) {
+        int64_t n = nb_elem(L, B, s, a, b, 3);
+        v = v + ad[n] * x[n];
+      }
+      if (f & NB_F) v = v + ab[e + 64] * x[e + 64];
+    }
+    y[e] = v;
+    acc += xc * v;
+  }
+  if (!PCG) return;
+  double bs = block_sum256(acc, sm);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+  if (last_block(&sc->done_ctr[DC_SPMV], gridDim.x, 256)) {
+    double sdotz = tree256(part, nchunks, sm);
+    if (threadIdx.x == 0) {
+      sc->sdotz = sdotz;
+      if (sdotz == 0.0) {
+        sc->converged = STOP_ABORT;
+        sc->status = FS_BREAKDOWN;
+        sc->conv_iter = iter;
+      } else {
+        sc->alpha = sc->sigma / sdotz;
+      }
+    }
+  }
+}
+
+// Forward sweep L q = r (project-solver.cc:275-288). UPDATE: first r += (-alpha) t and
+// ||r||_inf (saxpy + LinfNorm, :333-335); the last brick decides convergence.
+template <bool UPDATE>
+__global__ __launch_bounds__(64) void k_fwd(Lay L, double* __restrict__ r, const double* __restrict__ t,
+                                            const double* __restrict__ cl, const double* __restrict__ cd,
+                                            const double* __restrict__ cb, const double* __restrict__ E,
+                                            double* __restrict__ q, double* __restrict__ part2, uint64_t* ticket,
+                                            uint64_t tbase, const int* __restrict__ order, uint64_t* progress,
+                                            uint64_t pbase, PcgScalars* sc, int iter, int tol_mode, double tol,
+                                            double* __restrict__ part) {
+  const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
+  const int B = take_ticket(ticket, tbase, order, L.nlive);
+  if (UPDATE && sc->converged != STOP_RUN) return;
+  const double nalpha = UPDATE ? -sc->alpha : 0.0;
+  const int2 ij = L.bij[B];
+  const int BL = ij.x > 0 ? L.live[(ij.x - 1) * L.nbj + ij.y] : -1;
+  const int BD = ij.y > 0 ? L.live[ij.x * L.nbj + ij.y - 1] : -1;
+  uint64_t seenL = 0, seenD = 0;
+  double qprev = 0.0, mx = 0.0, ss = 0.0;
+  for (int s = 0; s < L.S; s++) {
+    const int64_t e = eidx(L, B, s, lane);
+    double rr = r[e];
+    if (UPDATE) {
+      rr = rr + nalpha * t[e];
+      r[e] = rr;
+      double av = fabs(rr);
+      mx = (mx < av) ? av : mx;
+      ss += rr * rr;
+    }
+    const int need = min(s + 8, L.S);
+    double qL = __shfl_up(qprev, 8, 64);
+    double qD = __shfl_up(qprev, 1, 64);
+    if (BL >= 0) seenL = wait_progress(progress + BL, pbase + need, seenL, sc);
+    if (BD >= 0) seenD = wait_progress(progress + BD, pbase + need, seenD, sc);
+    if (a == 0) qL = (BL >= 0 && s + 7 < L.S) ? q[eidx(L, BL, s + 7, 56 + b)] : 0.0;
+    if (b == 0) qD = (BD >= 0 && s + 7 < L.S) ? q[eidx(L, BD, s + 7, a * 8 + 7)] : 0.0;
+    double tt = rr - cl[e] * qL;
+    tt = tt - cd[e] * qD;
+    tt = tt - cb[e] * qprev;
+    const double qv = tt * E[e];
+    q[e] = qv;
+    qprev = qv;
+    if ((s + 1) % PUB == 0 || s == L.S - 1) publish(progress + B, pbase + s + 1);
+  }
+  if (UPDATE) {
+    double wm = wave_max(mx);
+    double wsum = wave_sum(ss);
+    if (lane == 0) {
+      part2[B] = wm;
+      part[B] = wsum;
+    }
+    if (last_block(&sc->done_ctr[DC_FWD], L.nlive, 64)) {
+      double res = max_wave(part2, L.nlive);
+      double rr2 = tree256_wave(part, L.nlive);
+      if (lane == 0) {
+        sc->residual = res;
+        sc->rnorm2sq = rr2;
+        bool conv = tol_mode == FS_TOL_REL_L2 ? (sqrt(rr2) <= tol * sqrt(sc->bnorm2sq)) : (res < tol);
+        if (!(res <= 1.7976931348623157e308)) {
+          sc->converged = STOP_ABORT;
+          sc->status = FS_NONFINITE;
+          sc->conv_iter = iter;
+        } else if (conv) {
+          sc->converged = STOP_CONVERGED;
+          sc->conv_iter = iter;
+        }
+      }
+    }
+  }
+}
+
+// Backward sweep L^T z = q (project-solver.cc:289-305) and the fixed-tree z.r; the last brick
+// forms beta = sigma_new / sigma (INIT: sigma = z.r).
+template <bool INIT>
+__global__ __launch_bounds__(64) void k_bwd(Lay L, const double* __restrict__ qv, const double* __restrict__ dr,
+                                            const double* __restrict__ du, const double* __restrict__ df,
+                                            const double* __restrict__ E, const double* __restrict__ r,
+                                            double* __restrict__ z, double* __restrict__ part, uint64_t* ticket,
+                                            uint64_t tbase, const int* __restrict__ order, uint64_t* progress,
+                                            uint64_t pbase, PcgScalars* sc) {
+  const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
+  const int B = take_ticket(ticket, tbase, order, L.nlive);
+  if (!INIT && sc->converged != STOP_RUN) return;
+  const int2 ij = L.bij[B];
+  const int BR = ij.x + 1 < L.nbi ? L.live[(ij.x + 1) * L.nbj + ij.y] : -1;
+  const int BU = ij.y + 1 < L.nbj ? L.live[ij.x * L.nbj + ij.y + 1] : -1;
+  uint64_t seenR = 0, seenU = 0;
+  double zprev = 0.0, acc = 0.0;
+  for (int u = 0; u < L.S; u++) {
+    const int s = L.S - 1 - u;
+    const int64_t e = eidx(L, B, s, lane);
+    const int need = min(u + 8, L.S);
+    double zR = __shfl_down(zprev, 8, 64);
+    double zU = __shfl_down(zprev, 1, 64);
+    if (BR >= 0) seenR = wait_progress(progress + BR, pbase + need, seenR, sc);
+    if (BU >= 0) seenU = wait_progress(progress + BU, pbase + need, seenU, sc);
+    if (a == 7) zR = (BR >= 0 && s >= 7) ? z[eidx(L, BR, s - 7, b)] : 0.0;
+    if (b == 7) zU = (BU >= 0 && s >= 7) ? z[eidx(L, BU, s - 7, a * 8)] : 0.0;
+    double tt = qv[e] - dr[e] * zR;
+    tt = tt - du[e] * zU;
+    tt = tt - df[e] * zprev;
+    const double zv = tt * E[e];
+    z[e] = zv;
+    zprev = zv;
+    acc += zv * r[e];
+    if ((u + 1) % PUB == 0 || u == L.S - 1) publish(progress + B, pbase + u + 1);
+  }
+  double ws = wave_sum(acc);
+  if (lane == 0) part[B] = ws;
+  if (last_block(&sc->done_ctr[DC_BWD], L.nlive, 64)) {
+    double sig = tree256_wave(part, L.nlive);
+    if (lane == 0) {
+      if (INIT) {
+        sc->sigma = sig;
+      } else {
+        sc->beta = sig / sc->sigma;
+        sc->sigma = sig;
+      }
+    }
+  }
+}
+
+// pg += alpha s (saxpy, :332); s = z + beta s (scaleAndAdd, :341) unless the solve stopped.
+__global__ __launch_bounds__(256) void k_update(int64_t ne, double* __restrict__ x, double* __restrict__ s,
+                                                const double* __restrict__ z, PcgScalars* sc, int iter) {
+  const int st = sc->converged;
+  if (st == STOP_ABORT) return;
+  if (st == STOP_CONVERGED && sc->conv_iter != iter) return;
+  const double alpha = sc->alpha, beta = sc->beta;
+  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double sv = s[e];
+  x[e] = x[e] + alpha * sv;
+  if (st == STOP_RUN) s[e] = z[e] + beta * sv;
+}
+
+__global__ void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
+  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e < n) b[e] = a[e];
+}
+
+// layout -> dense for active cells only (inactive dense entries untouched)
+__global__ void k_unpack(Lay L, const double* __restrict__ v, const uint8_t* __restrict__ nbm, double* __restrict__ dense) {
+  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= L.ne) return;
+  if (!(nbm[e] & NB_SELF)) return;
+  int B, s, a, b, i, j, k;
+  decode(L, e, B, s, a, b, i, j, k);
+  dense[k + L.nz * (j + L.ny * i)] = v[e];
+}
+// dense -> layout (zero where the cell is padding or inactive)
+__global__ void k_pack_vec(Lay L, const double* __restrict__ dense, const uint8_t* __restrict__ nbm, double* __restrict__ v) {
+  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= L.ne) return;
+  double x = 0.0;
+  if (nbm[e] & NB_SELF) {
+    int B, s, a, b, i, j, k;
+    decode(L, e, B, s, a, b, i, j, k);
+    x = dense[k + L.nz * (j + L.ny * i)];
+  }
+  v[e] = x;
+}
+
+// ---------------------------------------------------------------------------------------------
+Lay lay_of(const fs_ctx* c) {
+  const PcgState& P = c->pcg;
+  Lay L;
+  L.nx = c->nx;
+  L.ny = c->ny;
+  L.nz = c->nz;
+  L.nbi = P.nbi;
+  L.nbj = P.nbj;
+  L.S = P.S;
+  L.nlive = P.nlive;
+  L.ne = P.ne;
+  L.live = P.live_map.as<int>();
+  L.bij = P.brick_ij.as<int2>();
+  return L;
+}
+
+int realloc_if(fs_ctx* c, DBuf& b, size_t bytes) {
+  if (b.p && b.bytes >= bytes) return FS_OK;
+  fs_free(b);
+  return fs_alloc(c, b, bytes);
+}
+
+// Build the brick layout from the assembled (dense) system and pack it.
+int pcg_build(fs_ctx* c) {
+  PcgState& P = c->pcg;
+  P.nbi = (c->nx + 7) / 8;
+  P.nbj = (c->ny + 7) / 8;
+  P.S = c->nz + 14;
+  const int nb = P.nbi * P.nbj;
+  FS_TRY(realloc_if(c, P.brick_flag, nb * sizeof(int)));
+  FS_TRY(realloc_if(c, P.live_map, nb * sizeof(int)));
+  if (!P.scal.p) {
+    FS_TRY(fs_alloc(c, P.scal, sizeof(PcgScalars)));
+    FS_HIP(c, hipMemsetAsync(P.scal.p, 0, sizeof(PcgScalars), c->stream));
+    FS_HIP(c, hipHostMalloc((void**)&P.host_scal, sizeof(PcgScalars), hipHostMallocDefault));
+    FS_TRY(fs_alloc(c, P.tickets, 4 * sizeof(uint64_t)));
+    FS_HIP(c, hipMemsetAsync(P.tickets.p, 0, 4 * sizeof(uint64_t), c->stream));
+    for (int q = 0; q < 3; q++) P.ticket_base[q] = 0;
+  }
+  FS_HIP(c, hipMemsetAsync(P.brick_flag.p, 0, nb * sizeof(int), c->stream));
+  k_brick_flags<<<fs_blocks((int64_t)c->nx * c->ny, 256), 256, 0, c->stream>>>(
+      c->nx, c->ny, c->nz, c->active.as<uint8_t>(), P.brick_flag.as<int>(), P.nbj);
+  FS_HIP(c, hipGetLastError());
+  std::vector<int> flags(nb);
+  FS_HIP(c, hipMemcpyAsync(flags.data(), P.brick_flag.p, nb * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  FS_HIP(c, hipStreamSynchronize(c->stream));
+  P.live_host.assign(nb, -1);
+  std::vector<int2> bij;
+  for (int I = 0; I < P.nbi; I++)
+    for (int J = 0; J < P.nbj; J++)
+      if (flags[I * P.nbj + J]) {
+        P.live_host[I * P.nbj + J] = (int)bij.size();
+        bij.push_back(make_int2(I, J));
+      }
+  P.nlive = (int)bij.size();
+  P.ne = (int64_t)P.nlive * P.S * 64;
+  P.nchunks = (int)((P.ne + CH - 1) / CH);
+  // dependency order: anti-diagonals I+J ascending (then I); backward uses the reverse
+  std::vector<int> ord(P.nlive);
+  for (int q = 0; q < P.nlive; q++) ord[q] = q;
+  std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) {
+    int du = bij[u].x + bij[u].y, dv = bij[v].x + bij[v].y;
+    return du < dv;
+  });
+  std::vector<int> rord(ord.rbegin(), ord.rend());
+  size_t nl = std::max(P.nlive, 1);
+  FS_TRY(realloc_if(c, P.brick_ij, nl * sizeof(int2)));
+  FS_TRY(realloc_if(c, P.fwd_order, nl * sizeof(int)));
+  FS_TRY(realloc_if(c, P.bwd_order, nl * sizeof(int)));
+  FS_TRY(realloc_if(c, P.progress, 3 * nl * sizeof(uint64_t)));
+  if (P.nlive > 0) {
+    FS_HIP(c, hipMemcpyAsync(P.live_map.p, P.live_host.data(), nb * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    FS_HIP(c, hipMemcpyAsync(P.brick_ij.p, bij.data(), P.nlive * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    FS_HIP(c, hipMemcpyAsync(P.fwd_order.p, ord.data(), P.nlive * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    FS_HIP(c, hipMemcpyAsync(P.bwd_order.p, rord.data(), P.nlive * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  }
+  // progress counters restart at zero with a fresh epoch base
+  FS_HIP(c, hipMemsetAsync(P.progress.p, 0, 3 * nl * sizeof(uint64_t), c->stream));
+  for (int q = 0; q < 3; q++) P.epoch[q] = 0;
+  const size_t ne = (size_t)std::max<int64_t>(P.ne, 1);
+  for (DBuf* d : {&P.adiag, &P.al, &P.ad, &P.ab, &P.rhs, &P.e, &P.cl, &P.cd, &P.cb, &P.dr, &P.du, &P.df, &P.x, &P.r,
+                  &P.s, &P.t, &P.q, &P.z})
+    FS_TRY(realloc_if(c, *d, ne * sizeof(double)));
+  FS_TRY(realloc_if(c, P.nbm, ne));
+  FS_TRY(realloc_if(c, P.tflags, ne * sizeof(uint16_t)));
+  FS_TRY(realloc_if(c, P.tnum, 6 * ne * sizeof(double)));
+  size_t np_ = std::max<size_t>(std::max(P.nchunks, P.nlive), 1);
+  FS_TRY(realloc_if(c, P.partials, np_ * sizeof(double)));
+  FS_TRY(realloc_if(c, P.partials2, np_ * sizeof(double)));
+  if (P.nlive > 0) {
+    Lay L = lay_of(c);
+    // Aneighbour order Left, Right, Up, Down, Front, Back
+    k_pack<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(
+        L, c->cfg.mic_tau, c->adiag.as<double>(), c->an[0].as<double>(), c->an[3].as<double>(),
+        c->an[5].as<double>(), c->rhs.as<double>(), c->active.as<uint8_t>(), P.adiag.as<double>(),
+        P.al.as<double>(), P.ad.as<double>(), P.ab.as<double>(), P.rhs.as<double>(), P.nbm.as<uint8_t>(),
+        P.tflags.as<uint16_t>(), P.tnum.as<double>(), P.x.as<double>(), P.q.as<double>(), P.z.as<double>());
+    FS_HIP(c, hipGetLastError());
+  }
+  P.built = true;
+  return FS_OK;
+}
+
+// tickets: one counter per sweep kind; progress: one slot range per sweep kind.
+enum { SW_FACTOR = 0, SW_FWD = 1, SW_BWD = 2 };
+
+int pcg_factor(fs_ctx* c) {
+  PcgState& P = c->pcg;
+  Lay L = lay_of(c);
+  uint64_t* tk = P.tickets.as<uint64_t>();
+  uint64_t pb = (P.epoch[SW_FACTOR]++) * (uint64_t)(P.S + 1);
+  k_factor_sweep<<<P.nlive, 64, 0, c->stream>>>(
+      L, c->cfg.mic_sigma, P.adiag.as<double>(), P.al.as<double>(), P.ad.as<double>(), P.ab.as<double>(),
+      P.tflags.as<uint16_t>(), P.tnum.as<double>(), P.e.as<double>(), tk + SW_FACTOR, P.ticket_base[SW_FACTOR],
+      P.fwd_order.as<int>(), P.progress.as<uint64_t>() + SW_FACTOR * P.nlive, pb, P.scal.as<PcgScalars>());
+  P.ticket_base[SW_FACTOR] += P.nlive;
+  FS_HIP(c, hipGetLastError());
+  k_coef<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.al.as<double>(), P.ad.as<double>(), P.ab.as<double>(),
+                                                       P.nbm.as<uint8_t>(), P.e.as<double>(), P.cl.as<double>(),
+                                                       P.cd.as<double>(), P.cb.as<double>(), P.dr.as<double>(),
+                                                       P.du.as<double>(), P.df.as<double>());
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+// fwd sweep: UPDATE applies r -= alpha t first.
+int launch_fwd(fs_ctx* c, bool update, int iter) {
+  PcgState& P = c->pcg;
+  Lay L = lay_of(c);
+  uint64_t* tk = P.tickets.as<uint64_t>();
+  uint64_t pb = (P.epoch[SW_FWD]++) * (uint64_t)(P.S + 1);
+  auto args = [&](auto kern) {
+    kern<<<P.nlive, 64, 0, c->stream>>>(L, P.r.as<double>(), P.t.as<double>(), P.cl.as<double>(), P.cd.as<double>(),
+                                        P.cb.as<double>(), P.e.as<double>(), P.q.as<double>(), P.partials2.as<double>(),
+                                        tk + SW_FWD, P.ticket_base[SW_FWD], P.fwd_order.as<int>(),
+                                        P.progress.as<uint64_t>() + SW_FWD * P.nlive, pb, P.scal.as<PcgScalars>(),
+                                        iter, c->cfg.tol_mode, c->cfg.cg_tolerance, P.partials.as<double>());
+  };
+  if (update) args(k_fwd<true>);
+  else args(k_fwd<false>);
+  P.ticket_base[SW_FWD] += P.nlive;
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+int launch_bwd(fs_ctx* c, bool init) {
+  PcgState& P = c->pcg;
+  Lay L = lay_of(c);
+  uint64_t* tk = P.tickets.as<uint64_t>();
+  uint64_t pb = (P.epoch[SW_BWD]++) * (uint64_t)(P.S + 1);
+  auto args = [&](auto kern) {
+    kern<<<P.nlive, 64, 0, c->stream>>>(L, P.q.as<double>(), P.dr.as<double>(), P.du.as<double>(), P.df.as<double>(),
+                                        P.e.as<double>(), P.r.as<double>(), P.z.as<double>(), P.partials.as<double>(),
+                                        tk + SW_BWD, P.ticket_base[SW_BWD], P.bwd_order.as<int>(),
+                                        P.progress.as<uint64_t>() + SW_BWD * P.nlive, pb, P.scal.as<PcgScalars>());
+  };
+  if (init) args(k_bwd<true>);
+  else args(k_bwd<false>);
+  P.ticket_base[SW_BWD] += P.nlive;
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+int read_scalars(fs_ctx* c) {
+  PcgState& P = c->pcg;
+  FS_HIP(c, hipMemcpyAsync(P.host_scal, P.scal.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, c->stream));
+  FS_HIP(c, hipStreamSynchronize(c->stream));
+  return FS_OK;
+}
+
+}  // namespace
+
+// CgSolver3D::solve on the assembled system; writes pg (dense, active cells; others 0).
+int fs_stage_solve(fs_ctx* c, fs_stats* st) {
+  if (c->mixed) {
+    c->set_error("mixed-precision pressure solve not available in this build");
+    return FS_UNSUPPORTED;
+  }
+  FS_TRY(pcg_build(c));
+  PcgState& P = c->pcg;
+  FS_HIP(c, hipMemsetAsync(c->pg.p, 0, (size_t)c->n_cell * sizeof(double), c->stream));
+  int iters = 0;
+  int status = FS_OK;
+  PcgScalars* hs = P.host_scal;
+  if (P.nlive == 0) {
+    std::memset(hs, 0, sizeof(*hs));
+  } else {
+    Lay L = lay_of(c);
+    const double tol = c->cfg.cg_tolerance;
+    k_init<<<P.nchunks, 256, 0, c->stream>>>(L, P.rhs.as<double>(), P.r.as<double>(), P.partials.as<double>(),
+                                             P.partials2.as<double>(), P.nchunks, P.scal.as<PcgScalars>(),
+                                             c->cfg.tol_mode, tol);
+    FS_HIP(c, hipGetLastError());
+    FS_TRY(read_scalars(c));
+    if (hs->converged == 0) {
+      FS_TRY(pcg_factor(c));
+      FS_TRY(launch_fwd(c, false, 0));
+      FS_TRY(launch_bwd(c, true));
+      k_copy<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(P.ne, P.z.as<double>(), P.s.as<double>());
+      const int maxit = c->cfg.cg_max_iterations;
+      const int chk = std::max(1, c->cfg.check_interval);
+      int it = 1;
+      bool stopped = false;
+      while (it < maxit && !stopped) {
+        for (int m = 0; m < chk && it < maxit; m++, it++) {
+          k_spmv<true><<<P.nchunks, 256, 0, c->stream>>>(L, P.s.as<double>(), P.adiag.as<double>(), P.al.as<double>(),
+                                                         P.ad.as<double>(), P.ab.as<double>(), P.nbm.as<uint8_t>(),
+                                                         P.t.as<double>(), P.partials.as<double>(), P.nchunks,
+                                                         P.scal.as<PcgScalars>(), it);
+          FS_TRY(launch_fwd(c, true, it));
+          FS_TRY(launch_bwd(c, false));
+          k_update<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(P.ne, P.x.as<double>(), P.s.as<double>(),
+                                                                P.z.as<double>(), P.scal.as<PcgScalars>(), it);
+        }
+        FS_HIP(c, hipGetLastError());
+        FS_TRY(read_scalars(c));
+        if (hs->converged != 0 || hs->wait_timeouts != 0) stopped = true;
+      }
+      if (hs->wait_timeouts != 0) {
+        c->set_error("PCG sweep: a brick dependency wait exceeded its bound");
+        return FS_DEVICE_ERROR;
+      }
+      if (hs->converged == 1 || hs->converged == 2) iters = hs->conv_iter;
+      else iters = maxit;
+      if (hs->converged == 2) status = hs->status;
+      else if (hs->converged == 0) status = FS_NOT_CONVERGED;
+    }
+    k_unpack<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.x.as<double>(), P.nbm.as<uint8_t>(), c->pg.as<double>());
+    FS_HIP(c, hipGetLastError());
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  if (st) {
+    st->cg_iterations = iters;
+    st->residual_inf = hs->residual;
+    st->rhs_norm2 = std::sqrt(hs->bnorm2sq);
+    st->residual_rel = hs->bnorm2sq > 0 ? std::sqrt(hs->rnorm2sq / hs->bnorm2sq) : 0.0;
+    st->n_active = 0;
+    st->refinements = 0;
+  }
+  return status;
+}
+
+// applyCompressedMatrix on the assembled system for host vectors (dense); inactive entries of
+// y are left as given.
+int fs_pcg_spmv_dense(fs_ctx* c, const double* x, double* y) {
+  if (c->mixed) return FS_UNSUPPORTED;
+  FS_TRY(pcg_build(c));
+  PcgState& P = c->pcg;
+  if (P.nlive == 0) return FS_OK;
+  Lay L = lay_of(c);
+  DBuf dx;
+  FS_TRY(fs_alloc(c, dx, (size_t)c->n_cell * sizeof(double)));
+  FS_HIP(c, hipMemcpyAsync(dx.p, x, (size_t)c->n_cell * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  k_pack_vec<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, dx.as<double>(), P.nbm.as<uint8_t>(), P.s.as<double>());
+  k_spmv<false><<<P.nchunks, 256, 0, c->stream>>>(L, P.s.as<double>(), P.adiag.as<double>(), P.al.as<double>(),
+                                                  P.ad.as<double>(), P.ab.as<double>(), P.nbm.as<uint8_t>(),
+                                                  P.t.as<double>(), P.partials.as<double>(), P.nchunks,
+                                                  P.scal.as<PcgScalars>(), 0);
+  FS_HIP(c, hipMemcpyAsync(dx.p, y, (size_t)c->n_cell * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  k_unpack<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.t.as<double>(), P.nbm.as<uint8_t>(), dx.as<double>());
+  FS_HIP(c, hipGetLastError());
+  FS_HIP(c, hipMemcpyAsync(y, dx.p, (size_t)c->n_cell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  FS_HIP(c, hipStreamSynchronize(c->stream));
+  fs_free(dx);
+  return FS_OK;
+}
+
+// ModifiedIncompleteCholesky3D::precond for a host vector r: z = M^-1 r on active cells.
+int fs_pcg_mic_dense(fs_ctx* c, const double* r, double* z) {
+  if (c->mixed) return FS_UNSUPPORTED;
+  FS_TRY(pcg_build(c));
+  PcgState& P = c->pcg;
+  if (P.nlive == 0) return FS_OK;
+  Lay L = lay_of(c);
+  DBuf dx;
+  FS_TRY(fs_alloc(c, dx, (size_t)c->n_cell * sizeof(double)));
+  FS_HIP(c, hipMemcpyAsync(dx.p, r, (size_t)c->n_cell * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  k_pack_vec<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, dx.as<double>(), P.nbm.as<uint8_t>(), P.r.as<double>());
+  FS_TRY(pcg_factor(c));
+  FS_TRY(launch_fwd(c, false, 0));
+  FS_TRY(launch_bwd(c, true));
+  FS_HIP(c, hipMemcpyAsync(dx.p, z, (size_t)c->n_cell * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  k_unpack<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.z.as<double>(), P.nbm.as<uint8_t>(), dx.as<double>());
+  FS_HIP(c, hipGetLastError());
+  FS_HIP(c, hipMemcpyAsync(z, dx.p, (size_t)c->n_cell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  FS_HIP(c, hipStreamSynchronize(c->stream));
+  PcgScalars hs;
+  FS_HIP(c, hipMemcpy(&hs, P.scal.p, sizeof(hs), hipMemcpyDeviceToHost));
+  fs_free(dx);
+  if (hs.wait_timeouts) {
+    c->set_error("MIC sweep: a brick dependency wait exceeded its bound");
+    return FS_DEVICE_ERROR;
+  }
+  return FS_OK;
+}
+
+// MIC factor E of the assembled system (dense; zero on inactive cells).
+int fs_pcg_export_e(fs_ctx* c, double* out) {
+  if (c->mixed) return FS_UNSUPPORTED;
+  FS_TRY(pcg_build(c));
+  PcgState& P = c->pcg;
+  std::memset(out, 0, (size_t)c->n_cell * sizeof(double));
+  if (P.nlive == 0) return FS_OK;
+  Lay L = lay_of(c);
+  FS_TRY(pcg_factor(c));
+  DBuf dx;
+  FS_TRY(fs_alloc(c, dx, (size_t)c->n_cell * sizeof(double)));
+  FS_HIP(c, hipMemsetAsync(dx.p, 0, (size_t)c->n_cell * sizeof(double), c->stream));
+  k_unpack<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.e.as<double>(), P.nbm.as<uint8_t>(), dx.as<double>());
+  FS_HIP(c, hipGetLastError());
+  FS_HIP(c, hipMemcpyAsync(out, dx.p, (size_t)c->n_cell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  FS_HIP(c, hipStreamSynchronize(c->stream));
+  fs_free(dx);
+  return FS_OK;
+}
+
+void fs_pcg_release(fs_ctx* c) {
+  PcgState& P = c->pcg;
+  for (DBuf* d : {&P.live_map, &P.brick_ij, &P.fwd_order, &P.bwd_order, &P.brick_flag, &P.adiag, &P.al, &P.ad, &P.ab,
+                  &P.nbm, &P.rhs, &P.e, &P.cl, &P.cd, &P.cb, &P.dr, &P.du, &P.df, &P.tflags, &P.tnum, &P.x, &P.r,
+                  &P.s, &P.t, &P.q, &P.z, &P.partials, &P.partials2, &P.scal, &P.progress, &P.tickets})
+    fs_free(*d);
+  if (P.host_scal) (void)hipHostFree(P.host_scal);
+  P.host_scal = nullptr;
+}

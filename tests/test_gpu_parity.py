@@ -1,0 +1,122 @@
+"""Device vs oracle parity through the C ABI (FP64 path).
+
+Bar: bit-exact. The oracle runs the reference algorithm with the device's fixed dot-product
+tree (dot_mode=DOT_TREE) and the device's P2G summation order, so every stage, and a whole
+substep, must agree bit for bit. Against the reference's serial dot product the pressure is
+compared within tolerance (rel. L2 <= 1e-10, iteration count +-1).
+"""
+import numpy as np
+import pytest
+
+from tests._pairs import ALL_ARRAYS, STAGES, assert_same, dam_scene, overlap_scene, setup
+from tests.conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+SCENES = {"dam": dam_scene, "overlap": overlap_scene}
+
+
+def make_pair(hip, oracle, n, scene, dot_mode=None, **kw):
+    cfg = hip.default_config(nx=n, ny=n, nz=n, **kw)
+    g = hip.FluidSim(cfg)
+    o = oracle.OracleSim(cfg, dot_mode=oracle.DOT_TREE if dot_mode is None else dot_mode)
+    pos, vel, col = SCENES[scene](n)
+    setup(g, pos, vel, col)
+    setup(o, pos, vel, col)
+    return g, o
+
+
+@pytest.mark.parametrize("advection", [1, 0], ids=["flip", "pic"])
+@pytest.mark.parametrize("scene", ["dam", "overlap"])
+def test_stagewise_bitexact(hip, oracle, scene, advection):
+    n, dt = 16, 0.004
+    announce(f"stagewise scene={scene} advection={advection} n={n}")
+    g, o = make_pair(hip, oracle, n, scene, advection=advection, flip_alpha=0.95)
+    for step in range(3):
+        for name, fn in STAGES:
+            ga = fn(g, dt, 1.0 / n)
+            oa = fn(o, dt, 1.0 / n)
+            if name == "solve_pressure":
+                assert ga["cg_iterations"] == oa["cg_iterations"], (step, ga, oa)
+                assert ga["residual_inf"] == oa["residual_inf"], (step, ga, oa)
+            assert_same(g, o, ALL_ARRAYS, f"step {step} after {name}")
+
+
+@pytest.mark.parametrize("scene", ["dam", "overlap"])
+def test_substeps_bitexact_32(hip, oracle, scene):
+    """Dam break at 32^3 (dt 0.004, FLIP 0.95): whole substeps stay bit-identical."""
+    n, dt = 32, 0.004
+    announce(f"substeps scene={scene} n={n}")
+    g, o = make_pair(hip, oracle, n, scene, flip_alpha=0.95)
+    for step in range(10):
+        sg = g.substep(dt)
+        so = o.substep(dt)
+        assert sg["cg_iterations"] == so["cg_iterations"], (step, sg, so)
+        gp, gv = g.particles()
+        op, ov = o.particles()
+        assert np.array_equal(gp, op), f"positions differ at step {step}"
+        assert np.array_equal(gv, ov), f"velocities differ at step {step}"
+    assert_same(g, o, ["pg", "fluid_sdf", "active", "u", "v", "w"], "after 10 substeps")
+
+
+def test_operators_spmv_mic_bitexact(hip, oracle):
+    n, dt = 32, 0.004
+    announce("spmv / mic / factor n=32 overlap")
+    g, o = make_pair(hip, oracle, n, "overlap")
+    for s in (g, o):
+        s.substep(dt)
+        for name, fn in STAGES[:10]:  # up to build_system
+            fn(s, dt, 1.0 / n)
+    assert_same(g, o, ["adiag", "a_left", "a_down", "a_back", "rhs", "active"], "system")
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((n, n, n))
+    y0 = rng.standard_normal((n, n, n))
+    assert np.array_equal(g.spmv(x, y0), o.spmv(x, y0))
+    r = rng.standard_normal((n, n, n))
+    z0 = np.zeros((n, n, n))
+    assert np.array_equal(g.mic_apply(r, z0), o.mic_apply(r, z0))
+    assert np.array_equal(g.get("mic_e"), _oracle_e(o))
+
+
+def _oracle_e(o):
+    """E after the oracle's last MIC call (mic_apply refactors, as the reference does)."""
+    return o.get("mic_e")
+
+
+def test_solve_parity_64(hip, oracle):
+    """64^3 dam break: tree-dot oracle is bit-exact; serial-dot oracle within tolerance."""
+    n, dt = 64, 0.002
+    announce("solve parity n=64")
+    g, o = make_pair(hip, oracle, n, "dam", flip_alpha=0.95)
+    o2 = oracle.OracleSim(o.cfg, dot_mode=oracle.DOT_SERIAL)
+    pos, vel, col = dam_scene(n)
+    setup(o2, pos, vel, col)
+    for s in (g, o, o2):
+        s.substep(dt)
+        for name, fn in STAGES[:10]:
+            fn(s, dt, 1.0 / n)
+    sg, so, ss = g.solve_pressure(), o.solve_pressure(), o2.solve_pressure()
+    assert sg["cg_iterations"] == so["cg_iterations"]
+    pg, po, ps = g.get("pg"), o.get("pg"), o2.get("pg")
+    assert np.array_equal(pg, po)
+    assert abs(sg["cg_iterations"] - ss["cg_iterations"]) <= 1
+    rel = np.linalg.norm(pg - ps) / np.linalg.norm(ps)
+    assert rel <= 1e-10, rel
+
+
+def test_iteration_cap_and_empty(hip, oracle):
+    n = 16
+    announce("iteration cap / empty scene")
+    g, o = make_pair(hip, oracle, n, "dam", cg_max_iterations=3)
+    sg, so = g.substep(0.004), o.substep(0.004)
+    assert sg["status"] == so["status"] == 1  # FS_NOT_CONVERGED, the step still completes
+    assert sg["cg_iterations"] == so["cg_iterations"] == 3
+    assert_same(g, o, ["pg", "u", "v", "w"], "capped solve")
+    # no particles: no fluid, no active cell, the solve is naturally converged
+    cfg = hip.default_config(nx=n, ny=n, nz=n)
+    e = hip.FluidSim(cfg)
+    e.set_collider_sdf(np.full((n, n, n), 10.0))
+    e.set_particles(np.zeros((0, 3)))
+    st = e.substep(0.004)
+    assert st["cg_iterations"] == 0 and st["status"] == 0
+    assert not e.get("active").any()
