@@ -1,8 +1,17 @@
 """FLIP step throughput on one MI355X.
 
 Metric: FLIP steps/sec at 256^3 (MIC-PCG to 1e-6). One step = one fs_substep of the dam break
-with the static sphere collider (256^3 cells, 8 particles per fluid cell, dt 5e-4), inputs
-resident in HBM before the timed region. Prints one JSON line.
+(fluid block x < 0.4, y < 0.6, 8 jittered particles per fluid cell, at rest) with the static
+sphere collider (r 0.15 at (0.7, 0.3, 0.5)), 256^3 cells, dt 5e-4. Inputs are resident in HBM
+before the timed region; the timed region is K substeps between two device synchronisations.
+
+The JSON line carries:
+  roofline      the kernel class with the largest device time in the timed region: algorithmic
+                bytes per launch / mean launch duration, both measured live with HIP events on
+                the library's stream, against 8 TB/s HBM.
+  cpu_baseline  the FP64 CPU oracle (a restatement of the reference, which does not build) on
+                the same workload: one 256^3 substep with the PCG capped at a few iterations;
+                the PCG share is scaled to the device's mean iteration count.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
 """
@@ -20,21 +29,70 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+HBM_PEAK_GBS = 8000.0
+
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_sim(simcraft_amd, scenes, wl: dict, tol_mode: int):
+def workload_config(simcraft_amd, wl: dict, tol_mode: int, profile: int):
     n = wl["n"]
     prec = simcraft_amd.PRECISION_MIXED if wl["precision"] == "mixed" else simcraft_amd.PRECISION_FP64
-    cfg = simcraft_amd.default_config(nx=n, ny=n, nz=n, flip_alpha=wl["flip_alpha"], precision=prec,
-                                      tol_mode=tol_mode, cg_max_iterations=wl.get("max_iters", 300))
-    sim = simcraft_amd.FluidSim(cfg)
+    return simcraft_amd.default_config(nx=n, ny=n, nz=n, flip_alpha=wl["flip_alpha"], precision=prec,
+                                       tol_mode=tol_mode, cg_max_iterations=wl.get("max_iters", 300),
+                                       profile_stages=profile)
+
+
+def cpu_baseline(simcraft_amd, scenes, wl: dict, tol_mode: int, gpu_iters: float, cap: int) -> dict:
+    """Time the oracle on one substep of the same workload (PCG capped at `cap` iterations)."""
+    from oracle.binding import DOT_SERIAL, P2G_PARALLEL, OracleSim
+
+    n = wl["n"]
+    cfg = workload_config(simcraft_amd, wl, tol_mode, 0)
+    cfg.cg_max_iterations = cap + 1  # `cap` iterations of the loop body
     pos, vel = scenes.dam_break(n)
-    sim.set_collider_sdf(scenes.collider_for(wl["collider"], n))
-    sim.set_particles(pos, vel)
-    return sim, len(pos)
+    o = OracleSim(cfg, dot_mode=DOT_SERIAL, p2g_order=P2G_PARALLEL)
+    o.set_collider_sdf(scenes.collider_for(wl["collider"], n))
+    o.set_particles(pos, vel)
+    del pos, vel
+    dt, h = wl["dt"], 1.0 / n
+    stages = [("clear", lambda: o.clear()), ("advect", lambda: o.advect(dt)),
+              ("reconstruct", lambda: o.reconstruct(0.8 * h)),
+              ("extrapolate_fluid_sdf", lambda: o.extrapolate_fluid_sdf(10)),
+              ("smooth_fluid_surface", lambda: o.smooth_fluid_surface(5)), ("p2g", lambda: o.p2g(dt)),
+              ("dirichlet", lambda: o.apply_dirichlet_boundary()),
+              ("extrapolate_velocity", lambda: o.extrapolate_velocity(10)), ("force", lambda: o.apply_force(dt)),
+              ("build_system", lambda: o.build_system(dt)), ("solve_pressure", None),
+              ("project", lambda: o.project(dt)), ("collider", lambda: o.apply_collider()),
+              ("g2p", lambda: o.g2p(dt))]
+    times = {}
+    pcg_iters = 0
+    for name, fn in stages:
+        t0 = time.perf_counter()
+        if fn is None:
+            st = o.solve_pressure()
+            pcg_iters = st["cg_iterations"]
+        else:
+            fn()
+        times[name] = time.perf_counter() - t0
+    o.close()
+    t_pcg = times.pop("solve_pressure")
+    per_iter = t_pcg / max(pcg_iters, 1)
+    step_s = sum(times.values()) + per_iter * gpu_iters
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": 1.0 / step_s,
+        "unit": "steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"one {n}^3 substep of the same workload with the PCG capped at {pcg_iters} iterations "
+                   f"({t_pcg:.2f}s); PCG time scaled to the device's mean {gpu_iters:.1f} iterations; "
+                   f"non-PCG stages {sum(times.values()):.2f}s measured in full; P2G with the reference's "
+                   f"parallel particle loop"),
+        "seconds_per_step": step_s,
+        "stage_seconds": {k: round(v, 3) for k, v in times.items()},
+    }
 
 
 def main() -> int:
@@ -45,9 +103,11 @@ def main() -> int:
     ap.add_argument("--workload", default="dam_break_sphere_256_fp64")
     ap.add_argument("--tol-mode", choices=["abs", "rel"], default="abs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-pcg-iters", type=int, default=5)
+    ap.add_argument("--no-kernel-timing", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md"}))
+        print(json.dumps({"error": "this build runs on one GPU only (the path's sharding is designed, not built)"}))
         return 2
 
     import simcraft_amd
@@ -57,22 +117,49 @@ def main() -> int:
     tol_mode = simcraft_amd.TOL_REL_L2 if args.tol_mode == "rel" else simcraft_amd.TOL_ABS_LINF
     log(f"workload {args.workload}: {wl}")
     t0 = time.perf_counter()
-    sim, npart = make_sim(simcraft_amd, scenes, wl, tol_mode)
+    cfg = workload_config(simcraft_amd, wl, tol_mode, 0 if args.no_kernel_timing else 2)
+    sim = simcraft_amd.FluidSim(cfg)
+    pos, vel = scenes.dam_break(wl["n"])
+    npart = len(pos)
+    sim.set_collider_sdf(scenes.collider_for(wl["collider"], wl["n"]))
+    sim.set_particles(pos, vel)
+    del pos, vel
     log(f"setup {time.perf_counter() - t0:.1f}s, {npart} particles")
     dt = wl["dt"]
     for w in range(args.warmup):
         st = sim.substep(dt)
         log(f"warmup {w}: {st['cg_iterations']} PCG iterations, residual {st['residual_inf']:.3g}")
     sim.synchronize()
-    iters = []
+    sim.kernel_profile_reset()
+    iters, active = [], []
     t0 = time.perf_counter()
     for s in range(args.steps):
         st = sim.substep(dt)
         iters.append(st["cg_iterations"])
+        active.append(st["n_active"])
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     log(f"timed {args.steps} steps in {elapsed:.3f}s; PCG iterations {iters}")
     sps = args.steps / elapsed
+
+    roofline = None
+    kernels = {}
+    if not args.no_kernel_timing:
+        prof = sim.kernel_profile()
+        for name, p in prof.items():
+            if p["launches"]:
+                avg_ms = p["ms"] / p["launches"]
+                gbs = (p["bytes"] / p["launches"]) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+                kernels[name] = {"launches": p["launches"], "ms_total": round(p["ms"], 3),
+                                 "us_per_launch": round(1e3 * avg_ms, 2),
+                                 "bytes_per_launch": p["bytes"] / p["launches"], "GBps": round(gbs, 1)}
+        if kernels:
+            dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
+            kd = kernels[dom]
+            roofline = {"bound": "hbm", "kernel": dom, "achieved": kd["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(kd["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+                        "us_per_launch": kd["us_per_launch"], "bytes_per_launch": kd["bytes_per_launch"]}
+            log("kernels: " + json.dumps(kernels))
     out = {
         "metric": "FLIP steps/sec at 256^3 (MIC-PCG to 1e-6)",
         "value": sps,
@@ -90,15 +177,25 @@ def main() -> int:
             "workload": args.workload,
             "grid": wl["n"],
             "particles": npart,
+            "active_cells_mean": float(np.mean(active)),
             "dt": dt,
             "collider": wl["collider"],
+            "precision": wl["precision"],
             "pcg_tolerance": 1e-6,
-            "tol_mode": args.tol_mode,
+            "tol_mode": "abs_linf" if args.tol_mode == "abs" else "rel_l2",
             "pcg_iterations_mean": float(np.mean(iters)),
         },
-        "roofline": None,
+        "roofline": roofline,
+        "kernels": kernels,
         "cpu_baseline": None,
     }
+    sim.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        t0 = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(simcraft_amd, scenes, wl, tol_mode, float(np.mean(iters)),
+                                           args.cpu_pcg_iters)
+        log(f"cpu baseline took {time.perf_counter() - t0:.1f}s: {out['cpu_baseline']['value']:.5f} steps/s")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -178,6 +178,27 @@ int fs_get_array(fs_ctx* ctx, const char* name, void* out);
 int fs_set_array(fs_ctx* ctx, const char* name, const void* in);
 int fs_synchronize(fs_ctx* ctx);
 
+/* ---- Kernel timing (no counterpart in the reference) ----
+ * With fs_config.profile_stages >= 2 the library brackets every launch of the kernels below
+ * with HIP events on its own stream and accumulates, per kernel class: launches, device
+ * milliseconds and algorithmic bytes (the minimum HBM traffic of the launch: every array the
+ * kernel must read or write, counted once per active cell / face / particle it covers; masks
+ * and the P2G binning keys excluded). */
+enum {
+  FS_KERNEL_SPMV = 0,       /* applyCompressedMatrix + s.z          48 B / active cell (FP64) */
+  FS_KERNEL_MIC_FWD = 1,    /* r -= alpha t, ||r||inf, L q = r      64 B / active cell */
+  FS_KERNEL_MIC_BWD = 2,    /* L^T z = q, z.r                       56 B / active cell */
+  FS_KERNEL_PCG_UPDATE = 3, /* pg += alpha s, s = z + beta s        40 B / active cell */
+  FS_KERNEL_MIC_FACTOR = 4, /* E factor                             90 B / active cell */
+  FS_KERNEL_P2G = 5,        /* records + gather, one face grid */
+  FS_KERNEL_G2P = 6,
+  FS_KERNEL_ADVECT = 7,
+  FS_KERNEL_RECONSTRUCT = 8,
+  FS_KERNEL_COUNT = 9
+};
+int fs_kernel_profile(fs_ctx* ctx, int32_t kind, int64_t* launches, double* total_ms, double* total_bytes);
+int fs_kernel_profile_reset(fs_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

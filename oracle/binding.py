@@ -18,7 +18,7 @@ from simcraft_amd._capi import fs_config, fs_stats, default_config  # noqa: F401
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libfs_oracle.so"
 DOT_SERIAL, DOT_TREE = 0, 1
-P2G_SORTED, P2G_INDEX = 0, 1
+P2G_SORTED, P2G_INDEX, P2G_PARALLEL = 0, 1, 2
 
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)

@@ -220,7 +220,7 @@ static inline int morton3D(int x, int y, int z) { return morton10(x) | (morton10
 
 // ---------------------------------------------------------------------------------------------
 // Device PCG layout, restated for the fixed-tree dot mode: live 8x8 column bricks in (i,j),
-// each holding S = nz+14 slots of 64 lanes; cell (i,j,k) of brick (i/8, j/8) sits in lane
+// each holding S = nz+14 (rounded up to a multiple of 8) slots of 64 lanes; cell (i,j,k) of brick (i/8, j/8) sits in lane
 // (i%8)*8 + (j%8), slot k + i%8 + j%8. Element e = (brick*S + slot)*64 + lane.
 struct PcgLayout {
   int nbi = 0, nbj = 0, S = 0, nlive = 0;
@@ -230,7 +230,7 @@ struct PcgLayout {
     int nx = active.nx, ny = active.ny, nz = active.nz;
     nbi = (nx + 7) / 8;
     nbj = (ny + 7) / 8;
-    S = nz + 14;
+    S = ((nz + 14 + 7) / 8) * 8;  // rounded up to whole 8-slot chunks
     live.assign((size_t)nbi * nbj, -1);
     nlive = 0;
     for (int I = 0; I < nbi; I++)
@@ -679,10 +679,45 @@ struct Sim {
           }
     }
   }
+  // p2g_order 2: the reference's parallel structure (TBB parallel_for over particles,
+  // advect-solver.cc:201), with atomic adds in place of its racy ones. Used only to time the
+  // CPU path; the sums then depend on thread timing at the rounding level.
+  void p2g_grid_parallel(int axis, RArr& G, RArr& Wt) {
+    int n = (int)pos.size();
+    double o0 = face_off(axis, 0), o1 = face_off(axis, 1), o2 = face_off(axis, 2);
+#pragma omp parallel for schedule(static)
+    for (int idx = 0; idx < n; idx++) {
+      V3 p = pos[idx];
+      double vc = axis == 0 ? vel[idx].x : (axis == 1 ? vel[idx].y : vel[idx].z);
+      int b0 = nearest1(p.x, o0, h[0]), b1 = nearest1(p.y, o1, h[1]), b2 = nearest1(p.z, o2, h[2]);
+      for (int i = -1; i <= 2; i++)
+        for (int j = -1; j <= 2; j++)
+          for (int k = -1; k <= 2; k++) {
+            int x = b0 + i, y = b1 + j, zz = b2 + k;
+            if (x >= 0 && x < G.nx && y >= 0 && y < G.ny && zz >= 0 && zz < G.nz) {
+              V3 pn{coord1(x, o0, h[0]), coord1(y, o1, h[1]), coord1(zz, o2, h[2])};
+              double w = kweight(h[0], h[1], h[2], p - pn);
+              double vw = vc * w;
+              double& g = G(x, y, zz);
+              double& wt = Wt(x, y, zz);
+#pragma omp atomic
+              g += vw;
+#pragma omp atomic
+              wt += w;
+            }
+          }
+    }
+  }
   void p2g() {
-    p2g_grid(0, *ug, uw);
-    p2g_grid(1, *vg, vw);
-    p2g_grid(2, *wg, ww);
+    if (p2g_order == 2) {
+      p2g_grid_parallel(0, *ug, uw);
+      p2g_grid_parallel(1, *vg, vw);
+      p2g_grid_parallel(2, *wg, ww);
+    } else {
+      p2g_grid(0, *ug, uw);
+      p2g_grid(1, *vg, vw);
+      p2g_grid(2, *wg, ww);
+    }
     auto norm = [](RArr& G, RArr& Wt, BArr& V) {
       V.fill(0);
       for (size_t e = 0; e < G.size(); e++) {

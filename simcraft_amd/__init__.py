@@ -221,3 +221,16 @@ class FluidSim:
 
     def synchronize(self) -> None:
         self._check(self._lib.fs_synchronize(self._ctx))
+
+    # -- kernel timing (fs_config.profile_stages >= 2) ------------------------------------------
+    def kernel_profile(self) -> dict:
+        """Per kernel class: launches, device ms and algorithmic bytes since the last reset."""
+        out = {}
+        for k, name in enumerate(_capi.KERNEL_NAMES):
+            n, ms, by = C.c_int64(), C.c_double(), C.c_double()
+            self._check(self._lib.fs_kernel_profile(self._ctx, k, C.byref(n), C.byref(ms), C.byref(by)))
+            out[name] = {"launches": n.value, "ms": ms.value, "bytes": by.value}
+        return out
+
+    def kernel_profile_reset(self) -> None:
+        self._check(self._lib.fs_kernel_profile_reset(self._ctx))

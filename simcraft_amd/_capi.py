@@ -97,6 +97,19 @@ SIGNATURES = {
     "fs_get_array": (C.c_int, [_P, C.c_char_p, _P]),
     "fs_set_array": (C.c_int, [_P, C.c_char_p, _P]),
     "fs_synchronize": (C.c_int, [_P]),
+    "fs_kernel_profile": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]),
+    "fs_kernel_profile_reset": (C.c_int, [_P]),
+}
+
+KERNEL_NAMES = ["spmv", "mic_fwd", "mic_bwd", "pcg_update", "mic_factor", "p2g", "g2p", "advect", "reconstruct"]
+# HIP kernel symbols behind each class (for matching rocprofv3 traces)
+KERNEL_SYMBOLS = {
+    "spmv": ["k_spmv"], "mic_fwd": ["k_fwd"], "mic_bwd": ["k_bwd"], "pcg_update": ["k_update"],
+    "mic_factor": ["k_factor_sweep"], "p2g": ["k_p2g_records", "k_p2g_gather", "k_keys_nearest", "k_count",
+                                             "k_scatter", "k_sort_bins"],
+    "g2p": ["k_g2p"], "advect": ["k_advect"], "reconstruct": ["k_keys_morton", "k_bin_last", "k_recon_splat",
+                                                               "k_recon_final"],
 }
 
 _lib = None

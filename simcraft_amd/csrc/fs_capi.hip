@@ -36,6 +36,37 @@ int fs_zero(fs_ctx* c, DBuf& b) {
   return FS_OK;
 }
 
+int fs_prof_begin(fs_ctx* c, int kind, double bytes) {
+  if (c->cfg.profile_stages < 2 || !c->pev_ok) return -1;
+  if (c->pev_used + 2 > fs_ctx::NPROF_EV) {
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
+    fs_prof_flush(c);
+  }
+  int slot = c->pev_used / 2;
+  c->pev_used += 2;
+  c->pev_kind[slot] = kind;
+  c->pev_bytes[slot] = bytes;
+  (void)hipEventRecord(c->pev[2 * slot], c->stream);
+  return slot;
+}
+
+void fs_prof_end(fs_ctx* c, int slot) {
+  if (slot < 0) return;
+  (void)hipEventRecord(c->pev[2 * slot + 1], c->stream);
+}
+
+void fs_prof_flush(fs_ctx* c) {
+  for (int q = 0; q < c->pev_used / 2; q++) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->pev[2 * q], c->pev[2 * q + 1]) != hipSuccess) continue;
+    int k = c->pev_kind[q];
+    c->kprof_launches[k] += 1;
+    c->kprof_ms[k] += ms;
+    c->kprof_bytes[k] += c->pev_bytes[q];
+  }
+  c->pev_used = 0;
+}
+
 namespace {
 
 thread_local std::string g_create_error;
@@ -52,7 +83,8 @@ void release_all(fs_ctx* c) {
     for (DBuf* b : {&c->g[a], &c->gbuf[a], &c->gvalid[a], &c->gvalidbuf[a], &c->gw[a], &c->glast[a], &c->gweights[a]})
       fs_free(*b);
   for (DBuf* b : {&c->fsdf, &c->fsdfbuf, &c->sdfvalid, &c->sdfvalidbuf, &c->collider, &c->pg, &c->adiag, &c->rhs,
-                  &c->active, &c->keys, &c->slots, &c->perm, &c->counts, &c->offs, &c->scan_tmp, &c->red})
+                  &c->active, &c->keys, &c->slots, &c->perm, &c->counts, &c->offs, &c->scan_tmp, &c->red,
+                  &c->recon_tmp, &c->sorted[0], &c->sorted[1], &c->sorted[2], &c->sorted[3], &c->p2g_w})
     fs_free(*b);
   for (int q = 0; q < 6; q++) fs_free(c->an[q]);
   fs_pcg_release(c);
@@ -247,6 +279,11 @@ int fs_create(const fs_config* cfg, fs_ctx** out) {
   bool ok = true;
   for (int q = 0; q < 12 && ok; q++) ok = hipEventCreate(&c->ev[q]) == hipSuccess;
   c->ev_ok = ok;
+  if (cfg->profile_stages >= 2) {
+    ok = true;
+    for (int q = 0; q < fs_ctx::NPROF_EV && ok; q++) ok = hipEventCreate(&c->pev[q]) == hipSuccess;
+    c->pev_ok = ok;
+  }
   *out = c;
   return FS_OK;
 }
@@ -257,6 +294,8 @@ void fs_destroy(fs_ctx* c) {
   release_all(c);
   if (c->ev_ok)
     for (int q = 0; q < 12; q++) (void)hipEventDestroy(c->ev[q]);
+  if (c->pev_ok)
+    for (int q = 0; q < fs_ctx::NPROF_EV; q++) (void)hipEventDestroy(c->pev[q]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -292,6 +331,8 @@ int fs_set_particles(fs_ctx* c, const double* pos, const double* vel, int64_t n)
     fs_free(c->keys);
     fs_free(c->slots);
     fs_free(c->perm);
+    for (int q = 0; q < 4; q++) fs_free(c->sorted[q]);
+    fs_free(c->p2g_w);
     c->np = 0;
     for (int d = 0; d < 3; d++) {
       FS_TRY(fs_alloc(c, c->pos[d], (size_t)n * c->rsz));
@@ -301,6 +342,8 @@ int fs_set_particles(fs_ctx* c, const double* pos, const double* vel, int64_t n)
     FS_TRY(fs_alloc(c, c->keys, (size_t)n * sizeof(int)));
     FS_TRY(fs_alloc(c, c->slots, (size_t)n * sizeof(int)));
     FS_TRY(fs_alloc(c, c->perm, (size_t)n * sizeof(int)));
+    for (int q = 0; q < 4; q++) FS_TRY(fs_alloc(c, c->sorted[q], (size_t)n * c->rsz));
+    FS_TRY(fs_alloc(c, c->p2g_w, (size_t)n * 12 * c->rsz));
     c->np = n;
   }
   return fs_particles_upload(c, pos, vel);
@@ -558,6 +601,30 @@ int fs_set_array(fs_ctx* c, const char* name, const void* in) {
 int fs_synchronize(fs_ctx* c) {
   FS_TRY(check_ctx(c));
   FS_HIP(c, hipStreamSynchronize(c->stream));
+  fs_prof_flush(c);
+  return FS_OK;
+}
+
+int fs_kernel_profile(fs_ctx* c, int32_t kind, int64_t* launches, double* total_ms, double* total_bytes) {
+  FS_TRY(check_ctx(c));
+  if (kind < 0 || kind >= FS_KERNEL_COUNT) return FS_BAD_ARG;
+  FS_HIP(c, hipStreamSynchronize(c->stream));
+  fs_prof_flush(c);
+  if (launches) *launches = c->kprof_launches[kind];
+  if (total_ms) *total_ms = c->kprof_ms[kind];
+  if (total_bytes) *total_bytes = c->kprof_bytes[kind];
+  return FS_OK;
+}
+
+int fs_kernel_profile_reset(fs_ctx* c) {
+  FS_TRY(check_ctx(c));
+  FS_HIP(c, hipStreamSynchronize(c->stream));
+  fs_prof_flush(c);
+  for (int k = 0; k < FS_KERNEL_COUNT; k++) {
+    c->kprof_launches[k] = 0;
+    c->kprof_ms[k] = 0;
+    c->kprof_bytes[k] = 0;
+  }
   return FS_OK;
 }
 

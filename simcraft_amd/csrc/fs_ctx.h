@@ -71,10 +71,10 @@ struct PcgState {
   DBuf partials;                 // double[max(nchunks, nlive)]
   DBuf partials2;                // second partial array (max reductions)
   DBuf scal;                     // PcgScalars
-  DBuf progress;                 // uint64[3*nlive]: factor/fwd/bwd progress per brick
-  DBuf tickets;                  // uint64[3]
+  DBuf xf, xb;                   // boundary exchange (forward/factor, backward): [2][nlive][S][8] + zero row
+  int sweep_lag = 0;             // extra chunks a sweep lets its producer get ahead after a wait
+  DBuf tickets;                  // uint64[3]: brick tickets of the factor, forward and backward sweeps
   uint64_t ticket_base[3] = {0, 0, 0};
-  uint64_t epoch[3] = {0, 0, 0};
   PcgScalars* host_scal = nullptr;  // pinned
   bool built = false;
   // mixed-precision copies
@@ -106,12 +106,27 @@ struct fs_ctx {
   DBuf adiag, an[6], rhs, active;
   // binning
   DBuf keys, slots, perm, counts, offs, scan_tmp;
+  DBuf sorted[4];   // bin-ordered velocity component (P2G); spare particle-sized scratch
+  DBuf p2g_w;       // 12 kernel factors per particle in bin order (P2G)
+  DBuf recon_tmp;   // per-cell squared-distance minimum (reconstruction)
   size_t scan_tmp_bytes = 0;
   size_t bins_cap = 0;
   DBuf red;  // small reduction scratch
   PcgState pcg;
   hipEvent_t ev[12];
   bool ev_ok = false;
+  // kernel timing (profile_stages >= 2): event pairs recorded around launches, resolved at the
+  // next synchronisation point
+  static constexpr int NPROF_EV = 256;
+  hipEvent_t pev[NPROF_EV];
+  int pev_kind[NPROF_EV / 2];
+  double pev_bytes[NPROF_EV / 2];
+  int pev_used = 0;
+  bool pev_ok = false;
+  int64_t kprof_launches[FS_KERNEL_COUNT] = {};
+  double kprof_ms[FS_KERNEL_COUNT] = {};
+  double kprof_bytes[FS_KERNEL_COUNT] = {};
+  int64_t n_active_last = 0;
 
   void set_error(const std::string& s) { err = s; }
 };
@@ -149,3 +164,8 @@ int fs_upload_real(fs_ctx* c, DBuf& dst, const double* src, size_t n);
 int fs_download_real(fs_ctx* c, const DBuf& src, double* dst, size_t n);
 
 inline unsigned fs_blocks(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// Kernel timing helpers (active when cfg.profile_stages >= 2).
+int fs_prof_begin(fs_ctx* c, int kind, double bytes);  // returns a slot or -1
+void fs_prof_end(fs_ctx* c, int slot);
+void fs_prof_flush(fs_ctx* c);  // resolves recorded pairs; call after a stream synchronisation

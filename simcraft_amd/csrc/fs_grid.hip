@@ -283,37 +283,67 @@ __global__ void k_sort_bins(int nb, const int* __restrict__ offs, int* __restric
   }
 }
 
+// Bin-ordered particle records for the P2G gather of face grid `axis`: the velocity component
+// and the 12 one-dimensional kernel factors N(|p_c - x_node|/h_c) of the 4 nodes per axis
+// around TGrid::nearest. A particle in bin b (its nearest node) meets node n at offset
+// n - b + 1 on each axis, so the node weight CubicKernel::weight(h, p - pn) is
+// (W[ox] * W[4+oy]) * W[8+oz], the same product the reference forms for that pair.
+template <typename T>
+__global__ void k_p2g_records(int n, int axis, GridArgs<T> ga, const int* __restrict__ perm, const T* __restrict__ px,
+                              const T* __restrict__ py, const T* __restrict__ pz, const T* __restrict__ vc,
+                              T* __restrict__ W, T* __restrict__ sv) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int p = perm[t];
+  const T pp[3] = {px[p], py[p], pz[p]};
+  sv[t] = vc[p];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const T off = (c == axis) ? T(0) : T(0.5);
+    const int nb = nearest1(pp[c], off, ga.h[c]);
+#pragma unroll
+    for (int o = 0; o < 4; o++) {
+      const T pn = coord1(nb - 1 + o, off, ga.h[c]);
+      W[(int64_t)(4 * c + o) * n + t] = bspline(fabs(pp[c] - pn) / ga.h[c]);
+    }
+  }
+}
+
 // P2G gather for one face grid (FlipAdvectionSolver3D::solveP2G, advect-solver.cc:188-271):
 // node n sums v*w and w over particles whose nearest node b satisfies n-b in {-1..2}^3,
 // visiting bins b in lexicographic order and particles by index; then ug /= uw or 0.
 template <typename T>
-__global__ void k_p2g_gather(int axis, GridArgs<T> ga, int gx, int gy, int gz, const int* __restrict__ offs,
-                             const int* __restrict__ perm, const T* __restrict__ px, const T* __restrict__ py,
-                             const T* __restrict__ pz, const T* __restrict__ vc, T* __restrict__ G,
-                             T* __restrict__ Wt, uint8_t* __restrict__ valid) {
+__global__ void k_p2g_gather(GridArgs<T> ga, int gx, int gy, int gz, int n, const int* __restrict__ offs,
+                             const T* __restrict__ W, const T* __restrict__ vc, T* __restrict__ G, T* __restrict__ Wt,
+                             uint8_t* __restrict__ valid) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= gx * gy * gz) return;
   int x, y, z;
   unlin3(id, gy, gz, x, y, z);
   const int bx = ga.nx + 1, by = ga.ny + 1, bz = ga.nz + 1;
-  T o0 = axis == 0 ? T(0) : T(0.5), o1 = axis == 1 ? T(0) : T(0.5), o2 = axis == 2 ? T(0) : T(0.5);
-  T pnx = coord1(x, o0, ga.h[0]), pny = coord1(y, o1, ga.h[1]), pnz = coord1(z, o2, ga.h[2]);
   T acc = T(0), ws = T(0);
-  int x0 = max(x - 2, 0), x1 = min(x + 1, bx - 1);
-  int y0 = max(y - 2, 0), y1 = min(y + 1, by - 1);
-  int z0 = max(z - 2, 0), z1 = min(z + 1, bz - 1);
-  for (int b0 = x0; b0 <= x1; b0++)
+  const int x0 = max(x - 2, 0), x1 = min(x + 1, bx - 1);
+  const int y0 = max(y - 2, 0), y1 = min(y + 1, by - 1);
+  const int z0 = max(z - 2, 0), z1 = min(z + 1, bz - 1);
+  const T* Wx = W;
+  const T* Wy = W + (int64_t)4 * n;
+  const T* Wz = W + (int64_t)8 * n;
+  for (int b0 = x0; b0 <= x1; b0++) {
+    const T* wx = Wx + (int64_t)(x - b0 + 1) * n;
     for (int b1 = y0; b1 <= y1; b1++) {
-      int row = bz * (b1 + by * b0);
-      int s = offs[row + z0], e = offs[row + z1 + 1];  // bins of one z-run are contiguous
-      for (int t = s; t < e; t++) {
-        int p = perm[t];
-        T w = bspline(fabs(px[p] - pnx) / ga.h[0]) * bspline(fabs(py[p] - pny) / ga.h[1]) *
-              bspline(fabs(pz[p] - pnz) / ga.h[2]);
-        acc = acc + vc[p] * w;
-        ws = ws + w;
+      const T* wy = Wy + (int64_t)(y - b1 + 1) * n;
+      const int row = bz * (b1 + by * b0);
+      for (int b2 = z0; b2 <= z1; b2++) {
+        const T* wz = Wz + (int64_t)(z - b2 + 1) * n;
+        const int s = offs[row + b2], e = offs[row + b2 + 1];
+        for (int t = s; t < e; t++) {
+          const T w = wx[t] * wy[t] * wz[t];
+          acc = acc + vc[t] * w;
+          ws = ws + w;
+        }
       }
     }
+  }
   if (ws > T(0)) {
     G[id] = acc / ws;
     valid[id] = 1;
@@ -324,50 +354,109 @@ __global__ void k_p2g_gather(int axis, GridArgs<T> ga, int gx, int gy, int gz, c
   Wt[id] = ws;
 }
 
-// NaiveReconstructor::reconstruct (FluidSim/include/FluidSim/cpu/sdf.h:210-225) with
-// forNeighbours (ns-util.h:80-109). Each bin's last particle is skipped (inclusive end used
-// as exclusive bound). Since sqrt and x-radius are monotone, min(|x_c-x_p|-r) over the
-// included particles equals sqrt(min |x_c-x_p|^2) - r; the inclusion test |d| <= 2r is
-// d^2 <= s_max with s_max the largest double whose rounded sqrt is <= 2r.
+// NaiveReconstructor::reconstruct (FluidSim/include/FluidSim/cpu/sdf.h:210-225) as a particle
+// splat. The reference visits, for each cell, the particles of the Morton bins in
+// [x_min,x_max]x[y_min,y_max]x[z_min,z_max] (forNeighbours, ns-util.h:80-109) except the last
+// particle of each bin, keeps those within 2r and sets sdf = min(|x_c - x_p| - r). The minimum
+// is order-free and both sqrt and x - r are monotone, so every particle instead lowers a
+// per-cell minimum of the squared distance (atomicMin on the bit pattern, which orders like
+// the value for non-negative floats); every inclusion test is the reference's, evaluated with
+// the reference's arithmetic: the bin ranges of the cell, and |d| <= 2r as d^2 <= s_max where
+// s_max is the largest float whose rounded sqrt is <= 2r.
 template <typename T>
-__global__ void k_reconstruct(GridArgs<T> ga, const int* __restrict__ offs, const int* __restrict__ perm,
-                              const T* __restrict__ px, const T* __restrict__ py, const T* __restrict__ pz,
-                              T radius, T r2, T s_max, T* __restrict__ sdf, uint8_t* __restrict__ valid) {
+struct Bits;
+template <>
+struct Bits<double> {
+  using U = unsigned long long;
+  static constexpr U inf = 0x7FF0000000000000ull;
+  __device__ static U of(double v) { return (U)__double_as_longlong(v); }
+  __device__ static double val(U b) { return __longlong_as_double((long long)b); }
+};
+template <>
+struct Bits<float> {
+  using U = unsigned int;
+  static constexpr U inf = 0x7F800000u;
+  __device__ static U of(float v) { return (U)__float_as_uint(v); }
+  __device__ static float val(U b) { return __uint_as_float(b); }
+};
+
+// The last particle of a Morton bin in the stable order is the one with the largest index.
+__global__ void k_bin_last(int n, const int* __restrict__ keys, int* __restrict__ binmax) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  atomicMax(&binmax[keys[i]], i);
+}
+
+template <typename T>
+__global__ void k_fill_bits(int n, typename Bits<T>::U* __restrict__ d) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = Bits<T>::inf;
+}
+
+template <typename T>
+__global__ void k_recon_splat(int n, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
+                              const T* __restrict__ pz, const int* __restrict__ keys, const int* __restrict__ binmax,
+                              T r2, T s_max, typename Bits<T>::U* __restrict__ dmin) {
+  using UB = typename Bits<T>::U;
   int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= ga.nx * ga.ny * ga.nz) return;
-  int i, j, k;
-  unlin3(id, ga.ny, ga.nz, i, j, k);
-  T cx = coord1(i, T(0.5), ga.h[0]), cy = coord1(j, T(0.5), ga.h[1]), cz = coord1(k, T(0.5), ga.h[2]);
-  int x_min = max(0, (int)floor((cx - r2) / ga.h[0]));
-  int x_max = min(ga.nx - 1, (int)ceil((cx + r2) / ga.h[0]));
-  int y_min = max(0, (int)floor((cy - r2) / ga.h[1]));
-  int y_max = min(ga.ny - 1, (int)ceil((cy + r2) / ga.h[1]));
-  int z_min = max(0, (int)floor((cz - r2) / ga.h[2]));
-  int z_max = min(ga.nz - 1, (int)ceil((cz + r2) / ga.h[2]));
-  T dmin = T(0);
-  bool found = false;
-  for (int a = x_min; a <= x_max; a++)
-    for (int b = y_min; b <= y_max; b++)
-      for (int c = z_min; c <= z_max; c++) {
-        int m = morton3D(a, b, c);
-        int s = offs[m], e = offs[m + 1] - 1;
-        for (int t = s; t < e; t++) {
-          int p = perm[t];
-          T dx = cx - px[p], dy = cy - py[p], dz = cz - pz[p];
-          T d2 = dx * dx + dy * dy + dz * dz;
-          if (d2 <= s_max) {
-            if (!found || d2 < dmin) dmin = d2;
-            found = true;
-          }
-        }
-      }
-  T out = T(1e9);
-  if (found) {
-    T v = sqrt(dmin) - radius;
-    out = smin(out, v);
+  if (id >= n) return;
+  if (binmax[keys[id]] == id) return;  // forNeighbours stops one short of the inclusive bin end
+  const T p[3] = {px[id], py[id], pz[id]};
+  const int dims[3] = {ga.nx, ga.ny, ga.nz};
+  int bin[3], lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    int bb = (int)floor(p[d] / ga.h[d]);  // ParticleNeighbourSearcher::update (ns-util.h:52-60)
+    bb = bb < 0 ? 0 : bb;
+    bin[d] = bb >= dims[d] ? dims[d] - 1 : bb;
+    lo[d] = max(0, (int)floor((p[d] - r2) / ga.h[d] - T(0.5)) - 1);
+    hi[d] = min(dims[d] - 1, (int)floor((p[d] + r2) / ga.h[d] - T(0.5)) + 1);
   }
-  sdf[id] = out;
-  valid[id] = found ? 1 : 0;
+  // cell c keeps bin b on axis d when floor((x_c - 2r)/h) <= b <= ceil((x_c + 2r)/h) (clamped)
+  auto in_range = [&](int d, T xc) {
+    int mn = max(0, (int)floor((xc - r2) / ga.h[d]));
+    int mx = min(dims[d] - 1, (int)ceil((xc + r2) / ga.h[d]));
+    return bin[d] >= mn && bin[d] <= mx;
+  };
+  for (int i = lo[0]; i <= hi[0]; i++) {
+    const T xc = coord1(i, T(0.5), ga.h[0]);
+    const T dx = xc - p[0];
+    const T dx2 = dx * dx;
+    if (dx2 > s_max || !in_range(0, xc)) continue;
+    for (int j = lo[1]; j <= hi[1]; j++) {
+      const T yc = coord1(j, T(0.5), ga.h[1]);
+      const T dy = yc - p[1];
+      const T dxy = dx2 + dy * dy;
+      if (dxy > s_max || !in_range(1, yc)) continue;
+      for (int k = lo[2]; k <= hi[2]; k++) {
+        const T zc = coord1(k, T(0.5), ga.h[2]);
+        const T dz = zc - p[2];
+        const T d2 = dxy + dz * dz;
+        if (d2 > s_max || !in_range(2, zc)) continue;
+        UB* cell = dmin + lin3(i, j, k, ga.ny, ga.nz);
+        const UB b = Bits<T>::of(d2);
+        if (b < __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(cell, b);
+      }
+    }
+  }
+}
+
+// sdf.grid.fill(1e9); sdfValid.fill(false); then min(sdf, |d| - r) for the kept particles.
+template <typename T>
+__global__ void k_recon_final(int ncell, const typename Bits<T>::U* __restrict__ dmin, T radius, T* __restrict__ sdf,
+                              uint8_t* __restrict__ valid) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const typename Bits<T>::U b = dmin[c];
+  T out = T(1e9);
+  uint8_t v = 0;
+  if (b != Bits<T>::inf) {
+    const T dis = sqrt(Bits<T>::val(b)) - radius;
+    out = smin(out, dis);
+    v = 1;
+  }
+  sdf[c] = out;
+  valid[c] = v;
 }
 
 // extrapolateFluidSdf iteration (cpu/fluid-simulator.h:124-164): valid cells are not copied.
@@ -797,30 +886,44 @@ int advect_t(fs_ctx* c, double dt) {
   int n = (int)c->np;
   if (n == 0) return FS_OK;
   GridArgs<T> ga = grid_args<T>(c);
+  const int pf = fs_prof_begin(c, FS_KERNEL_ADVECT,
+                               (double)n * 12 * sizeof(T) + (double)(c->n_face[0] + c->n_face[1] + c->n_face[2]) * sizeof(T));
   k_advect<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, grids_of<T>(c, c->g), c->collider.as<T>(),
                                                         c->pos[0].as<T>(), c->pos[1].as<T>(), c->pos[2].as<T>(),
                                                         c->vel[0].as<T>(), c->vel[1].as<T>(), c->vel[2].as<T>(), (T)dt);
+  fs_prof_end(c, pf);
   FS_HIP(c, hipGetLastError());
   return FS_OK;
 }
 
 template <typename T>
 int reconstruct_t(fs_ctx* c, double radius) {
+  using UB = typename Bits<T>::U;
   GridArgs<T> ga = grid_args<T>(c);
   int n = (int)c->np;
   int nb = morton3D(c->nx - 1, c->ny - 1, c->nz - 1) + 1;
-  if (n > 0)
-    k_keys_morton<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
-                                                               c->pos[2].as<T>(), c->keys.as<int>());
-  FS_TRY(counting_sort(c, nb));
+  FS_TRY(ensure_bins(c, (size_t)nb));
+  if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)c->n_cell * sizeof(double)));
+  UB* dmin = c->recon_tmp.as<UB>();
+  k_fill_bits<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->n_cell, dmin);
   T rad = (T)radius;
   T r2 = T(2) * rad;
   T smax;
   if (sizeof(T) == 8) smax = (T)smax_for((double)r2);
   else smax = (T)smax_for_f((float)r2);
-  k_reconstruct<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(
-      ga, c->offs.as<int>(), c->perm.as<int>(), c->pos[0].as<T>(), c->pos[1].as<T>(), c->pos[2].as<T>(), rad, r2,
-      smax, c->fsdf.as<T>(), c->sdfvalid.as<uint8_t>());
+  const int pf = fs_prof_begin(c, FS_KERNEL_RECONSTRUCT, (double)n * 3 * sizeof(T) + (double)c->n_cell * (sizeof(T) + 1));
+  if (n > 0) {
+    FS_HIP(c, hipMemsetAsync(c->counts.p, 0xFF, (size_t)nb * sizeof(int), c->stream));
+    k_keys_morton<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
+                                                               c->pos[2].as<T>(), c->keys.as<int>());
+    k_bin_last<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->counts.as<int>());
+    k_recon_splat<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
+                                                               c->pos[2].as<T>(), c->keys.as<int>(),
+                                                               c->counts.as<int>(), r2, smax, dmin);
+  }
+  k_recon_final<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->n_cell, dmin, rad, c->fsdf.as<T>(),
+                                                                     c->sdfvalid.as<uint8_t>());
+  fs_prof_end(c, pf);
   FS_HIP(c, hipGetLastError());
   return FS_OK;
 }
@@ -858,11 +961,18 @@ int p2g_t(fs_ctx* c) {
     if (n > 0)
       k_keys_nearest<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, a, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
                                                                   c->pos[2].as<T>(), c->keys.as<int>());
+    const int pf = fs_prof_begin(c, FS_KERNEL_P2G, (double)n * 4 * sizeof(T) + (double)c->n_face[a] * 2 * sizeof(T));
     FS_TRY(counting_sort(c, nb));
+    if (n > 0)
+      k_p2g_records<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, a, ga, c->perm.as<int>(), c->pos[0].as<T>(),
+                                                                 c->pos[1].as<T>(), c->pos[2].as<T>(),
+                                                                 c->vel[a].as<T>(), c->p2g_w.as<T>(),
+                                                                 c->sorted[0].as<T>());
     int gx = c->fdim[a][0], gy = c->fdim[a][1], gz = c->fdim[a][2];
     k_p2g_gather<T><<<fs_blocks(c->n_face[a], TPB), TPB, 0, c->stream>>>(
-        a, ga, gx, gy, gz, c->offs.as<int>(), c->perm.as<int>(), c->pos[0].as<T>(), c->pos[1].as<T>(),
-        c->pos[2].as<T>(), c->vel[a].as<T>(), c->g[a].as<T>(), c->gw[a].as<T>(), c->gvalid[a].as<uint8_t>());
+        ga, gx, gy, gz, n, c->offs.as<int>(), c->p2g_w.as<T>(), c->sorted[0].as<T>(), c->g[a].as<T>(),
+        c->gw[a].as<T>(), c->gvalid[a].as<uint8_t>());
+    fs_prof_end(c, pf);
     FS_HIP(c, hipGetLastError());
     if (c->cfg.advection == FS_ADVECT_FLIP)
       FS_HIP(c, hipMemcpyAsync(c->glast[a].p, c->g[a].p, (size_t)c->n_face[a] * sizeof(T), hipMemcpyDeviceToDevice,
@@ -949,7 +1059,11 @@ int g2p_t(fs_ctx* c) {
   int n = (int)c->np;
   if (n == 0) return FS_OK;
   GridArgs<T> ga = grid_args<T>(c);
-  if (c->cfg.advection == FS_ADVECT_FLIP)
+  const bool flip = c->cfg.advection == FS_ADVECT_FLIP;
+  const int pf = fs_prof_begin(c, FS_KERNEL_G2P,
+                               (double)n * 9 * sizeof(T) +
+                                   (double)(c->n_face[0] + c->n_face[1] + c->n_face[2]) * (flip ? 2 : 1) * sizeof(T));
+  if (flip)
     k_g2p<T, true><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(
         n, ga, grids_of<T>(c, c->g), grids_of<T>(c, c->glast), c->pos[0].as<T>(), c->pos[1].as<T>(),
         c->pos[2].as<T>(), c->vel[0].as<T>(), c->vel[1].as<T>(), c->vel[2].as<T>(), (T)c->cfg.flip_alpha);
@@ -957,6 +1071,7 @@ int g2p_t(fs_ctx* c) {
     k_g2p<T, false><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(
         n, ga, grids_of<T>(c, c->g), grids_of<T>(c, c->g), c->pos[0].as<T>(), c->pos[1].as<T>(), c->pos[2].as<T>(),
         c->vel[0].as<T>(), c->vel[1].as<T>(), c->vel[2].as<T>(), (T)c->cfg.flip_alpha);
+  fs_prof_end(c, pf);
   FS_HIP(c, hipGetLastError());
   return FS_OK;
 }

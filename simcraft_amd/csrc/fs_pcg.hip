@@ -7,10 +7,11 @@
 // (i,j,k) sits in lane a*8+b (a=i%8, b=j%8) of slot k+a+b. The skew a+b turns the triangular
 // sweeps into one wavefront per brick: at step s every lane of the wave handles slot s, and
 // its (i-1) and (j-1) neighbours were finished by lanes a-1 / b-1 at step s-1, so they arrive
-// by a lane shuffle. Bricks hand their boundary rows to the next brick through global memory
-// with release/acquire progress counters; bricks are scheduled by ticket in dependency order,
-// so every wait is on a brick that is already running. Each slot reads/writes one contiguous
-// 512-byte row per array.
+// by a lane shuffle. Bricks hand their boundary rows to the next brick through small exchange
+// arrays whose entries double as ready flags (see the boundary exchange below); bricks are
+// scheduled by ticket in dependency order, so every wait is on a brick that is already
+// running. Each slot reads/writes one contiguous 512-byte row per array, and a brick streams
+// its slots in chunks of U with the next chunk's loads in flight.
 //
 // Exactness: every cell does the reference's arithmetic in the reference's order. The MIC
 // sweep coefficients are the products the reference forms first (AL(c)*E(i-1) and
@@ -29,7 +30,6 @@ constexpr int NB_SELF = 1, NB_L = 2, NB_D = 4, NB_B = 8, NB_R = 16, NB_U = 32, N
 // factor-sweep flags
 constexpr int TF_DO = 1, TF_GL = 2, TF_L1 = 4, TF_L2 = 8, TF_GD = 16, TF_D1 = 32, TF_D2 = 64, TF_GB = 128,
               TF_B1 = 256, TF_B2 = 512;
-constexpr int PUB = 4;                  // slots between progress publications
 constexpr long SPIN_LIMIT = 1000000;    // bounded wait (~1 s), then the solve reports a device error
 enum { STOP_RUN = 0, STOP_CONVERGED = 1, STOP_ABORT = 2 };
 enum { DC_SPMV = 0, DC_FWD = 1, DC_BWD = 2, DC_INIT = 3, DC_FACTOR = 4, DC_RES = 5 };
@@ -70,11 +70,14 @@ __device__ inline int64_t nb_elem(const Lay& L, int B, int s, int a, int b, int 
   }
 }
 
+// Element -> (brick, slot, lane, cell). Element counts stay below 2^31 (checked on the host),
+// so the division is 32-bit.
 __device__ inline bool decode(const Lay& L, int64_t e, int& B, int& s, int& a, int& b, int& i, int& j, int& k) {
-  int lane = (int)(e & 63);
-  int64_t t = e >> 6;
-  s = (int)(t % L.S);
-  B = (int)(t / L.S);
+  const unsigned ue = (unsigned)e;
+  int lane = (int)(ue & 63u);
+  const unsigned t = ue >> 6;
+  B = (int)(t / (unsigned)L.S);
+  s = (int)(t - (unsigned)B * (unsigned)L.S);
   a = lane >> 3;
   b = lane & 7;
   int2 ij = L.bij[B];
@@ -164,50 +167,94 @@ __device__ inline bool last_block(uint32_t* ctr, uint32_t nblocks, int nthreads)
   return is_last != 0;
 }
 
-// ---- bounded progress wait -------------------------------------------------------------------
-__device__ inline uint64_t wait_progress(const uint64_t* prog, uint64_t target, uint64_t seen, PcgScalars* sc) {
-  if (seen >= target) return seen;
-  uint64_t v = __hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// ---- boundary exchange between bricks ---------------------------------------------------------
+// A brick hands its boundary row to the next brick through a small exchange array. Every entry
+// holds a sentinel until the producer stores the value with an agent-scope atomic store; the
+// consumer polls the entry with agent-scope atomic loads until the sentinel is gone, uses the
+// value and stores the sentinel back. The sentinel is a signalling-NaN bit pattern, which
+// floating-point arithmetic never returns, so every exchanged value is its own ready flag and
+// no fence or cache maintenance is needed. Polls are bounded: after SPIN_LIMIT the wave gives
+// up, counts a timeout and the host reports FS_DEVICE_ERROR.
+constexpr unsigned long long XSENT = 0x7FF4DEADBEEF0001ull;
+constexpr int U = 8;  // slots per chunk; S is a multiple of U
+
+__device__ inline double xload(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void xstore(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline bool xready(double v) { return (unsigned long long)__double_as_longlong(v) != XSENT; }
+__device__ inline double xsent() { return __longlong_as_double((long long)XSENT); }
+
+// Wait until *p holds a written value; false when the wait gave up (the solve then reports
+// FS_DEVICE_ERROR and every other waiter gives up within 256 polls).
+__device__ inline bool xwait(const double* p, PcgScalars* sc) {
   long spins = 0;
-  while (v < target) {
+  while (!xready(xload(p))) {
     __builtin_amdgcn_s_sleep(1);
-    v = __hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    if ((++spins & 1023) == 0) {
+    if ((++spins & 255) == 0) {
       int to = __hip_atomic_load(&sc->wait_timeouts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (to != 0 || spins > SPIN_LIMIT) {
-        if (threadIdx.x == 0) atomicAdd(&sc->wait_timeouts, 1);
-        return target;  // give up: results are invalid and the solve reports FS_DEVICE_ERROR
+        atomicAdd(&sc->wait_timeouts, 1);
+        return false;
+      }
+    }
+  }
+  return true;
+}
+
+__device__ inline double xpoll(const double* p, double v, PcgScalars* sc) {
+  long spins = 0;
+  while (!xready(v)) {
+    __builtin_amdgcn_s_sleep(1);
+    v = xload(p);
+    if ((++spins & 255) == 0) {
+      int to = __hip_atomic_load(&sc->wait_timeouts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (to != 0 || spins > SPIN_LIMIT) {
+        atomicAdd(&sc->wait_timeouts, 1);
+        return 0.0;  // give up: results are invalid and the solve reports FS_DEVICE_ERROR
       }
     }
   }
   return v;
 }
-__device__ inline void publish(uint64_t* prog, uint64_t v) {
-  __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(prog, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+
+__global__ void k_fill_sentinel(int64_t n, double* __restrict__ x) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = __longlong_as_double((long long)XSENT);
 }
+
 __device__ inline int take_ticket(uint64_t* ctr, uint64_t base, const int* order, int n) {
   __shared__ int sB;
   if (threadIdx.x == 0) {
     uint64_t t = atomicAdd((unsigned long long*)ctr, 1ull) - base;
     // a ticket outside [0, n) can only follow a failed launch; clamp it so every access
-    // stays in bounds (the dependency waits then time out and the solve reports an error)
+    // stays in bounds (the exchange polls then time out and the solve reports an error)
     sB = order[t < (uint64_t)n ? t : (uint64_t)(n - 1)];
   }
   __syncthreads();
   return sB;
 }
 
+// Exchange entry of brick B, face f (0: i-face, 1: j-face), slot s, index w (0..7).
+__device__ inline double* xent(double* X, const Lay& L, int f, int B, int s, int w) {
+  return X + (((size_t)f * L.nlive + B) * L.S + s) * 8 + w;
+}
+
 // ---- setup kernels ------------------------------------------------------------------------------
 __global__ void k_brick_flags(int nx, int ny, int nz, const uint8_t* __restrict__ active, int* __restrict__ flag,
-                              int nbj) {
+                              int nbj, int* __restrict__ nact) {
   int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= nx * ny) return;
   int i = row / ny, j = row % ny;
   const uint8_t* a = active + (size_t)row * nz;
-  int any = 0;
-  for (int k = 0; k < nz; k++) any |= a[k];
-  if (any) atomicOr(&flag[(i / 8) * nbj + (j / 8)], 1);
+  int cnt = 0;
+  for (int k = 0; k < nz; k++) cnt += a[k] != 0;
+  if (cnt) {
+    atomicOr(&flag[(i / 8) * nbj + (j / 8)], 1);
+    atomicAdd(nact, cnt);
+  }
 }
 
 // Pack the dense system into the brick layout; neighbour-activity masks; factor flags and
@@ -296,57 +343,87 @@ __global__ void k_pack(Lay L, double tau, const double* __restrict__ Adiag, cons
 }
 
 // E factor sweep (project-solver.cc:237-274), one wave per brick, tickets in dependency order.
+// Chunks of U slots: the chunk's inputs are loaded together, then the U steps run.
 __global__ __launch_bounds__(64) void k_factor_sweep(Lay L, double sigma, const double* __restrict__ adiag,
                                                      const double* __restrict__ al, const double* __restrict__ ad,
                                                      const double* __restrict__ ab, const uint16_t* __restrict__ tfl,
                                                      const double* __restrict__ tnum, double* __restrict__ E,
-                                                     uint64_t* ticket, uint64_t tbase, const int* __restrict__ order,
-                                                     uint64_t* progress, uint64_t pbase, PcgScalars* sc) {
+                                                     double* X, uint64_t* ticket, uint64_t tbase,
+                                                     const int* __restrict__ order, PcgScalars* sc) {
   const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
   const int B = take_ticket(ticket, tbase, order, L.nlive);
   const int2 ij = L.bij[B];
   const int BL = ij.x > 0 ? L.live[(ij.x - 1) * L.nbj + ij.y] : -1;
   const int BD = ij.y > 0 ? L.live[ij.x * L.nbj + ij.y - 1] : -1;
-  uint64_t seenL = 0, seenD = 0;
+  const int S = L.S;
+  const bool cL = a == 0 && BL >= 0, cD = b == 0 && BD >= 0;
+  const int64_t base = (int64_t)B * S * 64 + lane;
   double eprev = 0.0;
-  for (int s = 0; s < L.S; s++) {
-    const int64_t e = eidx(L, B, s, lane);
-    const int need = min(s + 8, L.S);
-    double EL = __shfl_up(eprev, 8, 64);
-    double ED = __shfl_up(eprev, 1, 64);
-    if (BL >= 0) seenL = wait_progress(progress + BL, pbase + need, seenL, sc);
-    if (BD >= 0) seenD = wait_progress(progress + BD, pbase + need, seenD, sc);
-    if (a == 0) EL = (BL >= 0 && s + 7 < L.S) ? E[eidx(L, BL, s + 7, 56 + b)] : 0.0;
-    if (b == 0) ED = (BD >= 0 && s + 7 < L.S) ? E[eidx(L, BD, s + 7, a * 8 + 7)] : 0.0;
-    const int tf = tfl[e];
-    double ev = 0.0;
-    if (tf & TF_DO) {
-      const double dg = adiag[e];
-      double x = dg;
-      if (tf & TF_GL) {
-        double t = al[e] * EL;
-        x -= t * t;
-        if (tf & TF_L1) x -= tnum[e] / (EL * EL);
-        if (tf & TF_L2) x -= tnum[L.ne + e] / (EL * EL);
-      }
-      if (tf & TF_GD) {
-        double t = ad[e] * ED;
-        x -= t * t;
-        if (tf & TF_D1) x -= tnum[2 * L.ne + e] / (ED * ED);
-        if (tf & TF_D2) x -= tnum[3 * L.ne + e] / (ED * ED);
-      }
-      if (tf & TF_GB) {
-        double t = ab[e] * eprev;
-        x -= t * t;
-        if (tf & TF_B1) x -= tnum[4 * L.ne + e] / (eprev * eprev);
-        if (tf & TF_B2) x -= tnum[5 * L.ne + e] / (eprev * eprev);
-      }
-      if (x < sigma * dg) x = dg;
-      ev = 1.0 / sqrt(x);
+  for (int s0 = 0; s0 < S; s0 += U) {
+    int tf[U];
+    double dg[U], vl[U], vd[U], vb[U], xl[U], xd[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t e = base + (int64_t)(s0 + u) * 64;
+      tf[u] = tfl[e];
+      dg[u] = adiag[e];
+      vl[u] = al[e];
+      vd[u] = ad[e];
+      vb[u] = ab[e];
+      const int s = s0 + u;
+      xl[u] = (cL && s + 7 < S) ? xload(xent(X, L, 0, BL, s + 7, b)) : 0.0;
+      xd[u] = (cD && s + 7 < S) ? xload(xent(X, L, 1, BD, s + 7, a)) : 0.0;
     }
-    E[e] = ev;
-    eprev = ev;
-    if ((s + 1) % PUB == 0 || s == L.S - 1) publish(progress + B, pbase + s + 1);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int s = s0 + u;
+      if (cL && s + 7 < S) xl[u] = xpoll(xent(X, L, 0, BL, s + 7, b), xl[u], sc);
+      if (cD && s + 7 < S) xd[u] = xpoll(xent(X, L, 1, BD, s + 7, a), xd[u], sc);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int s = s0 + u;
+      const int64_t e = base + (int64_t)s * 64;
+      double EL = __shfl_up(eprev, 8, 64);
+      double ED = __shfl_up(eprev, 1, 64);
+      if (a == 0) EL = xl[u];
+      if (b == 0) ED = xd[u];
+      double ev = 0.0;
+      const int f = tf[u];
+      if (f & TF_DO) {
+        double x = dg[u];
+        if (f & TF_GL) {
+          double t = vl[u] * EL;
+          x -= t * t;
+          if (f & TF_L1) x -= tnum[e] / (EL * EL);
+          if (f & TF_L2) x -= tnum[L.ne + e] / (EL * EL);
+        }
+        if (f & TF_GD) {
+          double t = vd[u] * ED;
+          x -= t * t;
+          if (f & TF_D1) x -= tnum[2 * L.ne + e] / (ED * ED);
+          if (f & TF_D2) x -= tnum[3 * L.ne + e] / (ED * ED);
+        }
+        if (f & TF_GB) {
+          double t = vb[u] * eprev;
+          x -= t * t;
+          if (f & TF_B1) x -= tnum[4 * L.ne + e] / (eprev * eprev);
+          if (f & TF_B2) x -= tnum[5 * L.ne + e] / (eprev * eprev);
+        }
+        if (x < sigma * dg[u]) x = dg[u];
+        ev = 1.0 / sqrt(x);
+      }
+      E[e] = ev;
+      if (a == 7) xstore(xent(X, L, 0, B, s, b), ev);
+      if (b == 7) xstore(xent(X, L, 1, B, s, a), ev);
+      eprev = ev;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int s = s0 + u;
+      if (cL && s + 7 < S) xstore(xent(X, L, 0, BL, s + 7, b), xsent());
+      if (cD && s + 7 < S) xstore(xent(X, L, 1, BD, s + 7, a), xsent());
+    }
   }
 }
 
@@ -479,49 +556,207 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const double* __restrict__ 
   }
 }
 
+// ---- MIC sweeps ---------------------------------------------------------------------------------
+// One wave per live brick. Lane (a, b) = (lane >> 3, lane & 7) owns column (8I+a, 8J+b) and at
+// slot s handles cell k = s - a - b, so its -x and -y neighbours at the same k were finished one
+// slot earlier by lanes a-1 and b-1 (lane shuffles) or, on the brick face, 7 slots earlier by
+// the neighbouring brick (exchange entries X[f][B][s][w]).
+//
+// The sweeps are bound by instruction issue of their single wave, so every address in the
+// streaming loop is a wave-uniform base in scalar registers plus the lane's fixed offset plus
+// an immediate: chunk c of a brick array starts at element (B*S + c*U)*64 and its element u
+// is at +u*64 + lane. Every lane loads the exchange row of its face; the row ZROW (all zeros,
+// after the entries) stands in for a face without a neighbour or past the last slot, and only
+// the face lanes look at the value. Producer stores and sentinel resets are exec-masked.
+//
+// Two register sets alternate over the chunks: the inputs and boundary entries of chunk c+1 are
+// in flight while chunk c runs, and no set is ever copied (a copy would force the loads to
+// complete early). A chunk whose boundary entries were not all written when loaded is never
+// patched in place (that would merge register values and force every load to complete): the
+// pipeline is abandoned, the wave waits for the producer to get `lag` chunks ahead, and the
+// pipeline restarts at that chunk.
+
+// High word of the sentinel. Arithmetic never returns a signalling NaN, so for computed values
+// the high word alone identifies the sentinel.
+constexpr uint32_t XSENT_HI = (uint32_t)(XSENT >> 32);
+__device__ __forceinline__ bool xunset(double v) { return (uint32_t)__double2hiint(v) == XSENT_HI; }
+
+// v of lane-1 / lane+1 inside 16-lane rows (DPP row_shr:1 / row_shl:1); a lane whose source is
+// outside its row gets 0. Used for the b-1 / b+1 neighbour, which lanes b = 0 / 7 never take.
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double v) {
+  const unsigned long long x = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)x, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(x >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHL1 = 0x101;
+
 // Forward sweep L q = r (project-solver.cc:275-288). UPDATE: first r += (-alpha) t and
 // ||r||_inf (saxpy + LinfNorm, :333-335); the last brick decides convergence.
+struct FwdSet {
+  double r[U], t[U], c1[U], c2[U], c3[U], e[U], xl[U], xd[U];
+};
+
+struct FwdK {  // wave-uniform
+  double* r;
+  const double *t, *cl, *cd, *cb, *E;
+  double* q;        // brick bases: element B*S*64
+  double* xL;       // f 0 entries of the -x neighbour, nullptr when there is none
+  double* xD;       // f 1 entries of the -y neighbour
+  double* oL;       // this brick's f 0 entries (read by its +x neighbour)
+  double* oD;       // this brick's f 1 entries
+  const double* Z;  // zero row
+  int S, NC, lag;
+  double nalpha;
+};
+
+template <bool UPDATE>
+__device__ __forceinline__ void fwd_load(const FwdK& k, FwdSet& S, int c, int lane) {
+  c = min(c, k.NC - 1);  // past the end: reload the last chunk (unused) rather than branch
+  const size_t o = (size_t)c * U * 64;
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int i = u * 64 + lane;
+    S.r[u] = k.r[o + i];
+    S.t[u] = UPDATE ? k.t[o + i] : 0.0;
+    S.c1[u] = k.cl[o + i];
+    S.c2[u] = k.cd[o + i];
+    S.c3[u] = k.cb[o + i];
+    S.e[u] = k.E[o + i];
+  }
+}
+
+__device__ __forceinline__ void fwd_cross(const FwdK& k, FwdSet& S, int c, int a, int b) {
+  c = min(c, k.NC - 1);
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int s = c * U + u;
+    const double* pL = (k.xL != nullptr && s + 7 < k.S) ? k.xL + (size_t)(s + 7) * 8 : k.Z;
+    const double* pD = (k.xD != nullptr && s + 7 < k.S) ? k.xD + (size_t)(s + 7) * 8 : k.Z;
+    S.xl[u] = xload(pL + b);
+    S.xd[u] = xload(pD + a);
+  }
+}
+
+// Wave-uniform: every boundary entry of the chunk had been written when it was loaded.
+__device__ __forceinline__ bool fwd_ready(const FwdSet& S, int a, int b, bool give_up) {
+  bool wait = false;
+#pragma unroll
+  for (int u = 0; u < U; u++) wait |= ((a == 0) & xunset(S.xl[u])) | ((b == 0) & xunset(S.xd[u]));
+  return give_up || __builtin_amdgcn_ballot_w64(wait) == 0;
+}
+
+// Wait until the boundary entries of chunks c .. c+lag are written; true if a wait gave up.
+__device__ __forceinline__ bool fwd_wait(const FwdK& k, int c, int a, int b, PcgScalars* sc) {
+  bool ok = true;
+  const int c1 = min(c + k.lag, k.NC - 1);
+  for (int s = c * U; s < (c1 + 1) * U && s + 7 < k.S; s++) {
+    if (k.xL != nullptr && a == 0) ok = ok && xwait(k.xL + (size_t)(s + 7) * 8 + b, sc);
+    if (k.xD != nullptr && b == 0) ok = ok && xwait(k.xD + (size_t)(s + 7) * 8 + a, sc);
+  }
+  return __builtin_amdgcn_ballot_w64(!ok) != 0;
+}
+
+template <bool UPDATE>
+__device__ __forceinline__ void fwd_run(const FwdK& k, const FwdSet& S, int c, int lane, int a, int b, double& qprev,
+                                        double& mx, double& ss) {
+  const int s0 = c * U;
+  const size_t o = (size_t)s0 * 64;
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int s = s0 + u;
+    const int i = u * 64 + lane;
+    double rr = S.r[u];
+    if (UPDATE) {
+      rr = rr + k.nalpha * S.t[u];
+      k.r[o + i] = rr;
+      const double av = fabs(rr);
+      mx = (mx < av) ? av : mx;
+      ss += rr * rr;
+    }
+    double qL = __shfl_up(qprev, 8, 64);
+    double qD = dpp_row<DPP_ROW_SHR1>(qprev);
+    qL = a == 0 ? S.xl[u] : qL;
+    qD = b == 0 ? S.xd[u] : qD;
+    double tt = rr - S.c1[u] * qL;
+    tt = tt - S.c2[u] * qD;
+    tt = tt - S.c3[u] * qprev;
+    const double qv = tt * S.e[u];
+    k.q[o + i] = qv;
+    if (a == 7) xstore(k.oL + (size_t)s * 8 + b, qv);
+    if (b == 7) xstore(k.oD + (size_t)s * 8 + a, qv);
+    qprev = qv;
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {  // hand the consumed entries back
+    const int s = s0 + u;
+    if (s + 7 < k.S) {
+      if (k.xL != nullptr && a == 0) xstore(k.xL + (size_t)(s + 7) * 8 + b, xsent());
+      if (k.xD != nullptr && b == 0) xstore(k.xD + (size_t)(s + 7) * 8 + a, xsent());
+    }
+  }
+}
+
 template <bool UPDATE>
 __global__ __launch_bounds__(64) void k_fwd(Lay L, double* __restrict__ r, const double* __restrict__ t,
                                             const double* __restrict__ cl, const double* __restrict__ cd,
                                             const double* __restrict__ cb, const double* __restrict__ E,
-                                            double* __restrict__ q, double* __restrict__ part2, uint64_t* ticket,
-                                            uint64_t tbase, const int* __restrict__ order, uint64_t* progress,
-                                            uint64_t pbase, PcgScalars* sc, int iter, int tol_mode, double tol,
-                                            double* __restrict__ part) {
+                                            double* __restrict__ q, double* X, double* __restrict__ part2,
+                                            double* __restrict__ part, uint64_t* ticket, uint64_t tbase,
+                                            const int* __restrict__ order, PcgScalars* sc, int iter, int tol_mode,
+                                            double tol, int lag) {
   const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
-  const int B = take_ticket(ticket, tbase, order, L.nlive);
+  const int B = __builtin_amdgcn_readfirstlane(take_ticket(ticket, tbase, order, L.nlive));
   if (UPDATE && sc->converged != STOP_RUN) return;
-  const double nalpha = UPDATE ? -sc->alpha : 0.0;
+  // readfirstlane: the ticket branch above would otherwise leave nlive in a divergent phi
+  const size_t brick = (size_t)B * L.S * 64, face = (size_t)__builtin_amdgcn_readfirstlane(L.nlive) * L.S * 8;
   const int2 ij = L.bij[B];
-  const int BL = ij.x > 0 ? L.live[(ij.x - 1) * L.nbj + ij.y] : -1;
-  const int BD = ij.y > 0 ? L.live[ij.x * L.nbj + ij.y - 1] : -1;
-  uint64_t seenL = 0, seenD = 0;
+  const int BL = __builtin_amdgcn_readfirstlane(ij.x > 0 ? L.live[(ij.x - 1) * L.nbj + ij.y] : -1);
+  const int BD = __builtin_amdgcn_readfirstlane(ij.y > 0 ? L.live[ij.x * L.nbj + ij.y - 1] : -1);
+  FwdK k;
+  k.r = r + brick;
+  k.t = t + brick;
+  k.cl = cl + brick;
+  k.cd = cd + brick;
+  k.cb = cb + brick;
+  k.E = E + brick;
+  k.q = q + brick;
+  k.xL = BL >= 0 ? X + (size_t)BL * L.S * 8 : nullptr;
+  k.xD = BD >= 0 ? X + face + (size_t)BD * L.S * 8 : nullptr;
+  k.oL = X + (size_t)B * L.S * 8;
+  k.oD = X + face + (size_t)B * L.S * 8;
+  k.Z = X + 2 * face;
+  k.S = L.S;
+  k.NC = L.S / U;
+  k.lag = lag;
+  k.nalpha = UPDATE ? -sc->alpha : 0.0;
+  bool give_up = false;
+  FwdSet A, Bs;
   double qprev = 0.0, mx = 0.0, ss = 0.0;
-  for (int s = 0; s < L.S; s++) {
-    const int64_t e = eidx(L, B, s, lane);
-    double rr = r[e];
-    if (UPDATE) {
-      rr = rr + nalpha * t[e];
-      r[e] = rr;
-      double av = fabs(rr);
-      mx = (mx < av) ? av : mx;
-      ss += rr * rr;
+  for (int c0 = 0;;) {  // (re)start the pipeline at chunk c0
+    fwd_load<UPDATE>(k, A, c0, lane);
+    fwd_cross(k, A, c0, a, b);
+    int bail = -1, c = c0;
+    // Whole pairs of chunks only: an exit between the two halves would merge register states
+    // at the loop latch and force every load to complete there.
+    for (; c + 1 < k.NC; c += 2) {
+      fwd_load<UPDATE>(k, Bs, c + 1, lane);
+      fwd_cross(k, Bs, c + 1, a, b);
+      if (!fwd_ready(A, a, b, give_up)) { bail = c; break; }
+      fwd_run<UPDATE>(k, A, c, lane, a, b, qprev, mx, ss);
+      fwd_load<UPDATE>(k, A, c + 2, lane);
+      fwd_cross(k, A, c + 2, a, b);
+      if (!fwd_ready(Bs, a, b, give_up)) { bail = c + 1; break; }
+      fwd_run<UPDATE>(k, Bs, c + 1, lane, a, b, qprev, mx, ss);
     }
-    const int need = min(s + 8, L.S);
-    double qL = __shfl_up(qprev, 8, 64);
-    double qD = __shfl_up(qprev, 1, 64);
-    if (BL >= 0) seenL = wait_progress(progress + BL, pbase + need, seenL, sc);
-    if (BD >= 0) seenD = wait_progress(progress + BD, pbase + need, seenD, sc);
-    if (a == 0) qL = (BL >= 0 && s + 7 < L.S) ? q[eidx(L, BL, s + 7, 56 + b)] : 0.0;
-    if (b == 0) qD = (BD >= 0 && s + 7 < L.S) ? q[eidx(L, BD, s + 7, a * 8 + 7)] : 0.0;
-    double tt = rr - cl[e] * qL;
-    tt = tt - cd[e] * qD;
-    tt = tt - cb[e] * qprev;
-    const double qv = tt * E[e];
-    q[e] = qv;
-    qprev = qv;
-    if ((s + 1) % PUB == 0 || s == L.S - 1) publish(progress + B, pbase + s + 1);
+    if (bail < 0 && c < k.NC) {  // odd chunk count: A holds the last chunk
+      if (!fwd_ready(A, a, b, give_up)) bail = c;
+      else fwd_run<UPDATE>(k, A, c, lane, a, b, qprev, mx, ss);
+    }
+    if (bail < 0) break;
+    give_up = fwd_wait(k, bail, a, b, sc);
+    c0 = bail;
   }
   if (UPDATE) {
     double wm = wave_max(mx);
@@ -551,40 +786,157 @@ __global__ __launch_bounds__(64) void k_fwd(Lay L, double* __restrict__ r, const
 }
 
 // Backward sweep L^T z = q (project-solver.cc:289-305) and the fixed-tree z.r; the last brick
-// forms beta = sigma_new / sigma (INIT: sigma = z.r).
+// forms beta = sigma_new / sigma (INIT: sigma = z.r). Slots run from S-1 down to 0; chunk c
+// covers slots S-1-c*U down to S-U-c*U.
+struct BwdSet {
+  double q[U], d1[U], d2[U], d3[U], e[U], r[U], xr[U], xu[U];
+};
+
+struct BwdK {  // wave-uniform
+  const double *q, *d1, *d2, *d3, *E, *r;
+  double* z;
+  double* xR;  // f 0 entries of the +x neighbour, nullptr when there is none
+  double* xU;  // f 1 entries of the +y neighbour
+  double* oR;  // this brick's f 0 entries (read by its -x neighbour)
+  double* oU;
+  const double* Z;
+  int S, NC, lag;
+};
+
+// element offset of the lowest slot of chunk c; slot S-1-c*U-u is at +(U-1-u)*64
+__device__ __forceinline__ size_t bwd_base(const BwdK& k, int c) { return (size_t)(k.S - (c + 1) * U) * 64; }
+
+__device__ __forceinline__ void bwd_load(const BwdK& k, BwdSet& S, int c, int lane) {
+  c = min(c, k.NC - 1);
+  const size_t o = bwd_base(k, c);
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int i = (U - 1 - u) * 64 + lane;
+    S.q[u] = k.q[o + i];
+    S.d1[u] = k.d1[o + i];
+    S.d2[u] = k.d2[o + i];
+    S.d3[u] = k.d3[o + i];
+    S.e[u] = k.E[o + i];
+    S.r[u] = k.r[o + i];
+  }
+}
+
+__device__ __forceinline__ void bwd_cross(const BwdK& k, BwdSet& S, int c, int a, int b) {
+  c = min(c, k.NC - 1);
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int s = k.S - 1 - (c * U + u);
+    const double* pR = (k.xR != nullptr && s >= 7) ? k.xR + (size_t)(s - 7) * 8 : k.Z;
+    const double* pU = (k.xU != nullptr && s >= 7) ? k.xU + (size_t)(s - 7) * 8 : k.Z;
+    S.xr[u] = xload(pR + b);
+    S.xu[u] = xload(pU + a);
+  }
+}
+
+__device__ __forceinline__ bool bwd_ready(const BwdSet& S, int a, int b, bool give_up) {
+  bool wait = false;
+#pragma unroll
+  for (int u = 0; u < U; u++) wait |= ((a == 7) & xunset(S.xr[u])) | ((b == 7) & xunset(S.xu[u]));
+  return give_up || __builtin_amdgcn_ballot_w64(wait) == 0;
+}
+
+__device__ __forceinline__ bool bwd_wait(const BwdK& k, int c, int a, int b, PcgScalars* sc) {
+  bool ok = true;
+  const int c1 = min(c + k.lag, k.NC - 1);
+  for (int s = k.S - 1 - c * U; s > k.S - 1 - (c1 + 1) * U && s >= 7; s--) {
+    if (k.xR != nullptr && a == 7) ok = ok && xwait(k.xR + (size_t)(s - 7) * 8 + b, sc);
+    if (k.xU != nullptr && b == 7) ok = ok && xwait(k.xU + (size_t)(s - 7) * 8 + a, sc);
+  }
+  return __builtin_amdgcn_ballot_w64(!ok) != 0;
+}
+
+__device__ __forceinline__ void bwd_run(const BwdK& k, const BwdSet& S, int c, int lane, int a, int b, double& zprev,
+                                        double& acc) {
+  const size_t o = bwd_base(k, c);
+  const int s0 = k.S - 1 - c * U;
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int s = s0 - u;
+    const int i = (U - 1 - u) * 64 + lane;
+    double zR = __shfl_down(zprev, 8, 64);
+    double zU = dpp_row<DPP_ROW_SHL1>(zprev);
+    zR = a == 7 ? S.xr[u] : zR;
+    zU = b == 7 ? S.xu[u] : zU;
+    double tt = S.q[u] - S.d1[u] * zR;
+    tt = tt - S.d2[u] * zU;
+    tt = tt - S.d3[u] * zprev;
+    const double zv = tt * S.e[u];
+    k.z[o + i] = zv;
+    if (a == 0) xstore(k.oR + (size_t)s * 8 + b, zv);
+    if (b == 0) xstore(k.oU + (size_t)s * 8 + a, zv);
+    zprev = zv;
+    acc += zv * S.r[u];
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int s = s0 - u;
+    if (s >= 7) {
+      if (k.xR != nullptr && a == 7) xstore(k.xR + (size_t)(s - 7) * 8 + b, xsent());
+      if (k.xU != nullptr && b == 7) xstore(k.xU + (size_t)(s - 7) * 8 + a, xsent());
+    }
+  }
+}
+
 template <bool INIT>
 __global__ __launch_bounds__(64) void k_bwd(Lay L, const double* __restrict__ qv, const double* __restrict__ dr,
                                             const double* __restrict__ du, const double* __restrict__ df,
                                             const double* __restrict__ E, const double* __restrict__ r,
-                                            double* __restrict__ z, double* __restrict__ part, uint64_t* ticket,
-                                            uint64_t tbase, const int* __restrict__ order, uint64_t* progress,
-                                            uint64_t pbase, PcgScalars* sc) {
+                                            double* __restrict__ z, double* X, double* __restrict__ part,
+                                            uint64_t* ticket, uint64_t tbase, const int* __restrict__ order,
+                                            PcgScalars* sc, int lag) {
   const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
-  const int B = take_ticket(ticket, tbase, order, L.nlive);
+  const int B = __builtin_amdgcn_readfirstlane(take_ticket(ticket, tbase, order, L.nlive));
   if (!INIT && sc->converged != STOP_RUN) return;
+  // readfirstlane: the ticket branch above would otherwise leave nlive in a divergent phi
+  const size_t brick = (size_t)B * L.S * 64, face = (size_t)__builtin_amdgcn_readfirstlane(L.nlive) * L.S * 8;
   const int2 ij = L.bij[B];
-  const int BR = ij.x + 1 < L.nbi ? L.live[(ij.x + 1) * L.nbj + ij.y] : -1;
-  const int BU = ij.y + 1 < L.nbj ? L.live[ij.x * L.nbj + ij.y + 1] : -1;
-  uint64_t seenR = 0, seenU = 0;
+  const int BR = __builtin_amdgcn_readfirstlane(ij.x + 1 < L.nbi ? L.live[(ij.x + 1) * L.nbj + ij.y] : -1);
+  const int BU = __builtin_amdgcn_readfirstlane(ij.y + 1 < L.nbj ? L.live[ij.x * L.nbj + ij.y + 1] : -1);
+  BwdK k;
+  k.q = qv + brick;
+  k.d1 = dr + brick;
+  k.d2 = du + brick;
+  k.d3 = df + brick;
+  k.E = E + brick;
+  k.r = r + brick;
+  k.z = z + brick;
+  k.xR = BR >= 0 ? X + (size_t)BR * L.S * 8 : nullptr;
+  k.xU = BU >= 0 ? X + face + (size_t)BU * L.S * 8 : nullptr;
+  k.oR = X + (size_t)B * L.S * 8;
+  k.oU = X + face + (size_t)B * L.S * 8;
+  k.Z = X + 2 * face;
+  k.S = L.S;
+  k.NC = L.S / U;
+  k.lag = lag;
+  bool give_up = false;
+  BwdSet A, Bs;
   double zprev = 0.0, acc = 0.0;
-  for (int u = 0; u < L.S; u++) {
-    const int s = L.S - 1 - u;
-    const int64_t e = eidx(L, B, s, lane);
-    const int need = min(u + 8, L.S);
-    double zR = __shfl_down(zprev, 8, 64);
-    double zU = __shfl_down(zprev, 1, 64);
-    if (BR >= 0) seenR = wait_progress(progress + BR, pbase + need, seenR, sc);
-    if (BU >= 0) seenU = wait_progress(progress + BU, pbase + need, seenU, sc);
-    if (a == 7) zR = (BR >= 0 && s >= 7) ? z[eidx(L, BR, s - 7, b)] : 0.0;
-    if (b == 7) zU = (BU >= 0 && s >= 7) ? z[eidx(L, BU, s - 7, a * 8)] : 0.0;
-    double tt = qv[e] - dr[e] * zR;
-    tt = tt - du[e] * zU;
-    tt = tt - df[e] * zprev;
-    const double zv = tt * E[e];
-    z[e] = zv;
-    zprev = zv;
-    acc += zv * r[e];
-    if ((u + 1) % PUB == 0 || u == L.S - 1) publish(progress + B, pbase + u + 1);
+  for (int c0 = 0;;) {
+    bwd_load(k, A, c0, lane);
+    bwd_cross(k, A, c0, a, b);
+    int bail = -1, c = c0;
+    for (; c + 1 < k.NC; c += 2) {
+      bwd_load(k, Bs, c + 1, lane);
+      bwd_cross(k, Bs, c + 1, a, b);
+      if (!bwd_ready(A, a, b, give_up)) { bail = c; break; }
+      bwd_run(k, A, c, lane, a, b, zprev, acc);
+      bwd_load(k, A, c + 2, lane);
+      bwd_cross(k, A, c + 2, a, b);
+      if (!bwd_ready(Bs, a, b, give_up)) { bail = c + 1; break; }
+      bwd_run(k, Bs, c + 1, lane, a, b, zprev, acc);
+    }
+    if (bail < 0 && c < k.NC) {
+      if (!bwd_ready(A, a, b, give_up)) bail = c;
+      else bwd_run(k, A, c, lane, a, b, zprev, acc);
+    }
+    if (bail < 0) break;
+    give_up = bwd_wait(k, bail, a, b, sc);
+    c0 = bail;
   }
   double ws = wave_sum(acc);
   if (lane == 0) part[B] = ws;
@@ -670,9 +1022,9 @@ int pcg_build(fs_ctx* c) {
   PcgState& P = c->pcg;
   P.nbi = (c->nx + 7) / 8;
   P.nbj = (c->ny + 7) / 8;
-  P.S = c->nz + 14;
+  P.S = ((c->nz + 14 + U - 1) / U) * U;
   const int nb = P.nbi * P.nbj;
-  FS_TRY(realloc_if(c, P.brick_flag, nb * sizeof(int)));
+  FS_TRY(realloc_if(c, P.brick_flag, (nb + 1) * sizeof(int)));
   FS_TRY(realloc_if(c, P.live_map, nb * sizeof(int)));
   if (!P.scal.p) {
     FS_TRY(fs_alloc(c, P.scal, sizeof(PcgScalars)));
@@ -682,13 +1034,14 @@ int pcg_build(fs_ctx* c) {
     FS_HIP(c, hipMemsetAsync(P.tickets.p, 0, 4 * sizeof(uint64_t), c->stream));
     for (int q = 0; q < 3; q++) P.ticket_base[q] = 0;
   }
-  FS_HIP(c, hipMemsetAsync(P.brick_flag.p, 0, nb * sizeof(int), c->stream));
+  FS_HIP(c, hipMemsetAsync(P.brick_flag.p, 0, (nb + 1) * sizeof(int), c->stream));
   k_brick_flags<<<fs_blocks((int64_t)c->nx * c->ny, 256), 256, 0, c->stream>>>(
-      c->nx, c->ny, c->nz, c->active.as<uint8_t>(), P.brick_flag.as<int>(), P.nbj);
+      c->nx, c->ny, c->nz, c->active.as<uint8_t>(), P.brick_flag.as<int>(), P.nbj, P.brick_flag.as<int>() + nb);
   FS_HIP(c, hipGetLastError());
-  std::vector<int> flags(nb);
-  FS_HIP(c, hipMemcpyAsync(flags.data(), P.brick_flag.p, nb * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  std::vector<int> flags(nb + 1);
+  FS_HIP(c, hipMemcpyAsync(flags.data(), P.brick_flag.p, (nb + 1) * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   FS_HIP(c, hipStreamSynchronize(c->stream));
+  c->n_active_last = flags[nb];
   P.live_host.assign(nb, -1);
   std::vector<int2> bij;
   for (int I = 0; I < P.nbi; I++)
@@ -699,6 +1052,10 @@ int pcg_build(fs_ctx* c) {
       }
   P.nlive = (int)bij.size();
   P.ne = (int64_t)P.nlive * P.S * 64;
+  if (P.ne >= ((int64_t)1 << 31)) {
+    c->set_error("pressure system too large for 32-bit element indexing");
+    return FS_BAD_ARG;
+  }
   P.nchunks = (int)((P.ne + CH - 1) / CH);
   // dependency order: anti-diagonals I+J ascending (then I); backward uses the reverse
   std::vector<int> ord(P.nlive);
@@ -712,16 +1069,22 @@ int pcg_build(fs_ctx* c) {
   FS_TRY(realloc_if(c, P.brick_ij, nl * sizeof(int2)));
   FS_TRY(realloc_if(c, P.fwd_order, nl * sizeof(int)));
   FS_TRY(realloc_if(c, P.bwd_order, nl * sizeof(int)));
-  FS_TRY(realloc_if(c, P.progress, 3 * nl * sizeof(uint64_t)));
+  const size_t nent = 2 * nl * (size_t)P.S * 8, nx_ = nent + 64;  // entries + zero row (padded)
+  FS_TRY(realloc_if(c, P.xf, nx_ * sizeof(double)));
+  FS_TRY(realloc_if(c, P.xb, nx_ * sizeof(double)));
   if (P.nlive > 0) {
     FS_HIP(c, hipMemcpyAsync(P.live_map.p, P.live_host.data(), nb * sizeof(int), hipMemcpyHostToDevice, c->stream));
     FS_HIP(c, hipMemcpyAsync(P.brick_ij.p, bij.data(), P.nlive * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     FS_HIP(c, hipMemcpyAsync(P.fwd_order.p, ord.data(), P.nlive * sizeof(int), hipMemcpyHostToDevice, c->stream));
     FS_HIP(c, hipMemcpyAsync(P.bwd_order.p, rord.data(), P.nlive * sizeof(int), hipMemcpyHostToDevice, c->stream));
   }
-  // progress counters restart at zero with a fresh epoch base
-  FS_HIP(c, hipMemsetAsync(P.progress.p, 0, 3 * nl * sizeof(uint64_t), c->stream));
-  for (int q = 0; q < 3; q++) P.epoch[q] = 0;
+  // every exchange entry starts as the sentinel
+  k_fill_sentinel<<<fs_blocks((int64_t)nent, 256), 256, 0, c->stream>>>((int64_t)nent, P.xf.as<double>());
+  k_fill_sentinel<<<fs_blocks((int64_t)nent, 256), 256, 0, c->stream>>>((int64_t)nent, P.xb.as<double>());
+  FS_HIP(c, hipMemsetAsync(P.xf.as<double>() + nent, 0, 64 * sizeof(double), c->stream));
+  FS_HIP(c, hipMemsetAsync(P.xb.as<double>() + nent, 0, 64 * sizeof(double), c->stream));
+  if (const char* lag = getenv("FS_SWEEP_LAG")) P.sweep_lag = std::max(0, atoi(lag));
+  FS_HIP(c, hipGetLastError());
   const size_t ne = (size_t)std::max<int64_t>(P.ne, 1);
   for (DBuf* d : {&P.adiag, &P.al, &P.ad, &P.ab, &P.rhs, &P.e, &P.cl, &P.cd, &P.cb, &P.dr, &P.du, &P.df, &P.x, &P.r,
                   &P.s, &P.t, &P.q, &P.z})
@@ -746,18 +1109,20 @@ int pcg_build(fs_ctx* c) {
   return FS_OK;
 }
 
-// tickets: one counter per sweep kind; progress: one slot range per sweep kind.
+// tickets: one counter per sweep kind.
 enum { SW_FACTOR = 0, SW_FWD = 1, SW_BWD = 2 };
 
 int pcg_factor(fs_ctx* c) {
   PcgState& P = c->pcg;
   Lay L = lay_of(c);
   uint64_t* tk = P.tickets.as<uint64_t>();
-  uint64_t pb = (P.epoch[SW_FACTOR]++) * (uint64_t)(P.S + 1);
-  k_factor_sweep<<<P.nlive, 64, 0, c->stream>>>(
-      L, c->cfg.mic_sigma, P.adiag.as<double>(), P.al.as<double>(), P.ad.as<double>(), P.ab.as<double>(),
-      P.tflags.as<uint16_t>(), P.tnum.as<double>(), P.e.as<double>(), tk + SW_FACTOR, P.ticket_base[SW_FACTOR],
-      P.fwd_order.as<int>(), P.progress.as<uint64_t>() + SW_FACTOR * P.nlive, pb, P.scal.as<PcgScalars>());
+  const int pf = fs_prof_begin(c, FS_KERNEL_MIC_FACTOR, 90.0 * c->n_active_last);
+  k_factor_sweep<<<P.nlive, 64, 0, c->stream>>>(L, c->cfg.mic_sigma, P.adiag.as<double>(), P.al.as<double>(),
+                                                P.ad.as<double>(), P.ab.as<double>(), P.tflags.as<uint16_t>(),
+                                                P.tnum.as<double>(), P.e.as<double>(), P.xf.as<double>(),
+                                                tk + SW_FACTOR, P.ticket_base[SW_FACTOR], P.fwd_order.as<int>(),
+                                                P.scal.as<PcgScalars>());
+  fs_prof_end(c, pf);
   P.ticket_base[SW_FACTOR] += P.nlive;
   FS_HIP(c, hipGetLastError());
   k_coef<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.al.as<double>(), P.ad.as<double>(), P.ab.as<double>(),
@@ -773,16 +1138,17 @@ int launch_fwd(fs_ctx* c, bool update, int iter) {
   PcgState& P = c->pcg;
   Lay L = lay_of(c);
   uint64_t* tk = P.tickets.as<uint64_t>();
-  uint64_t pb = (P.epoch[SW_FWD]++) * (uint64_t)(P.S + 1);
-  auto args = [&](auto kern) {
+  auto go = [&](auto kern) {
     kern<<<P.nlive, 64, 0, c->stream>>>(L, P.r.as<double>(), P.t.as<double>(), P.cl.as<double>(), P.cd.as<double>(),
-                                        P.cb.as<double>(), P.e.as<double>(), P.q.as<double>(), P.partials2.as<double>(),
-                                        tk + SW_FWD, P.ticket_base[SW_FWD], P.fwd_order.as<int>(),
-                                        P.progress.as<uint64_t>() + SW_FWD * P.nlive, pb, P.scal.as<PcgScalars>(),
-                                        iter, c->cfg.tol_mode, c->cfg.cg_tolerance, P.partials.as<double>());
+                                        P.cb.as<double>(), P.e.as<double>(), P.q.as<double>(), P.xf.as<double>(),
+                                        P.partials2.as<double>(), P.partials.as<double>(), tk + SW_FWD,
+                                        P.ticket_base[SW_FWD], P.fwd_order.as<int>(), P.scal.as<PcgScalars>(), iter,
+                                        c->cfg.tol_mode, c->cfg.cg_tolerance, P.sweep_lag);
   };
-  if (update) args(k_fwd<true>);
-  else args(k_fwd<false>);
+  const int pf = fs_prof_begin(c, FS_KERNEL_MIC_FWD, (update ? 64.0 : 48.0) * c->n_active_last);
+  if (update) go(k_fwd<true>);
+  else go(k_fwd<false>);
+  fs_prof_end(c, pf);
   P.ticket_base[SW_FWD] += P.nlive;
   FS_HIP(c, hipGetLastError());
   return FS_OK;
@@ -792,15 +1158,16 @@ int launch_bwd(fs_ctx* c, bool init) {
   PcgState& P = c->pcg;
   Lay L = lay_of(c);
   uint64_t* tk = P.tickets.as<uint64_t>();
-  uint64_t pb = (P.epoch[SW_BWD]++) * (uint64_t)(P.S + 1);
-  auto args = [&](auto kern) {
+  auto go = [&](auto kern) {
     kern<<<P.nlive, 64, 0, c->stream>>>(L, P.q.as<double>(), P.dr.as<double>(), P.du.as<double>(), P.df.as<double>(),
-                                        P.e.as<double>(), P.r.as<double>(), P.z.as<double>(), P.partials.as<double>(),
-                                        tk + SW_BWD, P.ticket_base[SW_BWD], P.bwd_order.as<int>(),
-                                        P.progress.as<uint64_t>() + SW_BWD * P.nlive, pb, P.scal.as<PcgScalars>());
+                                        P.e.as<double>(), P.r.as<double>(), P.z.as<double>(), P.xb.as<double>(),
+                                        P.partials.as<double>(), tk + SW_BWD, P.ticket_base[SW_BWD],
+                                        P.bwd_order.as<int>(), P.scal.as<PcgScalars>(), P.sweep_lag);
   };
-  if (init) args(k_bwd<true>);
-  else args(k_bwd<false>);
+  const int pf = fs_prof_begin(c, FS_KERNEL_MIC_BWD, 56.0 * c->n_active_last);
+  if (init) go(k_bwd<true>);
+  else go(k_bwd<false>);
+  fs_prof_end(c, pf);
   P.ticket_base[SW_BWD] += P.nlive;
   FS_HIP(c, hipGetLastError());
   return FS_OK;
@@ -848,17 +1215,22 @@ int fs_stage_solve(fs_ctx* c, fs_stats* st) {
       bool stopped = false;
       while (it < maxit && !stopped) {
         for (int m = 0; m < chk && it < maxit; m++, it++) {
+          int pf = fs_prof_begin(c, FS_KERNEL_SPMV, 48.0 * c->n_active_last);
           k_spmv<true><<<P.nchunks, 256, 0, c->stream>>>(L, P.s.as<double>(), P.adiag.as<double>(), P.al.as<double>(),
                                                          P.ad.as<double>(), P.ab.as<double>(), P.nbm.as<uint8_t>(),
                                                          P.t.as<double>(), P.partials.as<double>(), P.nchunks,
                                                          P.scal.as<PcgScalars>(), it);
+          fs_prof_end(c, pf);
           FS_TRY(launch_fwd(c, true, it));
           FS_TRY(launch_bwd(c, false));
+          pf = fs_prof_begin(c, FS_KERNEL_PCG_UPDATE, 40.0 * c->n_active_last);
           k_update<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(P.ne, P.x.as<double>(), P.s.as<double>(),
                                                                 P.z.as<double>(), P.scal.as<PcgScalars>(), it);
+          fs_prof_end(c, pf);
         }
         FS_HIP(c, hipGetLastError());
         FS_TRY(read_scalars(c));
+        fs_prof_flush(c);
         if (hs->converged != 0 || hs->wait_timeouts != 0) stopped = true;
       }
       if (hs->wait_timeouts != 0) {
@@ -879,7 +1251,7 @@ int fs_stage_solve(fs_ctx* c, fs_stats* st) {
     st->residual_inf = hs->residual;
     st->rhs_norm2 = std::sqrt(hs->bnorm2sq);
     st->residual_rel = hs->bnorm2sq > 0 ? std::sqrt(hs->rnorm2sq / hs->bnorm2sq) : 0.0;
-    st->n_active = 0;
+    st->n_active = (int32_t)c->n_active_last;
     st->refinements = 0;
   }
   return status;
@@ -963,7 +1335,7 @@ void fs_pcg_release(fs_ctx* c) {
   PcgState& P = c->pcg;
   for (DBuf* d : {&P.live_map, &P.brick_ij, &P.fwd_order, &P.bwd_order, &P.brick_flag, &P.adiag, &P.al, &P.ad, &P.ab,
                   &P.nbm, &P.rhs, &P.e, &P.cl, &P.cd, &P.cb, &P.dr, &P.du, &P.df, &P.tflags, &P.tnum, &P.x, &P.r,
-                  &P.s, &P.t, &P.q, &P.z, &P.partials, &P.partials2, &P.scal, &P.progress, &P.tickets})
+                  &P.s, &P.t, &P.q, &P.z, &P.partials, &P.partials2, &P.scal, &P.xf, &P.xb, &P.tickets})
     fs_free(*d);
   if (P.host_scal) (void)hipHostFree(P.host_scal);
   P.host_scal = nullptr;
