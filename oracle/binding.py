@@ -13,7 +13,8 @@ from pathlib import Path
 
 import numpy as np
 
-from simcraft_amd._capi import fs_config, fs_stats, default_config  # noqa: F401  (structs only)
+# ctypes mirrors of the C structs only (no call into the device library)
+from simcraft_amd._capi import fs_config, fs_stats
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libfs_oracle.so"
@@ -23,6 +24,7 @@ P2G_SORTED, P2G_INDEX, P2G_PARALLEL = 0, 1, 2
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _SIGS = {
+    "fso_config_default": (None, [C.POINTER(fs_config)]),
     "fso_create": (_P, [C.POINTER(fs_config)]),
     "fso_destroy": (None, [_P]),
     "fso_set_dot_mode": (C.c_int, [_P, C.c_int]),
@@ -86,6 +88,17 @@ def load() -> C.CDLL:
 
 def _dp(a):
     return a.ctypes.data_as(_D)
+
+
+def default_config(**overrides) -> fs_config:
+    """The reference's parameters (fso_config_default) with the given overrides."""
+    cfg = fs_config()
+    load().fso_config_default(C.byref(cfg))
+    for k, v in overrides.items():
+        if not hasattr(cfg, k):
+            raise AttributeError(f"fs_config has no field {k}")
+        setattr(cfg, k, v)
+    return cfg
 
 
 def fraction_inside(lu, ru, rd, ld) -> float:

@@ -1380,6 +1380,35 @@ struct Sim {
 using fso::Sim;
 extern "C" {
 
+// The reference's parameters: alpha 0.97 (advect-solver.h:129), tau 0.97 and sigma 0.25
+// (project-solver.h:83-84), |r|_inf < 1e-6 within 300 iterations (project-solver.h:129-130,
+// project-solver.cc:327), gravity 9.8 (fluid-simulator.cc:13-18), 10 SDF extrapolation and 5
+// smoothing passes, 10 velocity extrapolation passes and radius 0.8h (fluid-simulator.cc:
+// 146-160), CFL factor 5 (fluid-simulator.cc:202).
+void fso_config_default(fs_config* cfg) {
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->nx = cfg->ny = cfg->nz = 32;
+  cfg->size_x = cfg->size_y = cfg->size_z = 1.0;
+  cfg->precision = FS_PRECISION_FP64;
+  cfg->advection = FS_ADVECT_FLIP;
+  cfg->flip_alpha = 0.97;
+  cfg->mic_tau = 0.97;
+  cfg->mic_sigma = 0.25;
+  cfg->cg_tolerance = 1e-6;
+  cfg->cg_max_iterations = 300;
+  cfg->tol_mode = FS_TOL_ABS_LINF;
+  cfg->gravity = 9.8;
+  cfg->sdf_extrapolate_iters = 10;
+  cfg->sdf_smooth_iters = 5;
+  cfg->vel_extrapolate_iters = 10;
+  cfg->recon_radius_factor = 0.8;
+  cfg->advect_on_cleared_grid = 1;
+  cfg->cfl_factor = 5.0;
+  cfg->inner_tolerance = 1e-3;
+  cfg->max_refinements = 8;
+  cfg->check_interval = 8;
+}
+
 void* fso_create(const fs_config* cfg) {
   if (!cfg || cfg->nx <= 0 || cfg->ny <= 0 || cfg->nz <= 0) return nullptr;
   return new Sim(*cfg);
