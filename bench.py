@@ -10,8 +10,8 @@ The JSON line carries:
                 bytes per launch / mean launch duration, both measured live with HIP events on
                 the library's stream, against 8 TB/s HBM.
   cpu_baseline  the FP64 CPU oracle (a restatement of the reference, which does not build) on
-                the same workload: one 256^3 substep with the PCG capped at a few iterations;
-                the PCG share is scaled to the device's mean iteration count.
+                a bounded sample: one substep of the same scene at 128^3 with the PCG capped at a
+                few iterations, scaled by the cell/particle count and the device's iterations.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
 """
@@ -44,12 +44,15 @@ def workload_config(simcraft_amd, wl: dict, tol_mode: int, profile: int):
                                        profile_stages=profile)
 
 
-def cpu_baseline(simcraft_amd, scenes, wl: dict, tol_mode: int, gpu_iters: float, cap: int) -> dict:
-    """Time the oracle on one substep of the same workload (PCG capped at `cap` iterations)."""
+def cpu_baseline(simcraft_amd, scenes, wl: dict, tol_mode: int, gpu_iters: float, cap: int, sample_n: int) -> dict:
+    """Time the oracle on one substep of the same scene at sample_n^3 (PCG capped at `cap`
+    iterations) and scale by the cell and particle count to the workload's n^3."""
     from oracle.binding import DOT_SERIAL, P2G_PARALLEL, OracleSim
 
-    n = wl["n"]
-    cfg = workload_config(simcraft_amd, wl, tol_mode, 0)
+    n_full = wl["n"]
+    n = min(sample_n, n_full)
+    scale = (n_full / n) ** 3
+    cfg = workload_config(simcraft_amd, dict(wl, n=n), tol_mode, 0)
     cfg.cg_max_iterations = cap + 1  # `cap` iterations of the loop body
     pos, vel = scenes.dam_break(n)
     o = OracleSim(cfg, dot_mode=DOT_SERIAL, p2g_order=P2G_PARALLEL)
@@ -79,19 +82,19 @@ def cpu_baseline(simcraft_amd, scenes, wl: dict, tol_mode: int, gpu_iters: float
     o.close()
     t_pcg = times.pop("solve_pressure")
     per_iter = t_pcg / max(pcg_iters, 1)
-    step_s = sum(times.values()) + per_iter * gpu_iters
+    step_s = scale * (sum(times.values()) + per_iter * gpu_iters)
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": 1.0 / step_s,
         "unit": "steps/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"one {n}^3 substep of the same workload with the PCG capped at {pcg_iters} iterations "
-                   f"({t_pcg:.2f}s); PCG time scaled to the device's mean {gpu_iters:.1f} iterations; "
-                   f"non-PCG stages {sum(times.values()):.2f}s measured in full; P2G with the reference's "
-                   f"parallel particle loop"),
+        "sample": (f"one substep of the same scene at {n}^3 (cells and particles 1/{scale:g} of {n_full}^3), "
+                   f"timed per stage and scaled x{scale:g}; its PCG capped at {pcg_iters} iterations "
+                   f"({t_pcg:.2f}s) and scaled to the device's mean {gpu_iters:.1f} iterations at {n_full}^3; "
+                   f"P2G with the reference's parallel particle loop"),
         "seconds_per_step": step_s,
-        "stage_seconds": {k: round(v, 3) for k, v in times.items()},
+        "stage_seconds_sample": {k: round(v, 3) for k, v in times.items()},
     }
 
 
@@ -104,6 +107,7 @@ def main() -> int:
     ap.add_argument("--tol-mode", choices=["abs", "rel"], default="abs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=5)
+    ap.add_argument("--cpu-sample-n", type=int, default=128)
     ap.add_argument("--no-kernel-timing", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -194,7 +198,7 @@ def main() -> int:
         log("cpu baseline (oracle) ...")
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(simcraft_amd, scenes, wl, tol_mode, float(np.mean(iters)),
-                                           args.cpu_pcg_iters)
+                                           args.cpu_pcg_iters, args.cpu_sample_n)
         log(f"cpu baseline took {time.perf_counter() - t0:.1f}s: {out['cpu_baseline']['value']:.5f} steps/s")
     print(json.dumps(out), flush=True)
     return 0
