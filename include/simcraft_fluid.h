@@ -199,6 +199,28 @@ enum {
 int fs_kernel_profile(fs_ctx* ctx, int32_t kind, int64_t* launches, double* total_ms, double* total_bytes);
 int fs_kernel_profile_reset(fs_ctx* ctx);
 
+/* ---- Reverse mode (no counterpart in the reference, which has no derivative code) ----
+ * Vector-Jacobian products of a rollout of fs_substep calls, for differentiable simulation.
+ * The discrete structure of each substep (fluid SDF, marking, face weights, active set,
+ * extrapolation masks, Poisson matrix) is held fixed; the transfers (including their kernel
+ * weights' dependence on positions), collider, projection (through the same MIC-PCG on the
+ * symmetric matrix), velocity extrapolation, gravity and advection are differentiated exactly.
+ * Requires advect_on_cleared_grid = 1 (the reference order). Arrays are AoS xyz doubles. */
+int fs_rollout_begin(fs_ctx* ctx, int32_t checkpoint_every); /* record the following substeps */
+int fs_rollout_end(fs_ctx* ctx);                             /* stop recording, free the tape */
+int fs_rollout_steps(const fs_ctx* ctx);                     /* substeps recorded so far */
+/* Given dL/dpos and dL/dvel of the current (final) particles, returns dL/dpos and dL/dvel of the
+ * particles at fs_rollout_begin and dL/dG for the gravity vector G = (0, -gravity, 0) (the
+ * generalised force u_a += G_a dt on valid interior faces). Recomputes the rollout segment by
+ * segment from checkpoints taken every `checkpoint_every` substeps; restores the final state.
+ * Any output may be NULL. */
+int fs_rollout_vjp(fs_ctx* ctx, const double* dpos, const double* dvel, double* dpos0, double* dvel0,
+                   double* dgravity);
+/* PIC transfer of the given face grids (dense, reference layout) to the current particles, and
+ * its transpose: grid gradients from particle-velocity gradients. */
+int fs_sample_grid(fs_ctx* ctx, const double* u, const double* v, const double* w, double* vel_xyz);
+int fs_sample_grid_vjp(fs_ctx* ctx, const double* dvel_xyz, double* du, double* dv, double* dw);
+
 #ifdef __cplusplus
 }
 #endif

@@ -234,3 +234,43 @@ class FluidSim:
 
     def kernel_profile_reset(self) -> None:
         self._check(self._lib.fs_kernel_profile_reset(self._ctx))
+
+    # -- reverse mode (no counterpart in the reference) ------------------------------------------
+    def rollout_begin(self, checkpoint_every: int = 8) -> None:
+        """Record the following substeps for rollout_vjp (checkpoints every `checkpoint_every`)."""
+        self._check(self._lib.fs_rollout_begin(self._ctx, checkpoint_every))
+
+    def rollout_end(self) -> None:
+        self._check(self._lib.fs_rollout_end(self._ctx))
+
+    def rollout_steps(self) -> int:
+        return int(self._lib.fs_rollout_steps(self._ctx))
+
+    def rollout_vjp(self, dpos: np.ndarray | None, dvel: np.ndarray | None):
+        """dL/d(initial positions), dL/d(initial velocities), dL/dG (gravity vector G = (0, -g, 0))
+        from dL/d(final positions) and dL/d(final velocities) of the recorded rollout."""
+        n = self.num_particles()
+        a = None if dpos is None else np.ascontiguousarray(dpos, dtype=np.float64).reshape(n, 3)
+        b = None if dvel is None else np.ascontiguousarray(dvel, dtype=np.float64).reshape(n, 3)
+        dp0, dv0, dg = np.empty((n, 3)), np.empty((n, 3)), np.empty(3)
+        self._check(self._lib.fs_rollout_vjp(self._ctx, _dptr(a) if a is not None else None,
+                                             _dptr(b) if b is not None else None, _dptr(dp0), _dptr(dv0),
+                                             _dptr(dg)))
+        return dp0, dv0, dg
+
+    def sample_grid(self, u: np.ndarray, v: np.ndarray, w: np.ndarray) -> np.ndarray:
+        """PIC transfer of the face grids u, v, w to the current particles (velocities, (N, 3))."""
+        arrs = [np.ascontiguousarray(g, dtype=np.float64) for g in (u, v, w)]
+        for nm, g in zip(("u", "v", "w"), arrs):
+            if g.shape != self.array_info(nm)[0]:
+                raise ValueError(f"{nm} must have shape {self.array_info(nm)[0]}")
+        out = np.empty((self.num_particles(), 3))
+        self._check(self._lib.fs_sample_grid(self._ctx, *[_dptr(g) for g in arrs], _dptr(out)))
+        return out
+
+    def sample_grid_vjp(self, dvel: np.ndarray):
+        """Transpose of sample_grid: face-grid gradients from particle-velocity gradients."""
+        d = np.ascontiguousarray(dvel, dtype=np.float64).reshape(self.num_particles(), 3)
+        outs = [np.empty(self.array_info(nm)[0]) for nm in ("u", "v", "w")]
+        self._check(self._lib.fs_sample_grid_vjp(self._ctx, _dptr(d), *[_dptr(o) for o in outs]))
+        return tuple(outs)

@@ -100,6 +100,12 @@ SIGNATURES = {
     "fs_kernel_profile": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),
     "fs_kernel_profile_reset": (C.c_int, [_P]),
+    "fs_rollout_begin": (C.c_int, [_P, C.c_int32]),
+    "fs_rollout_end": (C.c_int, [_P]),
+    "fs_rollout_steps": (C.c_int, [_P]),
+    "fs_rollout_vjp": (C.c_int, [_P, _D, _D, _D, _D, _D]),
+    "fs_sample_grid": (C.c_int, [_P, _D, _D, _D, _D]),
+    "fs_sample_grid_vjp": (C.c_int, [_P, _D, _D, _D, _D]),
 }
 
 KERNEL_NAMES = ["spmv", "mic_fwd", "mic_bwd", "pcg_update", "mic_factor", "p2g", "g2p", "advect", "reconstruct"]

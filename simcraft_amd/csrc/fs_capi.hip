@@ -88,6 +88,7 @@ void release_all(fs_ctx* c) {
     fs_free(*b);
   for (int q = 0; q < 6; q++) fs_free(c->an[q]);
   fs_pcg_release(c);
+  fs_adj_release(c);
 }
 
 struct NamedArray {
@@ -322,6 +323,7 @@ int fs_set_particles(fs_ctx* c, const double* pos, const double* vel, int64_t n)
       }
     }
   FS_HIP(c, hipStreamSynchronize(c->stream));
+  fs_adj_release(c);  // a recorded rollout refers to the previous particles
   if (n != c->np) {
     for (int d = 0; d < 3; d++) {
       fs_free(c->pos[d]);
@@ -476,7 +478,20 @@ int fs_substep(fs_ctx* c, double dt, fs_stats* st) {
   fs_stats local;
   fs_stats* S = st ? st : &local;
   std::memset(S, 0, sizeof(*S));
+  if (c->tape.on) FS_TRY(fs_adj_on_substep(c, dt));
+  return fs_substep_core(c, dt, S, nullptr);
+}
+
+}  // extern "C"
+
+// cpu::FluidSimulator::substep (fluid-simulator.cc:140-182); `rec` records what the reverse
+// pass needs.
+int fs_substep_core(fs_ctx* c, double dt, fs_stats* S, AdjStep* rec) {
   const bool prof = c->cfg.profile_stages && c->ev_ok;
+  if (rec) {
+    rec->dt = dt;
+    FS_TRY(fs_adj_record(c, rec, ADJ_INPUT));
+  }
   auto mark = [&](int q) {
     if (prof) (void)hipEventRecord(c->ev[q], c->stream);
   };
@@ -488,24 +503,29 @@ int fs_substep(fs_ctx* c, double dt, fs_stats* st) {
     FS_TRY(fs_stage_advect(c, dt));
     FS_TRY(fs_stage_clear(c));
   }
+  if (rec) FS_TRY(fs_adj_record(c, rec, ADJ_ADVECTED));
   mark(1);
   FS_TRY(fs_stage_reconstruct(c, c->cfg.recon_radius_factor * c->h[0]));
   FS_TRY(fs_stage_extrapolate_sdf(c, c->cfg.sdf_extrapolate_iters));
   FS_TRY(fs_stage_smooth(c, c->cfg.sdf_smooth_iters));
   mark(2);
   FS_TRY(fs_stage_p2g(c));
+  if (rec) FS_TRY(fs_adj_record(c, rec, ADJ_P2G));
   FS_TRY(fs_stage_dirichlet(c));
+  if (rec) FS_TRY(fs_adj_record(c, rec, ADJ_DIRICHLET));
   mark(3);
   FS_TRY(fs_stage_extrapolate_velocity(c, c->cfg.vel_extrapolate_iters));
   FS_TRY(fs_stage_force(c, dt));
   mark(4);
   FS_TRY(fs_stage_build(c, dt));
+  if (rec) FS_TRY(fs_adj_record(c, rec, ADJ_BUILT));
   mark(5);
   int solve = fs_stage_solve(c, S);
   if (solve == FS_DEVICE_ERROR || solve == FS_UNSUPPORTED || solve == FS_BAD_ARG) return solve;
   mark(6);
   FS_TRY(fs_stage_project(c, dt));
   FS_TRY(fs_stage_collider(c));
+  if (rec) FS_TRY(fs_adj_record(c, rec, ADJ_FINAL));
   mark(7);
   FS_TRY(fs_stage_g2p(c));
   mark(8);
@@ -524,6 +544,8 @@ int fs_substep(fs_ctx* c, double dt, fs_stats* st) {
   S->status = solve;
   return solve;
 }
+
+extern "C" {
 
 // cpu::FluidSimulator::step (fluid-simulator.cc:205-220)
 int fs_step_frame(fs_ctx* c, double frame_dt, fs_stats* st) {

@@ -82,6 +82,36 @@ struct PcgState {
   DBuf f_r, f_s, f_t, f_q, f_z, f_x;
 };
 
+// Reverse mode. One recorded substep (device copies, context precision) and the rollout tape:
+// checkpoints of the full state every `every` substeps, from which a segment is recomputed
+// with its steps recorded, then reversed.
+struct AdjStep {
+  double dt = 0;
+  DBuf x0[3], v0[3], x1[3], v1[3];  // particles before and after advection
+  DBuf g0[3], w0[3];                // P2G output (FLIP's u_last) and its weight sums
+  DBuf valid[3];                    // valid flags after the Dirichlet walls
+  DBuf g5[3];                       // grids after projection and collider (G2P input)
+  DBuf weights[3];                  // face weights of the assembly
+  DBuf fsdf, active, adiag, an[6];  // the pressure system's geometry
+  DBuf sdfbuf0;                     // SDF ping-pong buffer at the start (carried over)
+  DBuf pg;                          // pressure
+};
+struct AdjCheckpoint {
+  DBuf x[3], v[3], fsdf, fsdfbuf, sdfvalid, sdfvalidbuf;
+};
+struct AdjTape {
+  bool on = false;
+  int every = 8;
+  std::vector<double> dts;
+  std::vector<AdjCheckpoint> ck;
+  std::vector<AdjStep> seg;
+  AdjCheckpoint end;
+  // reverse-pass scratch of the fluid-surface chain (cell arrays; masks and branch flags)
+  DBuf sA, sB, sAb, sBb, sAb2, sBb2;
+  std::vector<DBuf> sV, sF;
+};
+enum AdjPoint { ADJ_INPUT, ADJ_ADVECTED, ADJ_P2G, ADJ_DIRICHLET, ADJ_BUILT, ADJ_FINAL };
+
 struct fs_ctx {
   fs_config cfg{};
   int device = 0;
@@ -127,6 +157,7 @@ struct fs_ctx {
   double kprof_ms[FS_KERNEL_COUNT] = {};
   double kprof_bytes[FS_KERNEL_COUNT] = {};
   int64_t n_active_last = 0;
+  AdjTape tape;
 
   void set_error(const std::string& s) { err = s; }
 };
@@ -152,6 +183,21 @@ int fs_stage_project(fs_ctx* c, double dt);
 int fs_stage_collider(fs_ctx* c);
 int fs_stage_g2p(fs_ctx* c);
 int fs_stage_cfl(fs_ctx* c, double* out);
+
+// Substep with an optional recording of the state the reverse pass needs (fs_capi.hip).
+int fs_substep_core(fs_ctx* c, double dt, fs_stats* S, AdjStep* rec);
+// Reverse mode (fs_adjoint.hip; particle kernels in fs_grid.hip).
+int fs_adj_record(fs_ctx* c, AdjStep* rec, AdjPoint at);
+int fs_adj_on_substep(fs_ctx* c, double dt);  // checkpoints of a recorded rollout
+void fs_adj_release(fs_ctx* c);
+int fs_adj_p2g_sum(fs_ctx* c, DBuf* pos, DBuf* coef, DBuf* out);
+int fs_adj_g2p(fs_ctx* c, DBuf* x1, DBuf* g5, DBuf* g0, DBuf* vbar, DBuf* xbar, DBuf* vbar1, DBuf* coef);
+int fs_adj_p2g(fs_ctx* c, DBuf* x1, DBuf* v1, DBuf* H, DBuf* HG, DBuf* vbar, DBuf* xbar);
+int fs_adj_advect(fs_ctx* c, DBuf* x0, DBuf* v0, DBuf* xbar, DBuf* vbar, double dt);
+// Reverse of reconstruct -> extrapolateFluidSdf -> smoothFluidSurface for one recorded step:
+// phibar (adjoint of the smoothed SDF) and psibar (in: adjoint of the SDF buffer the step leaves
+// behind; out: adjoint of the buffer it started from) -> position adjoint added to xbar.
+int fs_adj_sdf(fs_ctx* c, AdjStep& t, DBuf& phibar, DBuf& psibar, DBuf* xbar);
 int fs_pcg_spmv_dense(fs_ctx* c, const double* x, double* y);
 int fs_pcg_mic_dense(fs_ctx* c, const double* r, double* z);
 int fs_pcg_export_e(fs_ctx* c, double* out);

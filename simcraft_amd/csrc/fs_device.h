@@ -54,6 +54,17 @@ __device__ inline T bspline(T r) {
   return T(0);
 }
 
+// dN/dr of bspline (r >= 0); N is C2, so the branches agree at r = 1 and r = 2.
+template <typename T>
+__device__ inline T bspline_d(T r) {
+  if (r < T(1)) return T(1.5) * r * r - T(2) * r;
+  if (r < T(2)) {
+    T t = T(2) - r;
+    return T(-0.5) * t * t;
+  }
+  return T(0);
+}
+
 // Morton code (Spatify/include/Spatify/ns-util.h:36-45)
 __host__ __device__ inline int morton10(int x) {
   x = (x | (x << 16)) & 0x030000FF;

@@ -309,49 +309,111 @@ __global__ void k_p2g_records(int n, int axis, GridArgs<T> ga, const int* __rest
   }
 }
 
-// P2G gather for one face grid (FlipAdvectionSolver3D::solveP2G, advect-solver.cc:188-271):
-// node n sums v*w and w over particles whose nearest node b satisfies n-b in {-1..2}^3,
-// visiting bins b in lexicographic order and particles by index; then ug /= uw or 0.
-template <typename T>
-__global__ void k_p2g_gather(GridArgs<T> ga, int gx, int gy, int gz, int n, const int* __restrict__ offs,
-                             const T* __restrict__ W, const T* __restrict__ vc, T* __restrict__ G, T* __restrict__ Wt,
-                             uint8_t* __restrict__ valid) {
-  int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= gx * gy * gz) return;
-  int x, y, z;
-  unlin3(id, gy, gz, x, y, z);
+// P2G gather for one face grid (FlipAdvectionSolver3D::solveP2G, advect-solver.cc:188-271).
+// Node (x, y, z) sums v*w and w over the particles whose nearest node b has x-b0, y-b1, z-b2
+// in {-1..2}, bins in lexicographic (b0, b1, b2) order and particles by index inside a bin;
+// then ug = sum(v w) / sum(w) where the weight sum is positive, else 0 and not valid.
+// A thread owns a 2x2 (y, z) node column over P2G_LX consecutive x. It walks the bin slabs b0
+// in ascending order and adds every particle to the x-planes it reaches (a ring of four
+// accumulator planes, slot x & 3); plane x is complete after slab x+1. Each node still sees
+// its contributions in (b0, b1, b2, index) order, while a particle's record is read once per
+// thread instead of once per node.
+constexpr int P2G_LX = 16;  // x-nodes per thread; a multiple of 4 keeps the ring slots static
+
+template <typename T, bool NORM = true>
+__global__ __launch_bounds__(256) void k_p2g_gather(GridArgs<T> ga, int gx, int gy, int gz, int n,
+                                                    const int* __restrict__ offs, const T* __restrict__ W,
+                                                    const T* __restrict__ vc, T* __restrict__ G,
+                                                    T* __restrict__ Wt, uint8_t* __restrict__ valid) {
+  const int nty = (gy + 1) >> 1, ntz = (gz + 1) >> 1, nseg = (gx + P2G_LX - 1) / P2G_LX;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nseg * nty * ntz) return;
+  const int tz = id % ntz, ty = (id / ntz) % nty, seg = id / (ntz * nty);
+  const int y0 = 2 * ty, z0 = 2 * tz, xs = seg * P2G_LX, xe = min(xs + P2G_LX, gx);
   const int bx = ga.nx + 1, by = ga.ny + 1, bz = ga.nz + 1;
-  T acc = T(0), ws = T(0);
-  const int x0 = max(x - 2, 0), x1 = min(x + 1, bx - 1);
-  const int y0 = max(y - 2, 0), y1 = min(y + 1, by - 1);
-  const int z0 = max(z - 2, 0), z1 = min(z + 1, bz - 1);
-  const T* Wx = W;
-  const T* Wy = W + (int64_t)4 * n;
-  const T* Wz = W + (int64_t)8 * n;
-  for (int b0 = x0; b0 <= x1; b0++) {
-    const T* wx = Wx + (int64_t)(x - b0 + 1) * n;
-    for (int b1 = y0; b1 <= y1; b1++) {
-      const T* wy = Wy + (int64_t)(y - b1 + 1) * n;
-      const int row = bz * (b1 + by * b0);
-      for (int b2 = z0; b2 <= z1; b2++) {
-        const T* wz = Wz + (int64_t)(z - b2 + 1) * n;
-        const int s = offs[row + b2], e = offs[row + b2 + 1];
-        for (int t = s; t < e; t++) {
-          const T w = wx[t] * wy[t] * wz[t];
-          acc = acc + vc[t] * w;
-          ws = ws + w;
+  const bool y1ok = y0 + 1 < gy, z1ok = z0 + 1 < gz;
+  const int b1lo = max(y0 - 2, 0), b1hi = min(y0 + 2, by - 1);
+  const int b2lo = max(z0 - 2, 0), b2hi = min(z0 + 2, bz - 1);
+  T acc[4][2][2], ws[4][2][2];
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int k = 0; k < 2; k++) acc[q][j][k] = ws[q][j][k] = T(0);
+  for (int i0 = 0; i0 < P2G_LX + 3; i0 += 4) {
+#pragma unroll
+    for (int ii = 0; ii < 4; ii++) {
+      const int i = i0 + ii;
+      if (i >= P2G_LX + 3) break;
+      const int b0 = xs - 2 + i;  // planes x = b0-1+o, o = 0..3, ring slot (ii + o + 1) & 3
+      bool live[4];
+#pragma unroll
+      for (int o = 0; o < 4; o++) live[o] = b0 - 1 + o >= xs && b0 - 1 + o < xe;
+      if (b0 >= 0 && b0 < bx) {
+        for (int b1 = b1lo; b1 <= b1hi; b1++) {
+          const int iy0 = y0 - b1 + 1, iy1 = iy0 + 1;  // y factor index of nodes y0, y0+1
+          const bool oky0 = iy0 >= 0 && iy0 <= 3, oky1 = y1ok && iy1 >= 0 && iy1 <= 3;
+          const int row = bz * (b1 + by * b0);
+          for (int b2 = b2lo; b2 <= b2hi; b2++) {
+            const int iz0 = z0 - b2 + 1, iz1 = iz0 + 1;
+            const bool okz0 = iz0 >= 0 && iz0 <= 3, okz1 = z1ok && iz1 >= 0 && iz1 <= 3;
+            const int s = offs[row + b2], e = offs[row + b2 + 1];
+            for (int t = s; t < e; t++) {
+              T wx[4];
+#pragma unroll
+              for (int o = 0; o < 4; o++) wx[o] = W[(int64_t)o * n + t];
+              const T wy[2] = {oky0 ? W[(int64_t)(4 + iy0) * n + t] : T(0), oky1 ? W[(int64_t)(4 + iy1) * n + t] : T(0)};
+              const T wz[2] = {okz0 ? W[(int64_t)(8 + iz0) * n + t] : T(0), okz1 ? W[(int64_t)(8 + iz1) * n + t] : T(0)};
+              const bool oky[2] = {oky0, oky1}, okz[2] = {okz0, okz1};
+              const T v = vc[t];
+#pragma unroll
+              for (int o = 0; o < 4; o++) {
+                const int q = (ii + o + 1) & 3;
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                  for (int k = 0; k < 2; k++) {
+                    const T w = wx[o] * wy[j] * wz[k];
+                    const bool on = live[o] && oky[j] && okz[k];
+                    acc[q][j][k] = on ? acc[q][j][k] + v * w : acc[q][j][k];
+                    ws[q][j][k] = on ? ws[q][j][k] + w : ws[q][j][k];
+                  }
+              }
+            }
+          }
         }
       }
+      // plane x = b0 - 1 has seen every slab x-2 .. x+1
+      const int q = (ii + 1) & 3;
+      if (live[0]) {
+        const int x = b0 - 1;
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+          for (int k = 0; k < 2; k++) {
+            if ((j == 1 && !y1ok) || (k == 1 && !z1ok)) continue;
+            const int64_t nid = ((int64_t)x * gy + (y0 + j)) * gz + (z0 + k);
+            const T a = acc[q][j][k], s = ws[q][j][k];
+            if (NORM) {
+              G[nid] = s > T(0) ? a / s : T(0);
+              valid[nid] = s > T(0) ? 1 : 0;
+              Wt[nid] = s;
+            } else {
+              G[nid] = a;
+            }
+          }
+      }
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int k = 0; k < 2; k++) acc[q][j][k] = ws[q][j][k] = T(0);
     }
   }
-  if (ws > T(0)) {
-    G[id] = acc / ws;
-    valid[id] = 1;
-  } else {
-    G[id] = T(0);
-    valid[id] = 0;
-  }
-  Wt[id] = ws;
+}
+
+int p2g_gather_threads(int gx, int gy, int gz) {
+  return ((gx + P2G_LX - 1) / P2G_LX) * ((gy + 1) / 2) * ((gz + 1) / 2);
 }
 
 // NaiveReconstructor::reconstruct (FluidSim/include/FluidSim/cpu/sdf.h:210-225) as a particle
@@ -489,8 +551,10 @@ __global__ void k_extrap_sdf(int nx, int ny, int nz, const T* __restrict__ cur, 
 }
 
 // smoothFluidSurface iteration (FluidSim/src/cpu/fluid-simulator.cc:40-71): min(avg6, self).
-template <typename T>
-__global__ void k_smooth(int nx, int ny, int nz, const T* __restrict__ cur, T* __restrict__ buf) {
+// FLAG also records whether the cell took the average (the reverse pass needs the branch).
+template <typename T, bool FLAG = false>
+__global__ void k_smooth(int nx, int ny, int nz, const T* __restrict__ cur, T* __restrict__ buf,
+                         uint8_t* __restrict__ took_avg = nullptr) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nx * ny * nz) return;
   int i, j, k;
@@ -506,6 +570,7 @@ __global__ void k_smooth(int nx, int ny, int nz, const T* __restrict__ cur, T* _
   if (k < nz - 1) { sum += cur[id + 1]; count++; }
   T b = sum / (T)count;
   T c0 = cur[id];
+  if (FLAG) took_avg[id] = !(c0 < b);
   if (c0 < b) b = c0;
   buf[id] = b;
 }
@@ -969,7 +1034,7 @@ int p2g_t(fs_ctx* c) {
                                                                  c->vel[a].as<T>(), c->p2g_w.as<T>(),
                                                                  c->sorted[0].as<T>());
     int gx = c->fdim[a][0], gy = c->fdim[a][1], gz = c->fdim[a][2];
-    k_p2g_gather<T><<<fs_blocks(c->n_face[a], TPB), TPB, 0, c->stream>>>(
+    k_p2g_gather<T><<<fs_blocks(p2g_gather_threads(gx, gy, gz), TPB), TPB, 0, c->stream>>>(
         ga, gx, gy, gz, n, c->offs.as<int>(), c->p2g_w.as<T>(), c->sorted[0].as<T>(), c->g[a].as<T>(),
         c->gw[a].as<T>(), c->gvalid[a].as<uint8_t>());
     fs_prof_end(c, pf);
@@ -1096,6 +1161,492 @@ int cfl_t(fs_ctx* c, double* out) {
   return FS_OK;
 }
 
+
+
+// ---- reverse mode: the fluid-surface chain ---------------------------------------------------
+template <typename T>
+__global__ void k_accumulate(int64_t n, const T* __restrict__ a, T* __restrict__ b) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = b[i] + a[i];
+}
+
+__device__ inline int n_count(int i, int j, int k, int nx, int ny, int nz) {
+  return (i > 0) + (i < nx - 1) + (j > 0) + (j < ny - 1) + (k > 0) + (k < nz - 1);
+}
+__device__ inline int v_count(const uint8_t* V, int id, int i, int j, int k, int nx, int ny, int nz) {
+  const int sx = ny * nz, sy = nz;
+  return (i > 0 && V[id - sx]) + (i < nx - 1 && V[id + sx]) + (j > 0 && V[id - sy]) + (j < ny - 1 && V[id + sy]) +
+         (k > 0 && V[id - 1]) + (k < nz - 1 && V[id + 1]);
+}
+
+// One smoothing iteration reversed: out(c) = cur(c), or the mean of c's neighbours when c took
+// the average. curbar(n) = [n kept itself] outbar(n) + sum over neighbours c that averaged of
+// outbar(c) / count(c).
+template <typename T>
+__global__ void k_smooth_adj(int nx, int ny, int nz, const uint8_t* __restrict__ F, const T* __restrict__ ob,
+                             T* __restrict__ cb) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nx * ny * nz) return;
+  int i, j, k;
+  unlin3(id, ny, nz, i, j, k);
+  const int sx = ny * nz, sy = nz;
+  T g = F[id] ? T(0) : ob[id];
+  if (i > 0 && F[id - sx]) g += ob[id - sx] / (T)n_count(i - 1, j, k, nx, ny, nz);
+  if (i < nx - 1 && F[id + sx]) g += ob[id + sx] / (T)n_count(i + 1, j, k, nx, ny, nz);
+  if (j > 0 && F[id - sy]) g += ob[id - sy] / (T)n_count(i, j - 1, k, nx, ny, nz);
+  if (j < ny - 1 && F[id + sy]) g += ob[id + sy] / (T)n_count(i, j + 1, k, nx, ny, nz);
+  if (k > 0 && F[id - 1]) g += ob[id - 1] / (T)n_count(i, j, k - 1, nx, ny, nz);
+  if (k < nz - 1 && F[id + 1]) g += ob[id + 1] / (T)n_count(i, j, k + 1, nx, ny, nz);
+  cb[id] = g;
+}
+
+// One extrapolateFluidSdf iteration reversed. Forward with mask V: buf(c) = mean of the valid
+// neighbours' cur for an invalid c that has some, else buf(c) is left as it was (valid and far
+// cells); then (cur, buf) -> (buf, cur). From the adjoints (A, B) of the new pair:
+//   curbar(n) = B(n) + [n valid] sum over invalid neighbours c of A(c) / count(c)
+//   bufbar(c) = [c valid or count(c) = 0] A(c)
+template <typename T>
+__global__ void k_extrap_sdf_adj(int nx, int ny, int nz, const uint8_t* __restrict__ V, const T* __restrict__ A,
+                                 const T* __restrict__ B, T* __restrict__ cb, T* __restrict__ bb) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nx * ny * nz) return;
+  int i, j, k;
+  unlin3(id, ny, nz, i, j, k);
+  const int sx = ny * nz, sy = nz;
+  T g = B[id];
+  if (V[id]) {
+    if (i > 0 && !V[id - sx]) g += A[id - sx] / (T)v_count(V, id - sx, i - 1, j, k, nx, ny, nz);
+    if (i < nx - 1 && !V[id + sx]) g += A[id + sx] / (T)v_count(V, id + sx, i + 1, j, k, nx, ny, nz);
+    if (j > 0 && !V[id - sy]) g += A[id - sy] / (T)v_count(V, id - sy, i, j - 1, k, nx, ny, nz);
+    if (j < ny - 1 && !V[id + sy]) g += A[id + sy] / (T)v_count(V, id + sy, i, j + 1, k, nx, ny, nz);
+    if (k > 0 && !V[id - 1]) g += A[id - 1] / (T)v_count(V, id - 1, i, j, k - 1, nx, ny, nz);
+    if (k < nz - 1 && !V[id + 1]) g += A[id + 1] / (T)v_count(V, id + 1, i, j, k + 1, nx, ny, nz);
+  }
+  cb[id] = g;
+  bb[id] = (V[id] || v_count(V, id, i, j, k, nx, ny, nz) == 0) ? A[id] : T(0);
+}
+
+// Reverse of the reconstruction: a valid cell's value is |x_c - x_p| - r for its nearest kept
+// particle p (the one whose squared distance equals the cell's minimum), so p receives
+// abar(c) (x_p - x_c) / |x_c - x_p|. Same candidate cells and inclusion tests as k_recon_splat.
+template <typename T>
+__global__ void k_recon_adj(int n, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
+                            const T* __restrict__ pz, const int* __restrict__ keys, const int* __restrict__ binmax,
+                            T r2, T s_max, const typename Bits<T>::U* __restrict__ dmin, const T* __restrict__ abar,
+                            T* __restrict__ gx, T* __restrict__ gy, T* __restrict__ gz) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  if (binmax[keys[id]] == id) return;
+  const T p[3] = {px[id], py[id], pz[id]};
+  const int dims[3] = {ga.nx, ga.ny, ga.nz};
+  int bin[3], lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    int bb = (int)floor(p[d] / ga.h[d]);
+    bb = bb < 0 ? 0 : bb;
+    bin[d] = bb >= dims[d] ? dims[d] - 1 : bb;
+    lo[d] = max(0, (int)floor((p[d] - r2) / ga.h[d] - T(0.5)) - 1);
+    hi[d] = min(dims[d] - 1, (int)floor((p[d] + r2) / ga.h[d] - T(0.5)) + 1);
+  }
+  auto in_range = [&](int d, T xc) {
+    int mn = max(0, (int)floor((xc - r2) / ga.h[d]));
+    int mx = min(dims[d] - 1, (int)ceil((xc + r2) / ga.h[d]));
+    return bin[d] >= mn && bin[d] <= mx;
+  };
+  T g[3] = {T(0), T(0), T(0)};
+  for (int i = lo[0]; i <= hi[0]; i++) {
+    const T xc = coord1(i, T(0.5), ga.h[0]);
+    const T dx = xc - p[0];
+    const T dx2 = dx * dx;
+    if (dx2 > s_max || !in_range(0, xc)) continue;
+    for (int j = lo[1]; j <= hi[1]; j++) {
+      const T yc = coord1(j, T(0.5), ga.h[1]);
+      const T dy = yc - p[1];
+      const T dxy = dx2 + dy * dy;
+      if (dxy > s_max || !in_range(1, yc)) continue;
+      for (int k = lo[2]; k <= hi[2]; k++) {
+        const T zc = coord1(k, T(0.5), ga.h[2]);
+        const T dz = zc - p[2];
+        const T d2 = dxy + dz * dz;
+        if (d2 > s_max || !in_range(2, zc)) continue;
+        const int cell = lin3(i, j, k, ga.ny, ga.nz);
+        if (Bits<T>::of(d2) != dmin[cell] || !(d2 > T(0))) continue;
+        const T s = abar[cell] / sqrt(d2);
+        g[0] -= s * dx;
+        g[1] -= s * dy;
+        g[2] -= s * dz;
+      }
+    }
+  }
+  gx[id] += g[0];
+  gy[id] += g[1];
+  gz[id] += g[2];
+}
+
+int ensure_cells(fs_ctx* c, DBuf& b, size_t bytes) {
+  if (b.bytes != bytes) {
+    fs_free(b);
+    FS_TRY(fs_alloc(c, b, bytes));
+  }
+  return FS_OK;
+}
+
+template <typename T>
+int adj_sdf_t(fs_ctx* c, AdjStep& t, DBuf& phibar, DBuf& psibar, DBuf* xbar) {
+  using UB = typename Bits<T>::U;
+  AdjTape& tp = c->tape;
+  const int nc = c->n_cell, n = (int)c->np;
+  const int ie = c->cfg.sdf_extrapolate_iters, is = c->cfg.sdf_smooth_iters;
+  if (is < 1) {
+    c->set_error("the reverse pass needs at least one SDF smoothing iteration");
+    return FS_UNSUPPORTED;
+  }
+  const size_t cb = (size_t)nc * sizeof(T);
+  for (DBuf* b : {&tp.sA, &tp.sB, &tp.sAb, &tp.sBb, &tp.sAb2, &tp.sBb2}) FS_TRY(ensure_cells(c, *b, cb));
+  while ((int)tp.sV.size() < ie + 1) {
+    tp.sV.emplace_back();
+    FS_TRY(fs_alloc(c, tp.sV.back(), (size_t)nc));
+  }
+  while ((int)tp.sF.size() < is) {
+    tp.sF.emplace_back();
+    FS_TRY(fs_alloc(c, tp.sF.back(), (size_t)nc));
+  }
+  // forward: reconstruction of the recorded positions (values, mask, per-cell minima)
+  GridArgs<T> ga = grid_args<T>(c);
+  const int nb = morton3D(c->nx - 1, c->ny - 1, c->nz - 1) + 1;
+  FS_TRY(ensure_bins(c, (size_t)nb));
+  if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)nc * sizeof(double)));
+  UB* dmin = c->recon_tmp.as<UB>();
+  const T rad = (T)(c->cfg.recon_radius_factor * c->h[0]), r2 = T(2) * rad;
+  const T smx = sizeof(T) == 8 ? (T)smax_for((double)r2) : (T)smax_for_f((float)r2);
+  k_fill_bits<T><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(nc, dmin);
+  if (n > 0) {
+    FS_HIP(c, hipMemsetAsync(c->counts.p, 0xFF, (size_t)nb * sizeof(int), c->stream));
+    k_keys_morton<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, t.x1[0].as<T>(), t.x1[1].as<T>(),
+                                                               t.x1[2].as<T>(), c->keys.as<int>());
+    k_bin_last<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->counts.as<int>());
+    k_recon_splat<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, t.x1[0].as<T>(), t.x1[1].as<T>(),
+                                                               t.x1[2].as<T>(), c->keys.as<int>(),
+                                                               c->counts.as<int>(), r2, smx, dmin);
+  }
+  k_recon_final<T><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(nc, dmin, rad, tp.sA.as<T>(), tp.sV[0].as<uint8_t>());
+  FS_HIP(c, hipMemcpyAsync(tp.sB.p, t.sdfbuf0.p, cb, hipMemcpyDeviceToDevice, c->stream));
+  // forward: extrapolation (values and masks), then smoothing with its branch flags
+  DBuf *cur = &tp.sA, *buf = &tp.sB;
+  for (int it = 0; it < ie; it++) {
+    k_extrap_sdf<T><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(c->nx, c->ny, c->nz, cur->as<T>(), buf->as<T>(),
+                                                               tp.sV[it].as<uint8_t>(), tp.sV[it + 1].as<uint8_t>());
+    std::swap(cur, buf);
+  }
+  for (int it = 0; it < is; it++) {
+    k_smooth<T, true><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(c->nx, c->ny, c->nz, cur->as<T>(), buf->as<T>(),
+                                                                tp.sF[it].as<uint8_t>());
+    std::swap(cur, buf);
+  }
+  // reverse: smoothing (the buffer the step leaves behind is the last smoothing input)
+  DBuf *ob = &phibar, *ib = &tp.sAb;
+  for (int it = is - 1; it >= 0; it--) {
+    k_smooth_adj<T><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(c->nx, c->ny, c->nz, tp.sF[it].as<uint8_t>(),
+                                                               ob->as<T>(), ib->as<T>());
+    if (it == is - 1) {
+      k_accumulate<T><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(nc, psibar.as<T>(), ib->as<T>());
+      ob = &tp.sAb2;
+    }
+    std::swap(ob, ib);
+  }
+  // reverse: extrapolation, from (adjoint of the smoothing input, 0 for the dead buffer)
+  DBuf *A = ob, *B = &tp.sBb, *A2 = ib == &phibar ? &tp.sAb2 : ib, *B2 = &tp.sBb2;
+  FS_HIP(c, hipMemsetAsync(B->p, 0, cb, c->stream));
+  for (int it = ie - 1; it >= 0; it--) {
+    k_extrap_sdf_adj<T><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(c->nx, c->ny, c->nz, tp.sV[it].as<uint8_t>(),
+                                                                   A->as<T>(), B->as<T>(), A2->as<T>(), B2->as<T>());
+    std::swap(A, A2);
+    std::swap(B, B2);
+  }
+  // reconstruction reversed; the start buffer's adjoint is carried to the previous step
+  if (n > 0)
+    k_recon_adj<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, t.x1[0].as<T>(), t.x1[1].as<T>(),
+                                                             t.x1[2].as<T>(), c->keys.as<int>(), c->counts.as<int>(),
+                                                             r2, smx, dmin, A->as<T>(), xbar[0].as<T>(),
+                                                             xbar[1].as<T>(), xbar[2].as<T>());
+  FS_HIP(c, hipMemcpyAsync(psibar.p, B->p, cb, hipMemcpyDeviceToDevice, c->stream));
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+// ---- reverse mode: particle side ------------------------------------------------------------
+// Kernel factor of one stencil node and its derivative along the axis:
+// w_c = N(|p_c - n_c| / hs_c), dw_c/dp_c = N'(r_c) sign(p_c - n_c) / hs_c.
+template <typename T>
+struct StencilD {
+  int b[3];
+  T w[3][4], d[3][4];
+};
+
+template <typename T>
+__device__ inline void make_stencil_d(const T p[3], int axis, const T h[3], const T hs[3], StencilD<T>& st) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const T off = (c == axis) ? T(0) : T(0.5);
+    const int nb = nearest1(p[c], off, h[c]);
+    st.b[c] = nb - 1;
+#pragma unroll
+    for (int o = 0; o < 4; o++) {
+      const T dl = p[c] - coord1(nb - 1 + o, off, h[c]);
+      const T r = fabs(dl) / hs[c];
+      st.w[c][o] = bspline(r);
+      st.d[c][o] = bspline_d(r) * (dl < T(0) ? T(-1) : T(1)) / hs[c];
+    }
+  }
+}
+
+// Sums over the in-grid nodes of one stencil: w, w*A, w*B and the position gradients of the
+// same sums (node weight (wx*wy)*wz as in sample_grid).
+template <typename T, bool TWO>
+struct NodeSums {
+  T s, a, b, ds[3], da[3], db[3];
+};
+
+template <typename T, bool TWO>
+__device__ inline NodeSums<T, TWO> node_sums(const StencilD<T>& st, const T* __restrict__ A, const T* __restrict__ Bv,
+                                             int gx, int gy, int gz) {
+  NodeSums<T, TWO> r{};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int x = st.b[0] + i;
+    if (x < 0 || x >= gx) continue;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int y = st.b[1] + j;
+      if (y < 0 || y >= gy) continue;
+      const int base = gz * (y + gy * x);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int z = st.b[2] + k;
+        if (z < 0 || z >= gz) continue;
+        const T wx = st.w[0][i], wy = st.w[1][j], wz = st.w[2][k];
+        const T w = wx * wy * wz;
+        const T dw[3] = {st.d[0][i] * wy * wz, wx * st.d[1][j] * wz, wx * wy * st.d[2][k]};
+        const T av = A[base + z];
+        const T bv = TWO ? Bv[base + z] : T(0);
+        r.s += w;
+        r.a += w * av;
+        if (TWO) r.b += w * bv;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          r.ds[d] += dw[d];
+          r.da[d] += dw[d] * av;
+          if (TWO) r.db[d] += dw[d] * bv;
+        }
+      }
+    }
+  }
+  return r;
+}
+
+template <typename T>
+struct PArr {
+  T* v[3];
+};
+
+// Reverse of solveG2P (advect-solver.cc:7-17, 157-169). Per component a (its own face grid),
+// pic = sum w g / S and old = sum w g_last / S with S = sum w; the new velocity is
+// alpha*v + pic - alpha*old (FLIP) or pic (PIC). Writes the adjoint of the particle's own
+// velocity (alpha * vbar), the transfer coefficients vbar / S of the transposed gather, and
+// adds the position derivative to xbar.
+template <typename T, bool FLIP>
+__global__ void k_g2p_adj(int n, GridArgs<T> ga, Grids3<T> G, Grids3<T> L, PArr<T> pos, PArr<T> vbar, PArr<T> xbar,
+                          PArr<T> vbar1, PArr<T> coef, T alpha) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T p[3] = {pos.v[0][i], pos.v[1][i], pos.v[2][i]};
+  const T hs[3] = {ga.h[0], ga.h[0], ga.h[0]};  // sampleVelocity's scalar spacing (util.cc:47)
+  T gp[3] = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    StencilD<T> st;
+    make_stencil_d(p, a, ga.h, hs, st);
+    const NodeSums<T, FLIP> r = node_sums<T, FLIP>(st, G.g[a], L.g[a], G.d[a][0], G.d[a][1], G.d[a][2]);
+    const T vb = vbar.v[a][i];
+    T cf = T(0);
+    if (r.s > T(0)) {
+      const T pic = r.a / r.s, old = r.b / r.s;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        T g = r.da[d] - pic * r.ds[d];
+        if (FLIP) g = g - alpha * (r.db[d] - old * r.ds[d]);
+        gp[d] += vb * g / r.s;
+      }
+      cf = vb / r.s;
+    }
+    coef.v[a][i] = cf;
+    vbar1.v[a][i] = FLIP ? alpha * vb : T(0);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) xbar.v[d][i] += gp[d];
+}
+
+// Reverse of solveP2G (advect-solver.cc:188-271): g = sum_p w v_p / W where W = sum_p w > 0.
+// With H = gbar / W (0 where W = 0) and HG = H * g per node:
+// vbar_a += sum w H, xbar += v_a * sum dw H - sum dw HG.
+template <typename T>
+__global__ void k_p2g_adj(int n, GridArgs<T> ga, Grids3<T> H, Grids3<T> HG, PArr<T> pos, PArr<T> vel, PArr<T> vbar,
+                          PArr<T> xbar) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T p[3] = {pos.v[0][i], pos.v[1][i], pos.v[2][i]};
+  T gp[3] = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    StencilD<T> st;
+    make_stencil_d(p, a, ga.h, ga.h, st);  // the transfer divides by each axis' spacing
+    const NodeSums<T, true> r = node_sums<T, true>(st, H.g[a], HG.g[a], H.d[a][0], H.d[a][1], H.d[a][2]);
+    vbar.v[a][i] += r.a;
+    const T va = vel.v[a][i];
+#pragma unroll
+    for (int d = 0; d < 3; d++) gp[d] += va * r.da[d] - r.db[d];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) xbar.v[d][i] += gp[d];
+}
+
+// Reverse of advect + handleCollision (advect-solver.cc:100-137, 170-187) on a cleared grid,
+// where k2 = k3 = 0 and p = x + (2/9) dt v. The collider push-out p -= d(p) n is
+// differentiated with its normal n held fixed (dd/dp taken as SDF::grad), the velocity
+// projection v -= min(v.n, 0) n likewise; a clamped coordinate and a velocity component the
+// wall zeroes carry no derivative. In: xbar, vbar of the outputs; out: of the inputs.
+template <typename T>
+__global__ void k_advect_adj(int n, GridArgs<T> ga, const T* __restrict__ col, PArr<T> x0, PArr<T> v0, PArr<T> xbar,
+                             PArr<T> vbar, T dt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T a1 = T(2.0 / 9.0) * dt, a2 = T(1.0 / 3.0) * dt, a3 = T(4.0 / 9.0) * dt;
+  V3<T> v = {v0.v[0][i], v0.v[1][i], v0.v[2][i]};
+  V3<T> p = {x0.v[0][i] + a1 * v.x + a2 * T(0) + a3 * T(0), x0.v[1][i] + a1 * v.y + a2 * T(0) + a3 * T(0),
+             x0.v[2][i] + a1 * v.z + a2 * T(0) + a3 * T(0)};
+  const T d = sdf_eval(col, ga.nx, ga.ny, ga.nz, ga.h, p.x, p.y, p.z);
+  const bool inside = d < T(0);
+  V3<T> g = {T(0), T(0), T(0)}, nr = {T(0), T(0), T(0)};
+  bool vproj = false;
+  if (inside) {
+    g = sdf_grad(col, ga.nx, ga.ny, ga.nz, ga.h, p.x, p.y, p.z);
+    nr = gnormalize(g);
+    p.x = p.x - d * nr.x;
+    p.y = p.y - d * nr.y;
+    p.z = p.z - d * nr.z;
+    const T dv = v.x * nr.x + v.y * nr.y + v.z * nr.z;
+    vproj = dv < T(0);
+    const T m = smin(dv, T(0));
+    v.x = v.x - m * nr.x;
+    v.y = v.y - m * nr.y;
+    v.z = v.z - m * nr.z;
+  }
+  T pb[3] = {xbar.v[0][i], xbar.v[1][i], xbar.v[2][i]};
+  T vb[3] = {vbar.v[0][i], vbar.v[1][i], vbar.v[2][i]};
+  const T pc[3] = {p.x, p.y, p.z}, vc[3] = {v.x, v.y, v.z};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const bool lo = pc[c] < T(0), hi = pc[c] > ga.ext[c];
+    if (lo || hi) pb[c] = T(0);
+    if ((lo && vc[c] < T(0)) || (hi && vc[c] > T(0))) vb[c] = T(0);
+  }
+  if (inside) {
+    if (vproj) {
+      const T t = nr.x * vb[0] + nr.y * vb[1] + nr.z * vb[2];
+      vb[0] -= nr.x * t;
+      vb[1] -= nr.y * t;
+      vb[2] -= nr.z * t;
+    }
+    const T t = nr.x * pb[0] + nr.y * pb[1] + nr.z * pb[2];
+    pb[0] -= g.x * t;
+    pb[1] -= g.y * t;
+    pb[2] -= g.z * t;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    xbar.v[c][i] = pb[c];
+    vbar.v[c][i] = vb[c] + a1 * pb[c];
+  }
+}
+
+template <typename T>
+PArr<T> parr(DBuf* b) {
+  return {{b[0].as<T>(), b[1].as<T>(), b[2].as<T>()}};
+}
+template <typename T>
+Grids3<T> grids3(const fs_ctx* c, DBuf* b) {
+  Grids3<T> G;
+  for (int a = 0; a < 3; a++) {
+    G.g[a] = b[a].as<T>();
+    for (int d = 0; d < 3; d++) G.d[a][d] = c->fdim[a][d];
+  }
+  return G;
+}
+
+// out_a = sum_p w_pa(node) coef_a[p] for every node of face grid a (the transposed
+// transfer), binned like P2G so the sums are deterministic.
+template <typename T>
+int p2g_sum_t(fs_ctx* c, DBuf* pos, DBuf* coef, DBuf* out) {
+  GridArgs<T> ga = grid_args<T>(c);
+  const int n = (int)c->np;
+  const int nb = (c->nx + 1) * (c->ny + 1) * (c->nz + 1);
+  for (int a = 0; a < 3; a++) {
+    if (n > 0)
+      k_keys_nearest<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, a, ga, pos[0].as<T>(), pos[1].as<T>(),
+                                                                  pos[2].as<T>(), c->keys.as<int>());
+    FS_TRY(counting_sort(c, nb));
+    if (n > 0)
+      k_p2g_records<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, a, ga, c->perm.as<int>(), pos[0].as<T>(),
+                                                                 pos[1].as<T>(), pos[2].as<T>(), coef[a].as<T>(),
+                                                                 c->p2g_w.as<T>(), c->sorted[0].as<T>());
+    const int gx = c->fdim[a][0], gy = c->fdim[a][1], gz = c->fdim[a][2];
+    k_p2g_gather<T, false><<<fs_blocks(p2g_gather_threads(gx, gy, gz), TPB), TPB, 0, c->stream>>>(
+        ga, gx, gy, gz, n, c->offs.as<int>(), c->p2g_w.as<T>(), c->sorted[0].as<T>(), out[a].as<T>(), nullptr,
+        nullptr);
+    FS_HIP(c, hipGetLastError());
+  }
+  return FS_OK;
+}
+
+template <typename T>
+int adj_g2p_t(fs_ctx* c, DBuf* x1, DBuf* g5, DBuf* g0, DBuf* vbar, DBuf* xbar, DBuf* vbar1, DBuf* coef) {
+  const int n = (int)c->np;
+  if (n == 0) return FS_OK;
+  GridArgs<T> ga = grid_args<T>(c);
+  const bool flip = c->cfg.advection == FS_ADVECT_FLIP;
+  const T alpha = (T)c->cfg.flip_alpha;
+  if (flip)
+    k_g2p_adj<T, true><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, grids3<T>(c, g5), grids3<T>(c, g0), parr<T>(x1),
+                                                                 parr<T>(vbar), parr<T>(xbar), parr<T>(vbar1),
+                                                                 parr<T>(coef), alpha);
+  else
+    k_g2p_adj<T, false><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, grids3<T>(c, g5), grids3<T>(c, g5),
+                                                                  parr<T>(x1), parr<T>(vbar), parr<T>(xbar),
+                                                                  parr<T>(vbar1), parr<T>(coef), alpha);
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+template <typename T>
+int adj_p2g_t(fs_ctx* c, DBuf* x1, DBuf* v1, DBuf* H, DBuf* HG, DBuf* vbar, DBuf* xbar) {
+  const int n = (int)c->np;
+  if (n == 0) return FS_OK;
+  k_p2g_adj<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, grid_args<T>(c), grids3<T>(c, H), grids3<T>(c, HG),
+                                                         parr<T>(x1), parr<T>(v1), parr<T>(vbar), parr<T>(xbar));
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
+template <typename T>
+int adj_advect_t(fs_ctx* c, DBuf* x0, DBuf* v0, DBuf* xbar, DBuf* vbar, double dt) {
+  const int n = (int)c->np;
+  if (n == 0) return FS_OK;
+  k_advect_adj<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, grid_args<T>(c), c->collider.as<T>(), parr<T>(x0),
+                                                            parr<T>(v0), parr<T>(xbar), parr<T>(vbar), (T)dt);
+  FS_HIP(c, hipGetLastError());
+  return FS_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -1126,6 +1677,19 @@ int fs_stage_project(fs_ctx* c, double dt) { return FS_DISPATCH(c, project_t, dt
 int fs_stage_collider(fs_ctx* c) { return FS_DISPATCH(c, collider_t); }
 int fs_stage_g2p(fs_ctx* c) { return FS_DISPATCH(c, g2p_t); }
 int fs_stage_cfl(fs_ctx* c, double* out) { return FS_DISPATCH(c, cfl_t, out); }
+int fs_adj_p2g_sum(fs_ctx* c, DBuf* pos, DBuf* coef, DBuf* out) { return FS_DISPATCH(c, p2g_sum_t, pos, coef, out); }
+int fs_adj_g2p(fs_ctx* c, DBuf* x1, DBuf* g5, DBuf* g0, DBuf* vbar, DBuf* xbar, DBuf* vbar1, DBuf* coef) {
+  return FS_DISPATCH(c, adj_g2p_t, x1, g5, g0, vbar, xbar, vbar1, coef);
+}
+int fs_adj_p2g(fs_ctx* c, DBuf* x1, DBuf* v1, DBuf* H, DBuf* HG, DBuf* vbar, DBuf* xbar) {
+  return FS_DISPATCH(c, adj_p2g_t, x1, v1, H, HG, vbar, xbar);
+}
+int fs_adj_advect(fs_ctx* c, DBuf* x0, DBuf* v0, DBuf* xbar, DBuf* vbar, double dt) {
+  return FS_DISPATCH(c, adj_advect_t, x0, v0, xbar, vbar, dt);
+}
+int fs_adj_sdf(fs_ctx* c, AdjStep& t, DBuf& phibar, DBuf& psibar, DBuf* xbar) {
+  return FS_DISPATCH(c, adj_sdf_t, t, phibar, psibar, xbar);
+}
 
 int fs_upload_real(fs_ctx* c, DBuf& dst, const double* src, size_t n) {
   if (!c->mixed) {

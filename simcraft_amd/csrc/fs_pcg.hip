@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "fs_ctx.h"
@@ -259,8 +260,9 @@ __global__ void k_brick_flags(int nx, int ny, int nz, const uint8_t* __restrict_
 
 // Pack the dense system into the brick layout; neighbour-activity masks; factor flags and
 // tau numerators (tau*A(c))*A(diagonal neighbour) of ModifiedIncompleteCholesky3D::precond.
-__global__ void k_pack(Lay L, double tau, const double* __restrict__ Adiag, const double* __restrict__ AL,
-                       const double* __restrict__ AD, const double* __restrict__ AB, const double* __restrict__ rhs,
+template <typename TI>
+__global__ void k_pack(Lay L, double tau, const TI* __restrict__ Adiag, const TI* __restrict__ AL,
+                       const TI* __restrict__ AD, const TI* __restrict__ AB, const TI* __restrict__ rhs,
                        const uint8_t* __restrict__ act, double* __restrict__ padiag, double* __restrict__ pal,
                        double* __restrict__ pad, double* __restrict__ pab, double* __restrict__ prhs,
                        uint8_t* __restrict__ nbm, uint16_t* __restrict__ tfl, double* __restrict__ tnum,
@@ -296,33 +298,33 @@ __global__ void k_pack(Lay L, double tau, const double* __restrict__ Adiag, cons
           tf |= TF_GL;
           if (j < ny - 1 && act[c - sx + sy]) {
             tf |= TF_L1;
-            t6[0] = tau * vl * AL[c - sx + sy];
+            t6[0] = tau * vl * (double)AL[c - sx + sy];
           }
           if (k < nz - 1 && act[c - sx + 1]) {
             tf |= TF_L2;
-            t6[1] = tau * vl * AL[c - sx + 1];
+            t6[1] = tau * vl * (double)AL[c - sx + 1];
           }
         }
         if (j > 0 && act[c - sy] && Adiag[c - sy] != 0.0) {
           tf |= TF_GD;
           if (i < nx - 1 && act[c + sx - sy]) {
             tf |= TF_D1;
-            t6[2] = tau * vdn * AD[c + sx - sy];
+            t6[2] = tau * vdn * (double)AD[c + sx - sy];
           }
           if (k < nz - 1 && act[c - sy + 1]) {
             tf |= TF_D2;
-            t6[3] = tau * vdn * AD[c - sy + 1];
+            t6[3] = tau * vdn * (double)AD[c - sy + 1];
           }
         }
         if (k > 0 && act[c - 1] && Adiag[c - 1] != 0.0) {
           tf |= TF_GB;
           if (i < nx - 1 && act[c + sx - 1]) {
             tf |= TF_B1;
-            t6[4] = tau * vb * AB[c + sx - 1];
+            t6[4] = tau * vb * (double)AB[c + sx - 1];
           }
           if (j < ny - 1 && act[c + sy - 1]) {
             tf |= TF_B2;
-            t6[5] = tau * vb * AB[c + sy - 1];
+            t6[5] = tau * vb * (double)AB[c + sy - 1];
           }
         }
       }
@@ -428,10 +430,12 @@ __global__ __launch_bounds__(64) void k_factor_sweep(Lay L, double sigma, const 
 }
 
 // Sweep coefficients: forward cL = AL(c)*E(i-1) etc., backward dR = AL(i+1)*E(c) etc.
+// Es (optional) receives E in the storage type.
+template <typename S>
 __global__ void k_coef(Lay L, const double* __restrict__ al, const double* __restrict__ ad,
                        const double* __restrict__ ab, const uint8_t* __restrict__ nbm, const double* __restrict__ E,
-                       double* __restrict__ cl, double* __restrict__ cd, double* __restrict__ cb,
-                       double* __restrict__ dr, double* __restrict__ du, double* __restrict__ df) {
+                       S* __restrict__ cl, S* __restrict__ cd, S* __restrict__ cb, S* __restrict__ dr,
+                       S* __restrict__ du, S* __restrict__ df, S* __restrict__ Es) {
   int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= L.ne) return;
   int B, s, a, b, i, j, k;
@@ -447,16 +451,18 @@ __global__ void k_coef(Lay L, const double* __restrict__ al, const double* __res
     if (m & NB_U) vu = ad[nb_elem(L, B, s, a, b, 3)] * ec;
     if (m & NB_F) vf = ab[e + 64] * ec;
   }
-  cl[e] = vl;
-  cd[e] = vd;
-  cb[e] = vb;
-  dr[e] = vr;
-  du[e] = vu;
-  df[e] = vf;
+  cl[e] = (S)vl;
+  cd[e] = (S)vd;
+  cb[e] = (S)vb;
+  dr[e] = (S)vr;
+  du[e] = (S)vu;
+  df[e] = (S)vf;
+  if (Es) Es[e] = (S)E[e];
 }
 
 // r = rhs, ||r||_inf and ||r||_2^2 (fixed tree); the last block decides "naturally converged".
-__global__ __launch_bounds__(256) void k_init(Lay L, const double* __restrict__ rhs, double* __restrict__ r,
+template <typename S>
+__global__ __launch_bounds__(256) void k_init(Lay L, const double* __restrict__ rhs, S* __restrict__ r,
                                               double* __restrict__ part, double* __restrict__ part2, int nchunks,
                                               PcgScalars* sc, int tol_mode, double tol) {
   __shared__ double sm[4];
@@ -466,7 +472,7 @@ __global__ __launch_bounds__(256) void k_init(Lay L, const double* __restrict__ 
     double v = 0.0;
     if (e < L.ne) {
       v = rhs[e];
-      r[e] = v;
+      r[e] = (S)v;
     }
     double av = fabs(v);
     mx = (mx < av) ? av : mx;
@@ -497,11 +503,13 @@ __global__ __launch_bounds__(256) void k_init(Lay L, const double* __restrict__ 
 
 // z = A s with the fixed-tree s.z (applyCompressedMatrix + dotProduct(s, z)); last block
 // forms alpha = sigma / s.z.
-template <bool PCG>
-__global__ __launch_bounds__(256) void k_spmv(Lay L, const double* __restrict__ x, const double* __restrict__ adiag,
-                                              const double* __restrict__ al, const double* __restrict__ ad,
-                                              const double* __restrict__ ab, const uint8_t* __restrict__ nbm,
-                                              double* __restrict__ y, double* __restrict__ part, int nchunks,
+// S is the storage and arithmetic type of the vectors and the stencil; the dot product is
+// accumulated in FP64 either way.
+template <bool PCG, typename S>
+__global__ __launch_bounds__(256) void k_spmv(Lay L, const S* __restrict__ x, const S* __restrict__ adiag,
+                                              const S* __restrict__ al, const S* __restrict__ ad,
+                                              const S* __restrict__ ab, const uint8_t* __restrict__ nbm,
+                                              S* __restrict__ y, double* __restrict__ part, int nchunks,
                                               PcgScalars* sc, int iter) {
   __shared__ double sm[4];
   if (PCG && sc->converged != STOP_RUN) return;
@@ -510,8 +518,8 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const double* __restrict__ 
     int64_t e = (int64_t)blockIdx.x * CH + m * 256 + threadIdx.x;
     if (e >= L.ne) continue;
     const int f = nbm[e];
-    double v = 0.0;
-    const double xc = x[e];
+    S v = S(0);
+    const S xc = x[e];
     if (f & NB_SELF) {
       int B, s, a, b, i, j, k;
       decode(L, e, B, s, a, b, i, j, k);
@@ -536,7 +544,7 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const double* __restrict__ 
       if (f & NB_F) v = v + ab[e + 64] * x[e + 64];
     }
     y[e] = v;
-    acc += xc * v;
+    acc += (double)xc * (double)v;
   }
   if (!PCG) return;
   double bs = block_sum256(acc, sm);
@@ -551,6 +559,80 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const double* __restrict__ 
         sc->conv_iter = iter;
       } else {
         sc->alpha = sc->sigma / sdotz;
+      }
+    }
+  }
+}
+
+// Mixed precision: the FP64 residual b - A x of the FP64 iterate (A is the FP32 stencil, held
+// exactly in FP64; the products follow applyCompressedMatrix's order, project-solver.cc:23-35),
+// written into the FP32 recurrence residual. Its FP64 norms replace the recurrence's and
+// decide convergence (tolerance as in k_fwd).
+__global__ __launch_bounds__(256) void k_resid(Lay L, const double* __restrict__ rhs, const double* __restrict__ x,
+                                               const double* __restrict__ adiag, const double* __restrict__ al,
+                                               const double* __restrict__ ad, const double* __restrict__ ab,
+                                               const uint8_t* __restrict__ nbm, float* __restrict__ r,
+                                               double* __restrict__ part, double* __restrict__ part2, int nchunks,
+                                               PcgScalars* sc, int iter, int tol_mode, double tol) {
+  __shared__ double sm[4];
+  double mx = 0.0, ss = 0.0;
+  for (int m = 0; m < 16; m++) {
+    int64_t e = (int64_t)blockIdx.x * CH + m * 256 + threadIdx.x;
+    if (e >= L.ne) continue;
+    const int f = nbm[e];
+    double res = 0.0;
+    if (f & NB_SELF) {
+      int B, s, a, b, i, j, k;
+      decode(L, e, B, s, a, b, i, j, k);
+      double v = adiag[e] * x[e];
+      if (f & NB_L) {
+        int64_t n = nb_elem(L, B, s, a, b, 0);
+        v = v + al[e] * x[n];
+      }
+      if (f & NB_D) {
+        int64_t n = nb_elem(L, B, s, a, b, 2);
+        v = v + ad[e] * x[n];
+      }
+      if (f & NB_B) v = v + ab[e] * x[e - 64];
+      if (f & NB_R) {
+        int64_t n = nb_elem(L, B, s, a, b, 1);
+        v = v + al[n] * x[n];
+      }
+      if (f & NB_U) {
+        int64_t n = nb_elem(L, B, s, a, b, 3);
+        v = v + ad[n] * x[n];
+      }
+      if (f & NB_F) v = v + ab[e + 64] * x[e + 64];
+      res = rhs[e] - v;
+    }
+    r[e] = (float)res;
+    const double av = fabs(res);
+    mx = (mx < av) ? av : mx;
+    ss += res * res;
+  }
+  double bm = block_max256(mx, sm);
+  double bs = block_sum256(ss, sm);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = bs;
+    part2[blockIdx.x] = bm;
+  }
+  if (last_block(&sc->done_ctr[DC_INIT], gridDim.x, 256)) {
+    double tot = tree256(part, nchunks, sm);
+    double mxx = 0.0;
+    for (int q = 0; q < nchunks; q++) mxx = (mxx < part2[q]) ? part2[q] : mxx;
+    if (threadIdx.x == 0) {
+      sc->residual = mxx;
+      sc->rnorm2sq = tot;
+      const bool conv = tol_mode == FS_TOL_REL_L2 ? (sqrt(tot) <= tol * sqrt(sc->bnorm2sq)) : (mxx < tol);
+      if (!(mxx <= 1.7976931348623157e308)) {
+        sc->converged = STOP_ABORT;
+        sc->status = FS_NONFINITE;
+        sc->conv_iter = iter;
+      } else if (conv) {
+        sc->converged = STOP_CONVERGED;
+        sc->conv_iter = iter;
+      } else {
+        sc->converged = STOP_RUN;
       }
     }
   }
@@ -590,18 +672,26 @@ __device__ __forceinline__ double dpp_row(double v) {
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(x >> 32), CTRL, 0xF, 0xF, true);
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
+template <int CTRL>
+__device__ __forceinline__ float dpp_row(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
 constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHL1 = 0x101;
 
 // Forward sweep L q = r (project-solver.cc:275-288). UPDATE: first r += (-alpha) t and
 // ||r||_inf (saxpy + LinfNorm, :333-335); the last brick decides convergence.
+// V: storage and arithmetic type of vectors and coefficients; the exchange entries stay FP64.
+template <typename V>
 struct FwdSet {
-  double r[U], t[U], c1[U], c2[U], c3[U], e[U], xl[U], xd[U];
+  V r[U], t[U], c1[U], c2[U], c3[U], e[U];
+  double xl[U], xd[U];
 };
 
+template <typename V>
 struct FwdK {  // wave-uniform
-  double* r;
-  const double *t, *cl, *cd, *cb, *E;
-  double* q;        // brick bases: element B*S*64
+  V* r;
+  const V *t, *cl, *cd, *cb, *E;
+  V* q;             // brick bases: element B*S*64
   double* xL;       // f 0 entries of the -x neighbour, nullptr when there is none
   double* xD;       // f 1 entries of the -y neighbour
   double* oL;       // this brick's f 0 entries (read by its +x neighbour)
@@ -611,15 +701,15 @@ struct FwdK {  // wave-uniform
   double nalpha;
 };
 
-template <bool UPDATE>
-__device__ __forceinline__ void fwd_load(const FwdK& k, FwdSet& S, int c, int lane) {
+template <bool UPDATE, typename T>
+__device__ __forceinline__ void fwd_load(const FwdK<T>& k, FwdSet<T>& S, int c, int lane) {
   c = min(c, k.NC - 1);  // past the end: reload the last chunk (unused) rather than branch
   const size_t o = (size_t)c * U * 64;
 #pragma unroll
   for (int u = 0; u < U; u++) {
     const int i = u * 64 + lane;
     S.r[u] = k.r[o + i];
-    S.t[u] = UPDATE ? k.t[o + i] : 0.0;
+    S.t[u] = UPDATE ? k.t[o + i] : T(0);
     S.c1[u] = k.cl[o + i];
     S.c2[u] = k.cd[o + i];
     S.c3[u] = k.cb[o + i];
@@ -627,7 +717,8 @@ __device__ __forceinline__ void fwd_load(const FwdK& k, FwdSet& S, int c, int la
   }
 }
 
-__device__ __forceinline__ void fwd_cross(const FwdK& k, FwdSet& S, int c, int a, int b) {
+template <typename T>
+__device__ __forceinline__ void fwd_cross(const FwdK<T>& k, FwdSet<T>& S, int c, int a, int b) {
   c = min(c, k.NC - 1);
 #pragma unroll
   for (int u = 0; u < U; u++) {
@@ -640,7 +731,8 @@ __device__ __forceinline__ void fwd_cross(const FwdK& k, FwdSet& S, int c, int a
 }
 
 // Wave-uniform: every boundary entry of the chunk had been written when it was loaded.
-__device__ __forceinline__ bool fwd_ready(const FwdSet& S, int a, int b, bool give_up) {
+template <typename T>
+__device__ __forceinline__ bool fwd_ready(const FwdSet<T>& S, int a, int b, bool give_up) {
   bool wait = false;
 #pragma unroll
   for (int u = 0; u < U; u++) wait |= ((a == 0) & xunset(S.xl[u])) | ((b == 0) & xunset(S.xd[u]));
@@ -648,7 +740,8 @@ __device__ __forceinline__ bool fwd_ready(const FwdSet& S, int a, int b, bool gi
 }
 
 // Wait until the boundary entries of chunks c .. c+lag are written; true if a wait gave up.
-__device__ __forceinline__ bool fwd_wait(const FwdK& k, int c, int a, int b, PcgScalars* sc) {
+template <typename T>
+__device__ __forceinline__ bool fwd_wait(const FwdK<T>& k, int c, int a, int b, PcgScalars* sc) {
   bool ok = true;
   const int c1 = min(c + k.lag, k.NC - 1);
   for (int s = c * U; s < (c1 + 1) * U && s + 7 < k.S; s++) {
@@ -658,8 +751,8 @@ __device__ __forceinline__ bool fwd_wait(const FwdK& k, int c, int a, int b, Pcg
   return __builtin_amdgcn_ballot_w64(!ok) != 0;
 }
 
-template <bool UPDATE>
-__device__ __forceinline__ void fwd_run(const FwdK& k, const FwdSet& S, int c, int lane, int a, int b, double& qprev,
+template <bool UPDATE, typename T>
+__device__ __forceinline__ void fwd_run(const FwdK<T>& k, const FwdSet<T>& S, int c, int lane, int a, int b, T& qprev,
                                         double& mx, double& ss) {
   const int s0 = c * U;
   const size_t o = (size_t)s0 * 64;
@@ -667,25 +760,25 @@ __device__ __forceinline__ void fwd_run(const FwdK& k, const FwdSet& S, int c, i
   for (int u = 0; u < U; u++) {
     const int s = s0 + u;
     const int i = u * 64 + lane;
-    double rr = S.r[u];
+    T rr = S.r[u];
     if (UPDATE) {
-      rr = rr + k.nalpha * S.t[u];
+      rr = rr + (T)k.nalpha * S.t[u];
       k.r[o + i] = rr;
-      const double av = fabs(rr);
+      const double av = fabs((double)rr);
       mx = (mx < av) ? av : mx;
-      ss += rr * rr;
+      ss += (double)rr * (double)rr;
     }
-    double qL = __shfl_up(qprev, 8, 64);
-    double qD = dpp_row<DPP_ROW_SHR1>(qprev);
-    qL = a == 0 ? S.xl[u] : qL;
-    qD = b == 0 ? S.xd[u] : qD;
-    double tt = rr - S.c1[u] * qL;
+    T qL = __shfl_up(qprev, 8, 64);
+    T qD = dpp_row<DPP_ROW_SHR1>(qprev);
+    qL = a == 0 ? (T)S.xl[u] : qL;
+    qD = b == 0 ? (T)S.xd[u] : qD;
+    T tt = rr - S.c1[u] * qL;
     tt = tt - S.c2[u] * qD;
     tt = tt - S.c3[u] * qprev;
-    const double qv = tt * S.e[u];
+    const T qv = tt * S.e[u];
     k.q[o + i] = qv;
-    if (a == 7) xstore(k.oL + (size_t)s * 8 + b, qv);
-    if (b == 7) xstore(k.oD + (size_t)s * 8 + a, qv);
+    if (a == 7) xstore(k.oL + (size_t)s * 8 + b, (double)qv);
+    if (b == 7) xstore(k.oD + (size_t)s * 8 + a, (double)qv);
     qprev = qv;
   }
 #pragma unroll
@@ -698,11 +791,11 @@ __device__ __forceinline__ void fwd_run(const FwdK& k, const FwdSet& S, int c, i
   }
 }
 
-template <bool UPDATE>
-__global__ __launch_bounds__(64) void k_fwd(Lay L, double* __restrict__ r, const double* __restrict__ t,
-                                            const double* __restrict__ cl, const double* __restrict__ cd,
-                                            const double* __restrict__ cb, const double* __restrict__ E,
-                                            double* __restrict__ q, double* X, double* __restrict__ part2,
+template <bool UPDATE, typename T>
+__global__ __launch_bounds__(64) void k_fwd(Lay L, T* __restrict__ r, const T* __restrict__ t,
+                                            const T* __restrict__ cl, const T* __restrict__ cd,
+                                            const T* __restrict__ cb, const T* __restrict__ E,
+                                            T* __restrict__ q, double* X, double* __restrict__ part2,
                                             double* __restrict__ part, uint64_t* ticket, uint64_t tbase,
                                             const int* __restrict__ order, PcgScalars* sc, int iter, int tol_mode,
                                             double tol, int lag) {
@@ -714,7 +807,7 @@ __global__ __launch_bounds__(64) void k_fwd(Lay L, double* __restrict__ r, const
   const int2 ij = L.bij[B];
   const int BL = __builtin_amdgcn_readfirstlane(ij.x > 0 ? L.live[(ij.x - 1) * L.nbj + ij.y] : -1);
   const int BD = __builtin_amdgcn_readfirstlane(ij.y > 0 ? L.live[ij.x * L.nbj + ij.y - 1] : -1);
-  FwdK k;
+  FwdK<T> k;
   k.r = r + brick;
   k.t = t + brick;
   k.cl = cl + brick;
@@ -732,27 +825,28 @@ __global__ __launch_bounds__(64) void k_fwd(Lay L, double* __restrict__ r, const
   k.lag = lag;
   k.nalpha = UPDATE ? -sc->alpha : 0.0;
   bool give_up = false;
-  FwdSet A, Bs;
-  double qprev = 0.0, mx = 0.0, ss = 0.0;
+  FwdSet<T> A, Bs;
+  T qprev = T(0);
+  double mx = 0.0, ss = 0.0;
   for (int c0 = 0;;) {  // (re)start the pipeline at chunk c0
-    fwd_load<UPDATE>(k, A, c0, lane);
+    fwd_load<UPDATE, T>(k, A, c0, lane);
     fwd_cross(k, A, c0, a, b);
     int bail = -1, c = c0;
     // Whole pairs of chunks only: an exit between the two halves would merge register states
     // at the loop latch and force every load to complete there.
     for (; c + 1 < k.NC; c += 2) {
-      fwd_load<UPDATE>(k, Bs, c + 1, lane);
+      fwd_load<UPDATE, T>(k, Bs, c + 1, lane);
       fwd_cross(k, Bs, c + 1, a, b);
       if (!fwd_ready(A, a, b, give_up)) { bail = c; break; }
-      fwd_run<UPDATE>(k, A, c, lane, a, b, qprev, mx, ss);
-      fwd_load<UPDATE>(k, A, c + 2, lane);
+      fwd_run<UPDATE, T>(k, A, c, lane, a, b, qprev, mx, ss);
+      fwd_load<UPDATE, T>(k, A, c + 2, lane);
       fwd_cross(k, A, c + 2, a, b);
       if (!fwd_ready(Bs, a, b, give_up)) { bail = c + 1; break; }
-      fwd_run<UPDATE>(k, Bs, c + 1, lane, a, b, qprev, mx, ss);
+      fwd_run<UPDATE, T>(k, Bs, c + 1, lane, a, b, qprev, mx, ss);
     }
     if (bail < 0 && c < k.NC) {  // odd chunk count: A holds the last chunk
       if (!fwd_ready(A, a, b, give_up)) bail = c;
-      else fwd_run<UPDATE>(k, A, c, lane, a, b, qprev, mx, ss);
+      else fwd_run<UPDATE, T>(k, A, c, lane, a, b, qprev, mx, ss);
     }
     if (bail < 0) break;
     give_up = fwd_wait(k, bail, a, b, sc);
@@ -788,13 +882,16 @@ __global__ __launch_bounds__(64) void k_fwd(Lay L, double* __restrict__ r, const
 // Backward sweep L^T z = q (project-solver.cc:289-305) and the fixed-tree z.r; the last brick
 // forms beta = sigma_new / sigma (INIT: sigma = z.r). Slots run from S-1 down to 0; chunk c
 // covers slots S-1-c*U down to S-U-c*U.
+template <typename V>
 struct BwdSet {
-  double q[U], d1[U], d2[U], d3[U], e[U], r[U], xr[U], xu[U];
+  V q[U], d1[U], d2[U], d3[U], e[U], r[U];
+  double xr[U], xu[U];
 };
 
+template <typename V>
 struct BwdK {  // wave-uniform
-  const double *q, *d1, *d2, *d3, *E, *r;
-  double* z;
+  const V *q, *d1, *d2, *d3, *E, *r;
+  V* z;
   double* xR;  // f 0 entries of the +x neighbour, nullptr when there is none
   double* xU;  // f 1 entries of the +y neighbour
   double* oR;  // this brick's f 0 entries (read by its -x neighbour)
@@ -804,9 +901,11 @@ struct BwdK {  // wave-uniform
 };
 
 // element offset of the lowest slot of chunk c; slot S-1-c*U-u is at +(U-1-u)*64
-__device__ __forceinline__ size_t bwd_base(const BwdK& k, int c) { return (size_t)(k.S - (c + 1) * U) * 64; }
+template <typename T>
+__device__ __forceinline__ size_t bwd_base(const BwdK<T>& k, int c) { return (size_t)(k.S - (c + 1) * U) * 64; }
 
-__device__ __forceinline__ void bwd_load(const BwdK& k, BwdSet& S, int c, int lane) {
+template <typename T>
+__device__ __forceinline__ void bwd_load(const BwdK<T>& k, BwdSet<T>& S, int c, int lane) {
   c = min(c, k.NC - 1);
   const size_t o = bwd_base(k, c);
 #pragma unroll
@@ -821,7 +920,8 @@ __device__ __forceinline__ void bwd_load(const BwdK& k, BwdSet& S, int c, int la
   }
 }
 
-__device__ __forceinline__ void bwd_cross(const BwdK& k, BwdSet& S, int c, int a, int b) {
+template <typename T>
+__device__ __forceinline__ void bwd_cross(const BwdK<T>& k, BwdSet<T>& S, int c, int a, int b) {
   c = min(c, k.NC - 1);
 #pragma unroll
   for (int u = 0; u < U; u++) {
@@ -833,14 +933,16 @@ __device__ __forceinline__ void bwd_cross(const BwdK& k, BwdSet& S, int c, int a
   }
 }
 
-__device__ __forceinline__ bool bwd_ready(const BwdSet& S, int a, int b, bool give_up) {
+template <typename T>
+__device__ __forceinline__ bool bwd_ready(const BwdSet<T>& S, int a, int b, bool give_up) {
   bool wait = false;
 #pragma unroll
   for (int u = 0; u < U; u++) wait |= ((a == 7) & xunset(S.xr[u])) | ((b == 7) & xunset(S.xu[u]));
   return give_up || __builtin_amdgcn_ballot_w64(wait) == 0;
 }
 
-__device__ __forceinline__ bool bwd_wait(const BwdK& k, int c, int a, int b, PcgScalars* sc) {
+template <typename T>
+__device__ __forceinline__ bool bwd_wait(const BwdK<T>& k, int c, int a, int b, PcgScalars* sc) {
   bool ok = true;
   const int c1 = min(c + k.lag, k.NC - 1);
   for (int s = k.S - 1 - c * U; s > k.S - 1 - (c1 + 1) * U && s >= 7; s--) {
@@ -850,7 +952,8 @@ __device__ __forceinline__ bool bwd_wait(const BwdK& k, int c, int a, int b, Pcg
   return __builtin_amdgcn_ballot_w64(!ok) != 0;
 }
 
-__device__ __forceinline__ void bwd_run(const BwdK& k, const BwdSet& S, int c, int lane, int a, int b, double& zprev,
+template <typename T>
+__device__ __forceinline__ void bwd_run(const BwdK<T>& k, const BwdSet<T>& S, int c, int lane, int a, int b, T& zprev,
                                         double& acc) {
   const size_t o = bwd_base(k, c);
   const int s0 = k.S - 1 - c * U;
@@ -858,19 +961,19 @@ __device__ __forceinline__ void bwd_run(const BwdK& k, const BwdSet& S, int c, i
   for (int u = 0; u < U; u++) {
     const int s = s0 - u;
     const int i = (U - 1 - u) * 64 + lane;
-    double zR = __shfl_down(zprev, 8, 64);
-    double zU = dpp_row<DPP_ROW_SHL1>(zprev);
-    zR = a == 7 ? S.xr[u] : zR;
-    zU = b == 7 ? S.xu[u] : zU;
-    double tt = S.q[u] - S.d1[u] * zR;
+    T zR = __shfl_down(zprev, 8, 64);
+    T zU = dpp_row<DPP_ROW_SHL1>(zprev);
+    zR = a == 7 ? (T)S.xr[u] : zR;
+    zU = b == 7 ? (T)S.xu[u] : zU;
+    T tt = S.q[u] - S.d1[u] * zR;
     tt = tt - S.d2[u] * zU;
     tt = tt - S.d3[u] * zprev;
-    const double zv = tt * S.e[u];
+    const T zv = tt * S.e[u];
     k.z[o + i] = zv;
-    if (a == 0) xstore(k.oR + (size_t)s * 8 + b, zv);
-    if (b == 0) xstore(k.oU + (size_t)s * 8 + a, zv);
+    if (a == 0) xstore(k.oR + (size_t)s * 8 + b, (double)zv);
+    if (b == 0) xstore(k.oU + (size_t)s * 8 + a, (double)zv);
     zprev = zv;
-    acc += zv * S.r[u];
+    acc += (double)zv * (double)S.r[u];
   }
 #pragma unroll
   for (int u = 0; u < U; u++) {
@@ -882,11 +985,11 @@ __device__ __forceinline__ void bwd_run(const BwdK& k, const BwdSet& S, int c, i
   }
 }
 
-template <bool INIT>
-__global__ __launch_bounds__(64) void k_bwd(Lay L, const double* __restrict__ qv, const double* __restrict__ dr,
-                                            const double* __restrict__ du, const double* __restrict__ df,
-                                            const double* __restrict__ E, const double* __restrict__ r,
-                                            double* __restrict__ z, double* X, double* __restrict__ part,
+template <bool INIT, typename T>
+__global__ __launch_bounds__(64) void k_bwd(Lay L, const T* __restrict__ qv, const T* __restrict__ dr,
+                                            const T* __restrict__ du, const T* __restrict__ df,
+                                            const T* __restrict__ E, const T* __restrict__ r,
+                                            T* __restrict__ z, double* X, double* __restrict__ part,
                                             uint64_t* ticket, uint64_t tbase, const int* __restrict__ order,
                                             PcgScalars* sc, int lag) {
   const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
@@ -897,7 +1000,7 @@ __global__ __launch_bounds__(64) void k_bwd(Lay L, const double* __restrict__ qv
   const int2 ij = L.bij[B];
   const int BR = __builtin_amdgcn_readfirstlane(ij.x + 1 < L.nbi ? L.live[(ij.x + 1) * L.nbj + ij.y] : -1);
   const int BU = __builtin_amdgcn_readfirstlane(ij.y + 1 < L.nbj ? L.live[ij.x * L.nbj + ij.y + 1] : -1);
-  BwdK k;
+  BwdK<T> k;
   k.q = qv + brick;
   k.d1 = dr + brick;
   k.d2 = du + brick;
@@ -914,8 +1017,9 @@ __global__ __launch_bounds__(64) void k_bwd(Lay L, const double* __restrict__ qv
   k.NC = L.S / U;
   k.lag = lag;
   bool give_up = false;
-  BwdSet A, Bs;
-  double zprev = 0.0, acc = 0.0;
+  BwdSet<T> A, Bs;
+  T zprev = T(0);
+  double acc = 0.0;
   for (int c0 = 0;;) {
     bwd_load(k, A, c0, lane);
     bwd_cross(k, A, c0, a, b);
@@ -954,32 +1058,42 @@ __global__ __launch_bounds__(64) void k_bwd(Lay L, const double* __restrict__ qv
 }
 
 // pg += alpha s (saxpy, :332); s = z + beta s (scaleAndAdd, :341) unless the solve stopped.
-__global__ __launch_bounds__(256) void k_update(int64_t ne, double* __restrict__ x, double* __restrict__ s,
-                                                const double* __restrict__ z, PcgScalars* sc, int iter) {
+// The iterate x is FP64 in both precisions; s and z have the storage type.
+template <typename S>
+__global__ __launch_bounds__(256) void k_update(int64_t ne, double* __restrict__ x, S* __restrict__ s,
+                                                const S* __restrict__ z, PcgScalars* sc, int iter) {
   const int st = sc->converged;
   if (st == STOP_ABORT) return;
   if (st == STOP_CONVERGED && sc->conv_iter != iter) return;
-  const double alpha = sc->alpha, beta = sc->beta;
+  const double alpha = sc->alpha;
+  const S beta = (S)sc->beta;
   int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= ne) return;
-  double sv = s[e];
-  x[e] = x[e] + alpha * sv;
+  const S sv = s[e];
+  x[e] = x[e] + alpha * (double)sv;
   if (st == STOP_RUN) s[e] = z[e] + beta * sv;
 }
 
-__global__ void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
+template <typename S>
+__global__ void k_copy(int64_t n, const S* __restrict__ a, S* __restrict__ b) {
   int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e < n) b[e] = a[e];
 }
 
 // layout -> dense for active cells only (inactive dense entries untouched)
-__global__ void k_unpack(Lay L, const double* __restrict__ v, const uint8_t* __restrict__ nbm, double* __restrict__ dense) {
+template <typename TO>
+__global__ void k_unpack(Lay L, const double* __restrict__ v, const uint8_t* __restrict__ nbm, TO* __restrict__ dense) {
   int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= L.ne) return;
   if (!(nbm[e] & NB_SELF)) return;
   int B, s, a, b, i, j, k;
   decode(L, e, B, s, a, b, i, j, k);
-  dense[k + L.nz * (j + L.ny * i)] = v[e];
+  dense[k + L.nz * (j + L.ny * i)] = (TO)v[e];
+}
+
+__global__ void k_narrow(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e < n) b[e] = (float)a[e];
 }
 // dense -> layout (zero where the cell is padding or inactive)
 __global__ void k_pack_vec(Lay L, const double* __restrict__ dense, const uint8_t* __restrict__ nbm, double* __restrict__ v) {
@@ -1089,6 +1203,10 @@ int pcg_build(fs_ctx* c) {
   for (DBuf* d : {&P.adiag, &P.al, &P.ad, &P.ab, &P.rhs, &P.e, &P.cl, &P.cd, &P.cb, &P.dr, &P.du, &P.df, &P.x, &P.r,
                   &P.s, &P.t, &P.q, &P.z})
     FS_TRY(realloc_if(c, *d, ne * sizeof(double)));
+  if (c->mixed)
+    for (DBuf* d : {&P.f_adiag, &P.f_al, &P.f_ad, &P.f_ab, &P.f_e, &P.f_cl, &P.f_cd, &P.f_cb, &P.f_dr, &P.f_du,
+                    &P.f_df, &P.f_r, &P.f_s, &P.f_t, &P.f_q, &P.f_z})
+      FS_TRY(realloc_if(c, *d, ne * sizeof(float)));
   FS_TRY(realloc_if(c, P.nbm, ne));
   FS_TRY(realloc_if(c, P.tflags, ne * sizeof(uint16_t)));
   FS_TRY(realloc_if(c, P.tnum, 6 * ne * sizeof(double)));
@@ -1097,12 +1215,25 @@ int pcg_build(fs_ctx* c) {
   FS_TRY(realloc_if(c, P.partials2, np_ * sizeof(double)));
   if (P.nlive > 0) {
     Lay L = lay_of(c);
-    // Aneighbour order Left, Right, Up, Down, Front, Back
-    k_pack<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(
-        L, c->cfg.mic_tau, c->adiag.as<double>(), c->an[0].as<double>(), c->an[3].as<double>(),
-        c->an[5].as<double>(), c->rhs.as<double>(), c->active.as<uint8_t>(), P.adiag.as<double>(),
-        P.al.as<double>(), P.ad.as<double>(), P.ab.as<double>(), P.rhs.as<double>(), P.nbm.as<uint8_t>(),
-        P.tflags.as<uint16_t>(), P.tnum.as<double>(), P.x.as<double>(), P.q.as<double>(), P.z.as<double>());
+    // Aneighbour order Left, Right, Up, Down, Front, Back. The packed system is FP64 (exact
+    // for an FP32 assembly); the mixed-precision solve also gets FP32 copies of the stencil.
+    auto pack = [&](auto* tag) {
+      using TI = std::remove_pointer_t<decltype(tag)>;
+      k_pack<TI><<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(
+          L, c->cfg.mic_tau, c->adiag.as<TI>(), c->an[0].as<TI>(), c->an[3].as<TI>(), c->an[5].as<TI>(),
+          c->rhs.as<TI>(), c->active.as<uint8_t>(), P.adiag.as<double>(), P.al.as<double>(), P.ad.as<double>(),
+          P.ab.as<double>(), P.rhs.as<double>(), P.nbm.as<uint8_t>(), P.tflags.as<uint16_t>(), P.tnum.as<double>(),
+          P.x.as<double>(), P.q.as<double>(), P.z.as<double>());
+    };
+    if (c->mixed) {
+      pack((float*)nullptr);
+      const DBuf* src[4] = {&P.adiag, &P.al, &P.ad, &P.ab};
+      DBuf* dst[4] = {&P.f_adiag, &P.f_al, &P.f_ad, &P.f_ab};
+      for (int q = 0; q < 4; q++)
+        k_narrow<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(P.ne, src[q]->as<double>(), dst[q]->as<float>());
+    } else {
+      pack((double*)nullptr);
+    }
     FS_HIP(c, hipGetLastError());
   }
   P.built = true;
@@ -1112,6 +1243,31 @@ int pcg_build(fs_ctx* c) {
 // tickets: one counter per sweep kind.
 enum { SW_FACTOR = 0, SW_FWD = 1, SW_BWD = 2 };
 
+// Typed views of the packed system and PCG vectors: FP64 arrays, or their FP32 copies in the
+// mixed-precision solve. The iterate x is FP64 in both.
+template <typename S>
+struct Sys {
+  S *adiag, *al, *ad, *ab, *e, *cl, *cd, *cb, *dr, *du, *df, *r, *s, *t, *q, *z;
+};
+template <typename S>
+Sys<S> sys_of(PcgState& P);
+template <>
+Sys<double> sys_of<double>(PcgState& P) {
+  return {P.adiag.as<double>(), P.al.as<double>(), P.ad.as<double>(), P.ab.as<double>(), P.e.as<double>(),
+          P.cl.as<double>(),    P.cd.as<double>(), P.cb.as<double>(), P.dr.as<double>(), P.du.as<double>(),
+          P.df.as<double>(),    P.r.as<double>(),  P.s.as<double>(),  P.t.as<double>(),  P.q.as<double>(),
+          P.z.as<double>()};
+}
+template <>
+Sys<float> sys_of<float>(PcgState& P) {
+  return {P.f_adiag.as<float>(), P.f_al.as<float>(), P.f_ad.as<float>(), P.f_ab.as<float>(), P.f_e.as<float>(),
+          P.f_cl.as<float>(),    P.f_cd.as<float>(), P.f_cb.as<float>(), P.f_dr.as<float>(), P.f_du.as<float>(),
+          P.f_df.as<float>(),    P.f_r.as<float>(),  P.f_s.as<float>(),  P.f_t.as<float>(),  P.f_q.as<float>(),
+          P.f_z.as<float>()};
+}
+
+// MIC factor (FP64) and the sweep coefficients in the storage type S.
+template <typename S>
 int pcg_factor(fs_ctx* c) {
   PcgState& P = c->pcg;
   Lay L = lay_of(c);
@@ -1125,48 +1281,52 @@ int pcg_factor(fs_ctx* c) {
   fs_prof_end(c, pf);
   P.ticket_base[SW_FACTOR] += P.nlive;
   FS_HIP(c, hipGetLastError());
-  k_coef<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.al.as<double>(), P.ad.as<double>(), P.ab.as<double>(),
-                                                       P.nbm.as<uint8_t>(), P.e.as<double>(), P.cl.as<double>(),
-                                                       P.cd.as<double>(), P.cb.as<double>(), P.dr.as<double>(),
-                                                       P.du.as<double>(), P.df.as<double>());
+  Sys<S> y = sys_of<S>(P);
+  const bool narrow = !std::is_same<S, double>::value;
+  k_coef<S><<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.al.as<double>(), P.ad.as<double>(), P.ab.as<double>(),
+                                                          P.nbm.as<uint8_t>(), P.e.as<double>(), y.cl, y.cd, y.cb,
+                                                          y.dr, y.du, y.df, narrow ? y.e : nullptr);
   FS_HIP(c, hipGetLastError());
   return FS_OK;
 }
 
 // fwd sweep: UPDATE applies r -= alpha t first.
+template <typename S>
 int launch_fwd(fs_ctx* c, bool update, int iter) {
   PcgState& P = c->pcg;
   Lay L = lay_of(c);
+  Sys<S> y = sys_of<S>(P);
   uint64_t* tk = P.tickets.as<uint64_t>();
   auto go = [&](auto kern) {
-    kern<<<P.nlive, 64, 0, c->stream>>>(L, P.r.as<double>(), P.t.as<double>(), P.cl.as<double>(), P.cd.as<double>(),
-                                        P.cb.as<double>(), P.e.as<double>(), P.q.as<double>(), P.xf.as<double>(),
+    kern<<<P.nlive, 64, 0, c->stream>>>(L, y.r, y.t, y.cl, y.cd, y.cb, y.e, y.q, P.xf.as<double>(),
                                         P.partials2.as<double>(), P.partials.as<double>(), tk + SW_FWD,
                                         P.ticket_base[SW_FWD], P.fwd_order.as<int>(), P.scal.as<PcgScalars>(), iter,
                                         c->cfg.tol_mode, c->cfg.cg_tolerance, P.sweep_lag);
   };
-  const int pf = fs_prof_begin(c, FS_KERNEL_MIC_FWD, (update ? 64.0 : 48.0) * c->n_active_last);
-  if (update) go(k_fwd<true>);
-  else go(k_fwd<false>);
+  const double bytes = (update ? 8.0 : 6.0) * sizeof(S) * c->n_active_last;
+  const int pf = fs_prof_begin(c, FS_KERNEL_MIC_FWD, bytes);
+  if (update) go(k_fwd<true, S>);
+  else go(k_fwd<false, S>);
   fs_prof_end(c, pf);
   P.ticket_base[SW_FWD] += P.nlive;
   FS_HIP(c, hipGetLastError());
   return FS_OK;
 }
 
+template <typename S>
 int launch_bwd(fs_ctx* c, bool init) {
   PcgState& P = c->pcg;
   Lay L = lay_of(c);
+  Sys<S> y = sys_of<S>(P);
   uint64_t* tk = P.tickets.as<uint64_t>();
   auto go = [&](auto kern) {
-    kern<<<P.nlive, 64, 0, c->stream>>>(L, P.q.as<double>(), P.dr.as<double>(), P.du.as<double>(), P.df.as<double>(),
-                                        P.e.as<double>(), P.r.as<double>(), P.z.as<double>(), P.xb.as<double>(),
+    kern<<<P.nlive, 64, 0, c->stream>>>(L, y.q, y.dr, y.du, y.df, y.e, y.r, y.z, P.xb.as<double>(),
                                         P.partials.as<double>(), tk + SW_BWD, P.ticket_base[SW_BWD],
                                         P.bwd_order.as<int>(), P.scal.as<PcgScalars>(), P.sweep_lag);
   };
-  const int pf = fs_prof_begin(c, FS_KERNEL_MIC_BWD, 56.0 * c->n_active_last);
-  if (init) go(k_bwd<true>);
-  else go(k_bwd<false>);
+  const int pf = fs_prof_begin(c, FS_KERNEL_MIC_BWD, 7.0 * sizeof(S) * c->n_active_last);
+  if (init) go(k_bwd<true, S>);
+  else go(k_bwd<false, S>);
   fs_prof_end(c, pf);
   P.ticket_base[SW_BWD] += P.nlive;
   FS_HIP(c, hipGetLastError());
@@ -1180,58 +1340,93 @@ int read_scalars(fs_ctx* c) {
   return FS_OK;
 }
 
-}  // namespace
-
-// CgSolver3D::solve on the assembled system; writes pg (dense, active cells; others 0).
-int fs_stage_solve(fs_ctx* c, fs_stats* st) {
-  if (c->mixed) {
-    c->set_error("mixed-precision pressure solve not available in this build");
-    return FS_UNSUPPORTED;
-  }
+// CgSolver3D::solve (project-solver.cc:308-346) in storage type S.
+//   S = double: the reference's PCG in FP64, fixed-tree dot products.
+//   S = float:  vectors, stencil and sweeps in FP32 with FP64 dot products and norms; the
+//               iterate x is accumulated in FP64, and the FP32 recurrence residual is replaced
+//               by the FP64 residual b - A x whenever it has dropped tenfold since the last
+//               replacement and whenever it claims convergence, so the stopping test is always
+//               made on the FP64 residual ("reliable updates"; Clark et al., Comput. Phys.
+//               Commun. 181 (2010) 1517). A false convergence restarts the search direction
+//               from the replaced residual.
+template <typename S>
+int solve_t(fs_ctx* c, fs_stats* st) {
+  constexpr bool MIXED = !std::is_same<S, double>::value;
   FS_TRY(pcg_build(c));
   PcgState& P = c->pcg;
-  FS_HIP(c, hipMemsetAsync(c->pg.p, 0, (size_t)c->n_cell * sizeof(double), c->stream));
-  int iters = 0;
+  FS_HIP(c, hipMemsetAsync(c->pg.p, 0, (size_t)c->n_cell * c->rsz, c->stream));
+  int iters = 0, refinements = 0;
   int status = FS_OK;
   PcgScalars* hs = P.host_scal;
   if (P.nlive == 0) {
     std::memset(hs, 0, sizeof(*hs));
   } else {
     Lay L = lay_of(c);
+    Sys<S> y = sys_of<S>(P);
     const double tol = c->cfg.cg_tolerance;
-    k_init<<<P.nchunks, 256, 0, c->stream>>>(L, P.rhs.as<double>(), P.r.as<double>(), P.partials.as<double>(),
-                                             P.partials2.as<double>(), P.nchunks, P.scal.as<PcgScalars>(),
-                                             c->cfg.tol_mode, tol);
+    const int tmode = c->cfg.tol_mode;
+    k_init<S><<<P.nchunks, 256, 0, c->stream>>>(L, P.rhs.as<double>(), y.r, P.partials.as<double>(),
+                                                P.partials2.as<double>(), P.nchunks, P.scal.as<PcgScalars>(), tmode,
+                                                tol);
     FS_HIP(c, hipGetLastError());
     FS_TRY(read_scalars(c));
     if (hs->converged == 0) {
-      FS_TRY(pcg_factor(c));
-      FS_TRY(launch_fwd(c, false, 0));
-      FS_TRY(launch_bwd(c, true));
-      k_copy<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(P.ne, P.z.as<double>(), P.s.as<double>());
+      auto restart = [&]() -> int {  // z = M^-1 r, sigma = z.r, s = z
+        FS_TRY(launch_fwd<S>(c, false, 0));
+        FS_TRY(launch_bwd<S>(c, true));
+        k_copy<S><<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(P.ne, y.z, y.s);
+        return FS_OK;
+      };
+      FS_TRY(pcg_factor<S>(c));
+      FS_TRY(restart());
       const int maxit = c->cfg.cg_max_iterations;
       const int chk = std::max(1, c->cfg.check_interval);
+      double last_norm = std::sqrt(hs->rnorm2sq);
       int it = 1;
       bool stopped = false;
       while (it < maxit && !stopped) {
         for (int m = 0; m < chk && it < maxit; m++, it++) {
-          int pf = fs_prof_begin(c, FS_KERNEL_SPMV, 48.0 * c->n_active_last);
-          k_spmv<true><<<P.nchunks, 256, 0, c->stream>>>(L, P.s.as<double>(), P.adiag.as<double>(), P.al.as<double>(),
-                                                         P.ad.as<double>(), P.ab.as<double>(), P.nbm.as<uint8_t>(),
-                                                         P.t.as<double>(), P.partials.as<double>(), P.nchunks,
-                                                         P.scal.as<PcgScalars>(), it);
+          int pf = fs_prof_begin(c, FS_KERNEL_SPMV, 6.0 * sizeof(S) * c->n_active_last);
+          k_spmv<true, S><<<P.nchunks, 256, 0, c->stream>>>(L, y.s, y.adiag, y.al, y.ad, y.ab, P.nbm.as<uint8_t>(),
+                                                            y.t, P.partials.as<double>(), P.nchunks,
+                                                            P.scal.as<PcgScalars>(), it);
           fs_prof_end(c, pf);
-          FS_TRY(launch_fwd(c, true, it));
-          FS_TRY(launch_bwd(c, false));
-          pf = fs_prof_begin(c, FS_KERNEL_PCG_UPDATE, 40.0 * c->n_active_last);
-          k_update<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(P.ne, P.x.as<double>(), P.s.as<double>(),
-                                                                P.z.as<double>(), P.scal.as<PcgScalars>(), it);
+          FS_TRY(launch_fwd<S>(c, true, it));
+          FS_TRY(launch_bwd<S>(c, false));
+          pf = fs_prof_begin(c, FS_KERNEL_PCG_UPDATE, (16.0 + 3.0 * sizeof(S)) * c->n_active_last);
+          k_update<S><<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(P.ne, P.x.as<double>(), y.s, y.z,
+                                                                   P.scal.as<PcgScalars>(), it);
           fs_prof_end(c, pf);
         }
         FS_HIP(c, hipGetLastError());
         FS_TRY(read_scalars(c));
         fs_prof_flush(c);
-        if (hs->converged != 0 || hs->wait_timeouts != 0) stopped = true;
+        if (hs->wait_timeouts != 0) break;
+        if (MIXED && hs->converged != STOP_ABORT) {
+          const bool claimed = hs->converged == STOP_CONVERGED;
+          if (claimed || std::sqrt(hs->rnorm2sq) < 0.1 * last_norm) {
+            const int at = claimed ? hs->conv_iter : it - 1;
+            const int pf = fs_prof_begin(c, FS_KERNEL_PCG_UPDATE, (8.0 * 7 + 4.0) * c->n_active_last);
+            k_resid<<<P.nchunks, 256, 0, c->stream>>>(L, P.rhs.as<double>(), P.x.as<double>(), P.adiag.as<double>(),
+                                                      P.al.as<double>(), P.ad.as<double>(), P.ab.as<double>(),
+                                                      P.nbm.as<uint8_t>(), P.f_r.as<float>(),
+                                                      P.partials.as<double>(), P.partials2.as<double>(), P.nchunks,
+                                                      P.scal.as<PcgScalars>(), at, tmode, tol);
+            fs_prof_end(c, pf);
+            FS_HIP(c, hipGetLastError());
+            FS_TRY(read_scalars(c));
+            refinements++;
+            last_norm = std::sqrt(hs->rnorm2sq);
+            if (hs->converged == STOP_CONVERGED) {
+              stopped = true;
+            } else if (claimed) {
+              it = at + 1;
+              FS_TRY(restart());
+            }
+            continue;
+          }
+        }
+        if (hs->converged != 0) stopped = true;
       }
       if (hs->wait_timeouts != 0) {
         c->set_error("PCG sweep: a brick dependency wait exceeded its bound");
@@ -1242,7 +1437,12 @@ int fs_stage_solve(fs_ctx* c, fs_stats* st) {
       if (hs->converged == 2) status = hs->status;
       else if (hs->converged == 0) status = FS_NOT_CONVERGED;
     }
-    k_unpack<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.x.as<double>(), P.nbm.as<uint8_t>(), c->pg.as<double>());
+    if (MIXED)
+      k_unpack<float><<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.x.as<double>(), P.nbm.as<uint8_t>(),
+                                                                   c->pg.as<float>());
+    else
+      k_unpack<double><<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.x.as<double>(), P.nbm.as<uint8_t>(),
+                                                                    c->pg.as<double>());
     FS_HIP(c, hipGetLastError());
     FS_HIP(c, hipStreamSynchronize(c->stream));
   }
@@ -1252,15 +1452,18 @@ int fs_stage_solve(fs_ctx* c, fs_stats* st) {
     st->rhs_norm2 = std::sqrt(hs->bnorm2sq);
     st->residual_rel = hs->bnorm2sq > 0 ? std::sqrt(hs->rnorm2sq / hs->bnorm2sq) : 0.0;
     st->n_active = (int32_t)c->n_active_last;
-    st->refinements = 0;
+    st->refinements = refinements;
   }
   return status;
 }
 
+}  // namespace
+
+int fs_stage_solve(fs_ctx* c, fs_stats* st) { return c->mixed ? solve_t<float>(c, st) : solve_t<double>(c, st); }
+
 // applyCompressedMatrix on the assembled system for host vectors (dense); inactive entries of
 // y are left as given.
 int fs_pcg_spmv_dense(fs_ctx* c, const double* x, double* y) {
-  if (c->mixed) return FS_UNSUPPORTED;
   FS_TRY(pcg_build(c));
   PcgState& P = c->pcg;
   if (P.nlive == 0) return FS_OK;
@@ -1269,12 +1472,12 @@ int fs_pcg_spmv_dense(fs_ctx* c, const double* x, double* y) {
   FS_TRY(fs_alloc(c, dx, (size_t)c->n_cell * sizeof(double)));
   FS_HIP(c, hipMemcpyAsync(dx.p, x, (size_t)c->n_cell * sizeof(double), hipMemcpyHostToDevice, c->stream));
   k_pack_vec<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, dx.as<double>(), P.nbm.as<uint8_t>(), P.s.as<double>());
-  k_spmv<false><<<P.nchunks, 256, 0, c->stream>>>(L, P.s.as<double>(), P.adiag.as<double>(), P.al.as<double>(),
+  k_spmv<false, double><<<P.nchunks, 256, 0, c->stream>>>(L, P.s.as<double>(), P.adiag.as<double>(), P.al.as<double>(),
                                                   P.ad.as<double>(), P.ab.as<double>(), P.nbm.as<uint8_t>(),
                                                   P.t.as<double>(), P.partials.as<double>(), P.nchunks,
                                                   P.scal.as<PcgScalars>(), 0);
   FS_HIP(c, hipMemcpyAsync(dx.p, y, (size_t)c->n_cell * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  k_unpack<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.t.as<double>(), P.nbm.as<uint8_t>(), dx.as<double>());
+  k_unpack<double><<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.t.as<double>(), P.nbm.as<uint8_t>(), dx.as<double>());
   FS_HIP(c, hipGetLastError());
   FS_HIP(c, hipMemcpyAsync(y, dx.p, (size_t)c->n_cell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   FS_HIP(c, hipStreamSynchronize(c->stream));
@@ -1284,7 +1487,6 @@ int fs_pcg_spmv_dense(fs_ctx* c, const double* x, double* y) {
 
 // ModifiedIncompleteCholesky3D::precond for a host vector r: z = M^-1 r on active cells.
 int fs_pcg_mic_dense(fs_ctx* c, const double* r, double* z) {
-  if (c->mixed) return FS_UNSUPPORTED;
   FS_TRY(pcg_build(c));
   PcgState& P = c->pcg;
   if (P.nlive == 0) return FS_OK;
@@ -1293,11 +1495,11 @@ int fs_pcg_mic_dense(fs_ctx* c, const double* r, double* z) {
   FS_TRY(fs_alloc(c, dx, (size_t)c->n_cell * sizeof(double)));
   FS_HIP(c, hipMemcpyAsync(dx.p, r, (size_t)c->n_cell * sizeof(double), hipMemcpyHostToDevice, c->stream));
   k_pack_vec<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, dx.as<double>(), P.nbm.as<uint8_t>(), P.r.as<double>());
-  FS_TRY(pcg_factor(c));
-  FS_TRY(launch_fwd(c, false, 0));
-  FS_TRY(launch_bwd(c, true));
+  FS_TRY(pcg_factor<double>(c));
+  FS_TRY(launch_fwd<double>(c, false, 0));
+  FS_TRY(launch_bwd<double>(c, true));
   FS_HIP(c, hipMemcpyAsync(dx.p, z, (size_t)c->n_cell * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  k_unpack<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.z.as<double>(), P.nbm.as<uint8_t>(), dx.as<double>());
+  k_unpack<double><<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.z.as<double>(), P.nbm.as<uint8_t>(), dx.as<double>());
   FS_HIP(c, hipGetLastError());
   FS_HIP(c, hipMemcpyAsync(z, dx.p, (size_t)c->n_cell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   FS_HIP(c, hipStreamSynchronize(c->stream));
@@ -1313,17 +1515,16 @@ int fs_pcg_mic_dense(fs_ctx* c, const double* r, double* z) {
 
 // MIC factor E of the assembled system (dense; zero on inactive cells).
 int fs_pcg_export_e(fs_ctx* c, double* out) {
-  if (c->mixed) return FS_UNSUPPORTED;
   FS_TRY(pcg_build(c));
   PcgState& P = c->pcg;
   std::memset(out, 0, (size_t)c->n_cell * sizeof(double));
   if (P.nlive == 0) return FS_OK;
   Lay L = lay_of(c);
-  FS_TRY(pcg_factor(c));
+  FS_TRY(pcg_factor<double>(c));
   DBuf dx;
   FS_TRY(fs_alloc(c, dx, (size_t)c->n_cell * sizeof(double)));
   FS_HIP(c, hipMemsetAsync(dx.p, 0, (size_t)c->n_cell * sizeof(double), c->stream));
-  k_unpack<<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.e.as<double>(), P.nbm.as<uint8_t>(), dx.as<double>());
+  k_unpack<double><<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(L, P.e.as<double>(), P.nbm.as<uint8_t>(), dx.as<double>());
   FS_HIP(c, hipGetLastError());
   FS_HIP(c, hipMemcpyAsync(out, dx.p, (size_t)c->n_cell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   FS_HIP(c, hipStreamSynchronize(c->stream));
@@ -1335,7 +1536,9 @@ void fs_pcg_release(fs_ctx* c) {
   PcgState& P = c->pcg;
   for (DBuf* d : {&P.live_map, &P.brick_ij, &P.fwd_order, &P.bwd_order, &P.brick_flag, &P.adiag, &P.al, &P.ad, &P.ab,
                   &P.nbm, &P.rhs, &P.e, &P.cl, &P.cd, &P.cb, &P.dr, &P.du, &P.df, &P.tflags, &P.tnum, &P.x, &P.r,
-                  &P.s, &P.t, &P.q, &P.z, &P.partials, &P.partials2, &P.scal, &P.xf, &P.xb, &P.tickets})
+                  &P.s, &P.t, &P.q, &P.z, &P.partials, &P.partials2, &P.scal, &P.xf, &P.xb, &P.tickets, &P.f_adiag,
+                  &P.f_al, &P.f_ad, &P.f_ab, &P.f_e, &P.f_cl, &P.f_cd, &P.f_cb, &P.f_dr, &P.f_du, &P.f_df, &P.f_r,
+                  &P.f_s, &P.f_t, &P.f_q, &P.f_z, &P.f_x})
     fs_free(*d);
   if (P.host_scal) (void)hipHostFree(P.host_scal);
   P.host_scal = nullptr;
