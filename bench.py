@@ -98,16 +98,61 @@ def cpu_baseline(simcraft_amd, scenes, wl: dict, tol_mode: int, gpu_iters: float
     }
 
 
+def adjoint_ratio(simcraft_amd, scenes, n: int, steps: int, every: int, precision: str, x_min: float) -> dict:
+    """Forward rollout vs its reverse pass: dam break at n^3, `steps` substeps of dt 1e-3,
+    loss = kinetic energy of the particles with x > x_min at the end; gradients with respect to
+    the initial particle velocities and the gravity vector. The reverse time includes the
+    recomputation of every checkpoint segment. (After 40 substeps of 1e-3 s the front of the
+    x < 0.4 block has not reached x = 0.7, so a region starting there would hold no particle
+    and give a zero gradient and a trivial adjoint solve; the default region starts at 0.3.)"""
+    prec = simcraft_amd.PRECISION_MIXED if precision == "mixed" else simcraft_amd.PRECISION_FP64
+    cfg = simcraft_amd.default_config(nx=n, ny=n, nz=n, flip_alpha=0.95, precision=prec)
+    sim = simcraft_amd.FluidSim(cfg)
+    sim.set_collider_sdf(scenes.no_collider(n))
+    pos, vel = scenes.dam_break(n)
+    sim.set_particles(pos, vel)
+    dt = 1e-3
+    sim.substep(dt)  # warm-up outside the rollout
+    p_start, v_start = sim.particles()
+    sim.set_particles(p_start, v_start)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.rollout_begin(every)
+    for _ in range(steps):
+        sim.substep(dt)
+    sim.synchronize()
+    t_fwd = time.perf_counter() - t0
+    p, v = sim.particles()
+    mask = (p[:, 0] > x_min)[:, None]
+    dvel = np.where(mask, v, 0.0)  # d(0.5 |v|^2)/dv on the region
+    t0 = time.perf_counter()
+    _, dv0, dg = sim.rollout_vjp(None, dvel)
+    sim.synchronize()
+    t_bwd = time.perf_counter() - t0
+    sim.close()
+    return {"workload": f"dam_break_{n}_{precision}_rollout", "steps": steps, "checkpoint_every": every,
+            "loss": f"kinetic energy of particles with x > {x_min}", "particles_in_region": int(mask.sum()),
+            "particles": int(len(pos)), "forward_s": round(t_fwd, 3), "reverse_s": round(t_bwd, 3),
+            "ratio": round(t_bwd / t_fwd, 3), "dL_dgravity": [float(x) for x in dg],
+            "dL_dv0_norm": float(np.linalg.norm(dv0))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="dam_break_sphere_256_fp64")
+    ap.add_argument("--workload", default="dam_break_sphere_256_mixed")
     ap.add_argument("--tol-mode", choices=["abs", "rel"], default="abs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=5)
     ap.add_argument("--cpu-sample-n", type=int, default=128)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-n", type=int, default=128)
+    ap.add_argument("--adjoint-steps", type=int, default=40)
+    ap.add_argument("--adjoint-every", type=int, default=8)
+    ap.add_argument("--adjoint-precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--adjoint-x-min", type=float, default=0.3)
     ap.add_argument("--no-kernel-timing", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -165,7 +210,7 @@ def main() -> int:
                         "us_per_launch": kd["us_per_launch"], "bytes_per_launch": kd["bytes_per_launch"]}
             log("kernels: " + json.dumps(kernels))
     out = {
-        "metric": "FLIP steps/sec at 256^3 (MIC-PCG to 1e-6)",
+        "metric": "FLIP steps/sec at 256^3 (MIC-PCG to 1e-6) + adjoint/forward time ratio",
         "value": sps,
         "unit": "steps/s",
         "n_gpus": 1,
@@ -192,8 +237,14 @@ def main() -> int:
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": None,
+        "adjoint": None,
     }
     sim.close()
+    if not args.no_adjoint:
+        log("adjoint rollout ...")
+        out["adjoint"] = adjoint_ratio(simcraft_amd, scenes, args.adjoint_n, args.adjoint_steps, args.adjoint_every,
+                                       args.adjoint_precision, args.adjoint_x_min)
+        log(f"adjoint: {out['adjoint']}")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         t0 = time.perf_counter()

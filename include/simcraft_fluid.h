@@ -54,8 +54,9 @@ enum { FS_RECON_NAIVE = 0 };
 /* Arithmetic of the whole step. */
 enum {
   FS_PRECISION_FP64 = 0,  /* reference arithmetic: FP64 everywhere */
-  FS_PRECISION_MIXED = 1  /* FP32 particles/grids/stencil/PCG vectors; FP64 dot products,
-                             norms and an outer FP64 iterative refinement of the pressure */
+  FS_PRECISION_MIXED = 1  /* FP32 particles/grids/stencil/PCG vectors and sweeps; FP64 dot
+                             products, norms and iterate, and FP64 residual replacement
+                             (b - Ax) deciding convergence */
 };
 
 /* PCG stopping rule. */
@@ -85,9 +86,8 @@ typedef struct fs_config {
   double recon_radius_factor;     /* fluid-simulator.cc:147 (0.8 * h) */
   int32_t advect_on_cleared_grid; /* 1 = reference order: clear() before advect (fluid-simulator.cc:141-143) */
   double cfl_factor;              /* fluid-simulator.cc:202 (5.0) */
-  /* mixed precision only */
-  double inner_tolerance;         /* relative ||r||_2 reduction of each FP32 inner solve */
-  int32_t max_refinements;        /* outer FP64 refinement passes */
+  double inner_tolerance;         /* unused (kept for layout) */
+  int32_t max_refinements;        /* unused (kept for layout) */
   int32_t check_interval;         /* PCG iterations between host convergence reads */
   int32_t profile_stages;         /* 1 = record per-stage device times into fs_stats */
   int32_t reserved[7];
@@ -95,9 +95,8 @@ typedef struct fs_config {
 
 typedef struct fs_stats {
   int32_t status;         /* fs_status of the step */
-  int32_t cg_iterations;  /* PCG iterations, counted as CgSolver3D::solve counts them (project-solver.cc:326-344);
-                             mixed precision: total inner iterations */
-  int32_t refinements;    /* outer FP64 passes (mixed precision), else 0 */
+  int32_t cg_iterations;  /* PCG iterations, counted as CgSolver3D::solve counts them (project-solver.cc:326-344) */
+  int32_t refinements;    /* FP64 residual replacements (mixed precision), else 0 */
   int32_t n_active;       /* active cells in the last pressure solve */
   double residual_inf;    /* final ||r||_inf of the pressure system (FP64) */
   double residual_rel;    /* final ||r||_2 / ||rhs||_2 (FP64) */
@@ -216,10 +215,13 @@ int fs_rollout_steps(const fs_ctx* ctx);                     /* substeps recorde
  * Any output may be NULL. */
 int fs_rollout_vjp(fs_ctx* ctx, const double* dpos, const double* dvel, double* dpos0, double* dvel0,
                    double* dgravity);
-/* PIC transfer of the given face grids (dense, reference layout) to the current particles, and
- * its transpose: grid gradients from particle-velocity gradients. */
-int fs_sample_grid(fs_ctx* ctx, const double* u, const double* v, const double* w, double* vel_xyz);
-int fs_sample_grid_vjp(fs_ctx* ctx, const double* dvel_xyz, double* du, double* dv, double* dw);
+/* PIC transfer of the given face grids (dense, reference layout) to particles at pos_xyz (NULL:
+ * the simulation's current particles; the simulation state is not changed), and its transpose:
+ * grid gradients from particle-velocity gradients. */
+int fs_sample_grid(fs_ctx* ctx, const double* pos_xyz, const double* u, const double* v, const double* w,
+                   double* vel_xyz);
+int fs_sample_grid_vjp(fs_ctx* ctx, const double* pos_xyz, const double* dvel_xyz, double* du, double* dv,
+                       double* dw);
 
 #ifdef __cplusplus
 }

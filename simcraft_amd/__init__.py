@@ -258,19 +258,28 @@ class FluidSim:
                                              _dptr(dg)))
         return dp0, dv0, dg
 
-    def sample_grid(self, u: np.ndarray, v: np.ndarray, w: np.ndarray) -> np.ndarray:
-        """PIC transfer of the face grids u, v, w to the current particles (velocities, (N, 3))."""
+    def _pos_arg(self, pos):
+        if pos is None:
+            return None, None
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.num_particles(), 3)
+        return p, _dptr(p)
+
+    def sample_grid(self, u: np.ndarray, v: np.ndarray, w: np.ndarray, pos: np.ndarray | None = None) -> np.ndarray:
+        """PIC transfer of the face grids u, v, w to particles at `pos` (default: the current
+        particles) as velocities (N, 3); the simulation state is not changed."""
         arrs = [np.ascontiguousarray(g, dtype=np.float64) for g in (u, v, w)]
         for nm, g in zip(("u", "v", "w"), arrs):
             if g.shape != self.array_info(nm)[0]:
                 raise ValueError(f"{nm} must have shape {self.array_info(nm)[0]}")
+        p, pp = self._pos_arg(pos)
         out = np.empty((self.num_particles(), 3))
-        self._check(self._lib.fs_sample_grid(self._ctx, *[_dptr(g) for g in arrs], _dptr(out)))
+        self._check(self._lib.fs_sample_grid(self._ctx, pp, *[_dptr(g) for g in arrs], _dptr(out)))
         return out
 
-    def sample_grid_vjp(self, dvel: np.ndarray):
+    def sample_grid_vjp(self, dvel: np.ndarray, pos: np.ndarray | None = None):
         """Transpose of sample_grid: face-grid gradients from particle-velocity gradients."""
         d = np.ascontiguousarray(dvel, dtype=np.float64).reshape(self.num_particles(), 3)
+        p, pp = self._pos_arg(pos)
         outs = [np.empty(self.array_info(nm)[0]) for nm in ("u", "v", "w")]
-        self._check(self._lib.fs_sample_grid_vjp(self._ctx, _dptr(d), *[_dptr(o) for o in outs]))
+        self._check(self._lib.fs_sample_grid_vjp(self._ctx, pp, _dptr(d), *[_dptr(o) for o in outs]))
         return tuple(outs)

@@ -104,8 +104,8 @@ SIGNATURES = {
     "fs_rollout_end": (C.c_int, [_P]),
     "fs_rollout_steps": (C.c_int, [_P]),
     "fs_rollout_vjp": (C.c_int, [_P, _D, _D, _D, _D, _D]),
-    "fs_sample_grid": (C.c_int, [_P, _D, _D, _D, _D]),
-    "fs_sample_grid_vjp": (C.c_int, [_P, _D, _D, _D, _D]),
+    "fs_sample_grid": (C.c_int, [_P, _D, _D, _D, _D, _D]),
+    "fs_sample_grid_vjp": (C.c_int, [_P, _D, _D, _D, _D, _D]),
 }
 
 KERNEL_NAMES = ["spmv", "mic_fwd", "mic_bwd", "pcg_update", "mic_factor", "p2g", "g2p", "advect", "reconstruct"]

@@ -44,22 +44,24 @@ class FlipRollout(torch.autograd.Function):
 
 
 class GridToParticles(torch.autograd.Function):
-    """PIC transfer of face grids (u, v, w) to the simulator's current particles."""
+    """PIC transfer of face grids (u, v, w) to particles at `pos` (default: the simulator's
+    current particles, read at forward time)."""
 
     @staticmethod
-    def forward(ctx, u, v, w, sim: FluidSim):
+    def forward(ctx, u, v, w, sim: FluidSim, pos=None):
         ctx.sim = sim
-        return torch.from_numpy(sim.sample_grid(_np(u), _np(v), _np(w)))
+        ctx.pos = sim.positions() if pos is None else _np(pos)
+        return torch.from_numpy(sim.sample_grid(_np(u), _np(v), _np(w), pos=ctx.pos))
 
     @staticmethod
     def backward(ctx, gvel):
-        du, dv, dw = ctx.sim.sample_grid_vjp(_np(gvel))
-        return torch.from_numpy(du), torch.from_numpy(dv), torch.from_numpy(dw), None
+        du, dv, dw = ctx.sim.sample_grid_vjp(_np(gvel), pos=ctx.pos)
+        return torch.from_numpy(du), torch.from_numpy(dv), torch.from_numpy(dw), None, None
 
 
 def flip_rollout(sim: FluidSim, pos0, vel0, gravity, dts, checkpoint_every: int = 8):
     return FlipRollout.apply(pos0, vel0, gravity, sim, list(dts), checkpoint_every)
 
 
-def grid_to_particles(sim: FluidSim, u, v, w):
-    return GridToParticles.apply(u, v, w, sim)
+def grid_to_particles(sim: FluidSim, u, v, w, pos=None):
+    return GridToParticles.apply(u, v, w, sim, pos)

@@ -639,17 +639,23 @@ int rollout_vjp_t(fs_ctx* c, const double* dpos, const double* dvel, double* dpo
 }
 
 template <typename T>
-int sample_vjp_t(fs_ctx* c, const double* dvel, double* dgrid[3]) {
+int sample_vjp_t(fs_ctx* c, const double* pos, const double* dvel, double* dgrid[3]) {
   // v_a(p) = sum_n w g_a(n) / S: the transposed transfer of dvel / S, as in the G2P reverse
   AdjWork w;
   FS_TRY(alloc_work(c, w));
-  int st = upload_particles<T>(c, dvel, w.vb);
+  DBuf* P = c->pos;
+  int st = FS_OK;
+  if (pos) {  // explicit positions: the G2P reverse's scratch holds them
+    st = upload_particles<T>(c, pos, w.vb1);
+    P = w.vb1;
+  }
+  if (st == FS_OK) st = upload_particles<T>(c, dvel, w.vb);
   // with FLIP off the G2P reverse gives coef = dvel / S; the grids only enter the position terms
   const int32_t adv = c->cfg.advection;
   c->cfg.advection = FS_ADVECT_PIC;
-  if (st == FS_OK) st = fs_adj_g2p(c, c->pos, c->g, c->g, w.vb, w.xb, w.vb1, w.coef);
+  if (st == FS_OK) st = fs_adj_g2p(c, P, c->g, c->g, w.vb, w.xb, w.xb, w.coef);
   c->cfg.advection = adv;
-  if (st == FS_OK) st = fs_adj_p2g_sum(c, c->pos, w.coef, w.gb5);
+  if (st == FS_OK) st = fs_adj_p2g_sum(c, P, w.coef, w.gb5);
   for (int a = 0; a < 3 && st == FS_OK; a++) {
     const int64_t nf = c->n_face[a];
     DBuf tmp;
@@ -773,23 +779,45 @@ int fs_rollout_vjp(fs_ctx* c, const double* dpos, const double* dvel, double* dp
                   : rollout_vjp_t<double>(c, dpos, dvel, dpos0, dvel0, dgravity);
 }
 
-int fs_sample_grid(fs_ctx* c, const double* u, const double* v, const double* w, double* vel) {
+int fs_sample_grid(fs_ctx* c, const double* pos, const double* u, const double* v, const double* w, double* vel) {
   if (!c || !u || !v || !w || !vel) return FS_BAD_ARG;
-  // PIC G2P of the given grids at the current particle positions
+  // PIC G2P of the given grids at the given (or the current) positions; the simulation's
+  // particles are left as they were (the face grids are scratch until the next substep)
   const double* src[3] = {u, v, w};
   for (int a = 0; a < 3; a++) FS_TRY(fs_upload_real(c, c->g[a], src[a], (size_t)c->n_face[a]));
-  const int32_t adv = c->cfg.advection;
-  c->cfg.advection = FS_ADVECT_PIC;
-  int st = fs_stage_g2p(c);
-  c->cfg.advection = adv;
-  FS_TRY(st);
-  return fs_particles_download(c, nullptr, vel);
+  DBuf tp[3], tv[3];
+  int st = FS_OK;
+  for (int d = 0; d < 3 && st == FS_OK; d++) {
+    st = fs_alloc(c, tv[d], (size_t)std::max<int64_t>(c->np, 1) * c->rsz);
+    if (st == FS_OK && pos) st = fs_alloc(c, tp[d], (size_t)std::max<int64_t>(c->np, 1) * c->rsz);
+  }
+  if (st == FS_OK && pos) st = c->mixed ? upload_particles<float>(c, pos, tp) : upload_particles<double>(c, pos, tp);
+  if (st == FS_OK) {
+    for (int d = 0; d < 3; d++) {
+      std::swap(c->vel[d], tv[d]);
+      if (pos) std::swap(c->pos[d], tp[d]);
+    }
+    const int32_t adv = c->cfg.advection;
+    c->cfg.advection = FS_ADVECT_PIC;
+    st = fs_stage_g2p(c);
+    c->cfg.advection = adv;
+    if (st == FS_OK) st = fs_particles_download(c, nullptr, vel);
+    for (int d = 0; d < 3; d++) {
+      std::swap(c->vel[d], tv[d]);
+      if (pos) std::swap(c->pos[d], tp[d]);
+    }
+  }
+  for (int d = 0; d < 3; d++) {
+    fs_free(tp[d]);
+    fs_free(tv[d]);
+  }
+  return st;
 }
 
-int fs_sample_grid_vjp(fs_ctx* c, const double* dvel, double* du, double* dv, double* dw) {
+int fs_sample_grid_vjp(fs_ctx* c, const double* pos, const double* dvel, double* du, double* dv, double* dw) {
   if (!c || !dvel || !du || !dv || !dw) return FS_BAD_ARG;
   double* out[3] = {du, dv, dw};
-  return c->mixed ? sample_vjp_t<float>(c, dvel, out) : sample_vjp_t<double>(c, dvel, out);
+  return c->mixed ? sample_vjp_t<float>(c, pos, dvel, out) : sample_vjp_t<double>(c, pos, dvel, out);
 }
 
 }  // extern "C"

@@ -501,6 +501,39 @@ __global__ __launch_bounds__(256) void k_init(Lay L, const double* __restrict__ 
   }
 }
 
+// A block's 4096 elements are 16 groups of 256; a group is 4 slots of one brick (S*64 is a
+// multiple of 512), so the brick, its neighbour bricks and the group's first slot are uniform
+// and every neighbour element is a fixed offset inside the brick or a uniform base plus the
+// lane's index outside it.
+struct Group {
+  int B, s0, BL, BR, BD, BU;
+};
+__device__ inline Group group_of(const Lay& L, int base) {
+  Group g;
+  const int SL = L.S * 64;
+  g.B = base / SL;
+  g.s0 = (base - g.B * SL) >> 6;
+  const int2 ij = L.bij[g.B];
+  g.BL = ij.x > 0 ? L.live[(ij.x - 1) * L.nbj + ij.y] : -1;
+  g.BR = ij.x + 1 < L.nbi ? L.live[(ij.x + 1) * L.nbj + ij.y] : -1;
+  g.BD = ij.y > 0 ? L.live[ij.x * L.nbj + ij.y - 1] : -1;
+  g.BU = ij.y + 1 < L.nbj ? L.live[ij.x * L.nbj + ij.y + 1] : -1;
+  return g;
+}
+// neighbour elements of element e = (B*S + s)*64 + a*8 + b (the neighbour must be active)
+__device__ inline int nb_left(const Lay& L, const Group& g, int e, int s, int a, int b) {
+  return a > 0 ? e - 72 : (g.BL * L.S + s + 7) * 64 + 56 + b;
+}
+__device__ inline int nb_right(const Lay& L, const Group& g, int e, int s, int a, int b) {
+  return a < 7 ? e + 72 : (g.BR * L.S + s - 7) * 64 + b;
+}
+__device__ inline int nb_down(const Lay& L, const Group& g, int e, int s, int a, int b) {
+  return b > 0 ? e - 65 : (g.BD * L.S + s + 7) * 64 + a * 8 + 7;
+}
+__device__ inline int nb_up(const Lay& L, const Group& g, int e, int s, int a, int b) {
+  return b < 7 ? e + 65 : (g.BU * L.S + s - 7) * 64 + a * 8;
+}
+
 // z = A s with the fixed-tree s.z (applyCompressedMatrix + dotProduct(s, z)); last block
 // forms alpha = sigma / s.z.
 // S is the storage and arithmetic type of the vectors and the stencil; the dot product is
@@ -515,30 +548,25 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const S* __restrict__ x, co
   if (PCG && sc->converged != STOP_RUN) return;
   double acc = 0.0;
   for (int m = 0; m < 16; m++) {
-    int64_t e = (int64_t)blockIdx.x * CH + m * 256 + threadIdx.x;
-    if (e >= L.ne) continue;
+    const int base = blockIdx.x * CH + m * 256;
+    if (base >= L.ne) break;  // L.ne is a multiple of 512
+    const Group g = group_of(L, base);
+    const int e = base + threadIdx.x, lane = threadIdx.x & 63, s = g.s0 + (threadIdx.x >> 6);
+    const int a = lane >> 3, b = lane & 7;
     const int f = nbm[e];
     S v = S(0);
     const S xc = x[e];
     if (f & NB_SELF) {
-      int B, s, a, b, i, j, k;
-      decode(L, e, B, s, a, b, i, j, k);
       v = adiag[e] * xc;
-      if (f & NB_L) {
-        int64_t n = nb_elem(L, B, s, a, b, 0);
-        v = v + al[e] * x[n];
-      }
-      if (f & NB_D) {
-        int64_t n = nb_elem(L, B, s, a, b, 2);
-        v = v + ad[e] * x[n];
-      }
+      if (f & NB_L) v = v + al[e] * x[nb_left(L, g, e, s, a, b)];
+      if (f & NB_D) v = v + ad[e] * x[nb_down(L, g, e, s, a, b)];
       if (f & NB_B) v = v + ab[e] * x[e - 64];
       if (f & NB_R) {
-        int64_t n = nb_elem(L, B, s, a, b, 1);
+        const int n = nb_right(L, g, e, s, a, b);
         v = v + al[n] * x[n];
       }
       if (f & NB_U) {
-        int64_t n = nb_elem(L, B, s, a, b, 3);
+        const int n = nb_up(L, g, e, s, a, b);
         v = v + ad[n] * x[n];
       }
       if (f & NB_F) v = v + ab[e + 64] * x[e + 64];
@@ -577,29 +605,24 @@ __global__ __launch_bounds__(256) void k_resid(Lay L, const double* __restrict__
   __shared__ double sm[4];
   double mx = 0.0, ss = 0.0;
   for (int m = 0; m < 16; m++) {
-    int64_t e = (int64_t)blockIdx.x * CH + m * 256 + threadIdx.x;
-    if (e >= L.ne) continue;
+    const int base = blockIdx.x * CH + m * 256;
+    if (base >= L.ne) break;
+    const Group g = group_of(L, base);
+    const int e = base + threadIdx.x, lane = threadIdx.x & 63, s = g.s0 + (threadIdx.x >> 6);
+    const int a = lane >> 3, b = lane & 7;
     const int f = nbm[e];
     double res = 0.0;
     if (f & NB_SELF) {
-      int B, s, a, b, i, j, k;
-      decode(L, e, B, s, a, b, i, j, k);
       double v = adiag[e] * x[e];
-      if (f & NB_L) {
-        int64_t n = nb_elem(L, B, s, a, b, 0);
-        v = v + al[e] * x[n];
-      }
-      if (f & NB_D) {
-        int64_t n = nb_elem(L, B, s, a, b, 2);
-        v = v + ad[e] * x[n];
-      }
+      if (f & NB_L) v = v + al[e] * x[nb_left(L, g, e, s, a, b)];
+      if (f & NB_D) v = v + ad[e] * x[nb_down(L, g, e, s, a, b)];
       if (f & NB_B) v = v + ab[e] * x[e - 64];
       if (f & NB_R) {
-        int64_t n = nb_elem(L, B, s, a, b, 1);
+        const int n = nb_right(L, g, e, s, a, b);
         v = v + al[n] * x[n];
       }
       if (f & NB_U) {
-        int64_t n = nb_elem(L, B, s, a, b, 3);
+        const int n = nb_up(L, g, e, s, a, b);
         v = v + ad[n] * x[n];
       }
       if (f & NB_F) v = v + ab[e + 64] * x[e + 64];

@@ -81,7 +81,8 @@ def test_rollout_vjp_matches_oracle_finite_differences(hip, kind, alpha):
     cx, cv = rng.standard_normal(pos.shape), rng.standard_normal(pos.shape)
     dp0, dv0, dg = _vjp(hip, pos, vel, cx, cv, steps, every=2, alpha=alpha)
     # directional derivatives along random velocity and position perturbations, and gravity
-    for which, grad, eps in (("vel", dv0, 1e-5), ("pos", dp0, 1e-7)):
+    # small steps: the discrete structure (marking, masks, branches) must not change under them
+    for which, grad, eps in (("vel", dv0, 1e-6), ("pos", dp0, 1e-8)):
         d = rng.standard_normal(pos.shape)
         if which == "pos":
             d *= (pos > 2e-6) & (pos < 1 - 2e-6)  # stay inside the domain
@@ -91,12 +92,14 @@ def test_rollout_vjp_matches_oracle_finite_differences(hip, kind, alpha):
                           cx, cv, steps, alpha=alpha)
         fd = (fp - fm) / (2 * eps)
         an = float(np.sum(grad * d))
-        assert an == pytest.approx(fd, rel=2e-3, abs=1e-6 * max(1.0, abs(fd))), (which, an, fd)
-    eps = 1e-4
+        assert an == pytest.approx(fd, rel=1e-4, abs=1e-6 * max(1.0, abs(fd))), (which, an, fd)
+    if kind == "full":
+        return  # in a closed full box the projection cancels gravity: dL/dg is solver-tolerance noise
+    eps = 1e-6
     fd = (_oracle_loss(pos, vel, cx, cv, steps, gravity=9.8 + eps, alpha=alpha) -
           _oracle_loss(pos, vel, cx, cv, steps, gravity=9.8 - eps, alpha=alpha)) / (2 * eps)
     # G = (0, -g, 0): dL/dg = -dL/dG_y
-    assert -dg[1] == pytest.approx(fd, rel=2e-3, abs=1e-8), (dg, fd)
+    assert -dg[1] == pytest.approx(fd, rel=1e-4, abs=1e-8), (dg, fd)
 
 
 @pytest.mark.gpu
@@ -148,8 +151,7 @@ def test_torch_autograd_binding(hip):
     sim2.substep(DT)
     sim2.substep(DT)
     _, dv0, dg = sim2.rollout_vjp(dpos, None)
-    sim2.set_particles(pos, v0.detach().numpy())
-    gu = sim2.sample_grid_vjp(dv0)
+    gu = sim2.sample_grid_vjp(dv0, pos=pos)
     sim2.close()
     sim.close()
     np.testing.assert_allclose(grav.grad.numpy(), dg, rtol=1e-12, atol=1e-15)
