@@ -313,107 +313,209 @@ __global__ void k_p2g_records(int n, int axis, GridArgs<T> ga, const int* __rest
 // Node (x, y, z) sums v*w and w over the particles whose nearest node b has x-b0, y-b1, z-b2
 // in {-1..2}, bins in lexicographic (b0, b1, b2) order and particles by index inside a bin;
 // then ug = sum(v w) / sum(w) where the weight sum is positive, else 0 and not valid.
-// A thread owns a 2x2 (y, z) node column over P2G_LX consecutive x. It walks the bin slabs b0
-// in ascending order and adds every particle to the x-planes it reaches (a ring of four
-// accumulator planes, slot x & 3); plane x is complete after slab x+1. Each node still sees
-// its contributions in (b0, b1, b2, index) order, while a particle's record is read once per
-// thread instead of once per node.
-constexpr int P2G_LX = 16;  // x-nodes per thread; a multiple of 4 keeps the ring slots static
+//
+// A workgroup owns a tile of up to P2G_TZ consecutive z-nodes (one per lane) by up to P2G_TY
+// y-nodes (in each lane's registers) over a segment [xs, xe) of x-planes. It walks the bin
+// slabs b0 = xs-2 .. xe and, inside a slab, the bin rows b1 = y0-2 .. y1 in ascending order.
+// The bins z0-2 .. z1 of a row hold one contiguous run of bin-ordered particle records, which
+// the workgroup copies to LDS with coalesced loads. Wave k owns the x-planes x with
+// (x - xs) % 4 == k: at slab b0 the four planes b0-1 .. b0+2 belong to four different waves,
+// so every wave uses every staged row, and plane b0-1 is complete (its wave writes and clears
+// it) once slab b0 is done. A lane walks the particles of bins z-2 .. z+1 of the row in
+// order, so every node still sees its contributions in (b0, b1, b2, index) order.
+constexpr int P2G_TZ = 64;    // z-nodes per workgroup, one per lane of a wave
+constexpr int P2G_TY = 8;     // y-nodes per lane
+constexpr int P2G_WAVES = 4;  // one per x-plane of the four a particle reaches
 
-template <typename T, bool NORM = true>
-__global__ __launch_bounds__(256) void k_p2g_gather(GridArgs<T> ga, int gx, int gy, int gz, int n,
-                                                    const int* __restrict__ offs, const T* __restrict__ W,
-                                                    const T* __restrict__ vc, T* __restrict__ G,
-                                                    T* __restrict__ Wt, uint8_t* __restrict__ valid) {
-  const int nty = (gy + 1) >> 1, ntz = (gz + 1) >> 1, nseg = (gx + P2G_LX - 1) / P2G_LX;
-  const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= nseg * nty * ntz) return;
-  const int tz = id % ntz, ty = (id / ntz) % nty, seg = id / (ntz * nty);
-  const int y0 = 2 * ty, z0 = 2 * tz, xs = seg * P2G_LX, xe = min(xs + P2G_LX, gx);
-  const int bx = ga.nx + 1, by = ga.ny + 1, bz = ga.nz + 1;
-  const bool y1ok = y0 + 1 < gy, z1ok = z0 + 1 < gz;
-  const int b1lo = max(y0 - 2, 0), b1hi = min(y0 + 2, by - 1);
-  const int b2lo = max(z0 - 2, 0), b2hi = min(z0 + 2, bz - 1);
-  T acc[4][2][2], ws[4][2][2];
+// Particles per staged chunk (13 fields each); a row with more is staged in several chunks.
+template <typename T>
+struct P2GCap {
+  static constexpr int value = sizeof(T) == 8 ? 640 : 768;
+  static constexpr int padded = value + value / 8;
+};
+
+// LDS slot of particle i of a chunk: one pad word after every 8, so that the lanes of a wave,
+// whose particles lie about one bin (about 8 particles) apart, fall into different banks.
+__device__ __forceinline__ int p2g_slot(int i) { return i + (i >> 3); }
+
+// One lane over its particles [t0, t1) of a staged row whose y-offset is C = y0 - b1 + 1:
+// y-node j (y = y0 + j) takes the y-factor W[4 + j + C] and is reached when 0 <= j + C <= 3.
+// The z-factor index is 3 - (bins of the lane's four passed), the x-factor that of the
+// wave's plane.
+template <typename T, bool NORM, int C>
+__device__ __forceinline__ void p2g_row(const T* __restrict__ L, int o, int t0, int t1, int c0, int B1, int B2,
+                                        int B3, T (&acc)[P2G_TY], T (&ws)[P2G_TY]) {
+  constexpr int CP = P2GCap<T>::padded;
+  for (int t = t0; t < t1; t++) {
+    const int s = p2g_slot(t - c0);
+    const int iz = 3 - (int)(t >= B1) - (int)(t >= B2) - (int)(t >= B3);
+    const T wxo = L[o * CP + s];
+    const T wz = L[(8 + iz) * CP + s];
+    const T v = L[12 * CP + s];
 #pragma unroll
-  for (int q = 0; q < 4; q++)
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-      for (int k = 0; k < 2; k++) acc[q][j][k] = ws[q][j][k] = T(0);
-  for (int i0 = 0; i0 < P2G_LX + 3; i0 += 4) {
-#pragma unroll
-    for (int ii = 0; ii < 4; ii++) {
-      const int i = i0 + ii;
-      if (i >= P2G_LX + 3) break;
-      const int b0 = xs - 2 + i;  // planes x = b0-1+o, o = 0..3, ring slot (ii + o + 1) & 3
-      bool live[4];
-#pragma unroll
-      for (int o = 0; o < 4; o++) live[o] = b0 - 1 + o >= xs && b0 - 1 + o < xe;
-      if (b0 >= 0 && b0 < bx) {
-        for (int b1 = b1lo; b1 <= b1hi; b1++) {
-          const int iy0 = y0 - b1 + 1, iy1 = iy0 + 1;  // y factor index of nodes y0, y0+1
-          const bool oky0 = iy0 >= 0 && iy0 <= 3, oky1 = y1ok && iy1 >= 0 && iy1 <= 3;
-          const int row = bz * (b1 + by * b0);
-          for (int b2 = b2lo; b2 <= b2hi; b2++) {
-            const int iz0 = z0 - b2 + 1, iz1 = iz0 + 1;
-            const bool okz0 = iz0 >= 0 && iz0 <= 3, okz1 = z1ok && iz1 >= 0 && iz1 <= 3;
-            const int s = offs[row + b2], e = offs[row + b2 + 1];
-            for (int t = s; t < e; t++) {
-              T wx[4];
-#pragma unroll
-              for (int o = 0; o < 4; o++) wx[o] = W[(int64_t)o * n + t];
-              const T wy[2] = {oky0 ? W[(int64_t)(4 + iy0) * n + t] : T(0), oky1 ? W[(int64_t)(4 + iy1) * n + t] : T(0)};
-              const T wz[2] = {okz0 ? W[(int64_t)(8 + iz0) * n + t] : T(0), okz1 ? W[(int64_t)(8 + iz1) * n + t] : T(0)};
-              const bool oky[2] = {oky0, oky1}, okz[2] = {okz0, okz1};
-              const T v = vc[t];
-#pragma unroll
-              for (int o = 0; o < 4; o++) {
-                const int q = (ii + o + 1) & 3;
-#pragma unroll
-                for (int j = 0; j < 2; j++)
-#pragma unroll
-                  for (int k = 0; k < 2; k++) {
-                    const T w = wx[o] * wy[j] * wz[k];
-                    const bool on = live[o] && oky[j] && okz[k];
-                    acc[q][j][k] = on ? acc[q][j][k] + v * w : acc[q][j][k];
-                    ws[q][j][k] = on ? ws[q][j][k] + w : ws[q][j][k];
-                  }
-              }
-            }
-          }
-        }
-      }
-      // plane x = b0 - 1 has seen every slab x-2 .. x+1
-      const int q = (ii + 1) & 3;
-      if (live[0]) {
-        const int x = b0 - 1;
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-#pragma unroll
-          for (int k = 0; k < 2; k++) {
-            if ((j == 1 && !y1ok) || (k == 1 && !z1ok)) continue;
-            const int64_t nid = ((int64_t)x * gy + (y0 + j)) * gz + (z0 + k);
-            const T a = acc[q][j][k], s = ws[q][j][k];
-            if (NORM) {
-              G[nid] = s > T(0) ? a / s : T(0);
-              valid[nid] = s > T(0) ? 1 : 0;
-              Wt[nid] = s;
-            } else {
-              G[nid] = a;
-            }
-          }
-      }
-#pragma unroll
-      for (int j = 0; j < 2; j++)
-#pragma unroll
-        for (int k = 0; k < 2; k++) acc[q][j][k] = ws[q][j][k] = T(0);
+    for (int j = 0; j < P2G_TY; j++) {
+      if (j + C < 0 || j + C > 3) continue;
+      const T w = wxo * L[(4 + j + C) * CP + s] * wz;
+      acc[j] = acc[j] + v * w;
+      if (NORM) ws[j] = ws[j] + w;
     }
   }
 }
 
-int p2g_gather_threads(int gx, int gy, int gz) {
-  return ((gx + P2G_LX - 1) / P2G_LX) * ((gy + 1) / 2) * ((gz + 1) / 2);
+template <typename T, bool NORM = true>
+__global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga, int gx, int gy, int gz, int n,
+                                                                   int nzt, int nyt, int nseg,
+                                                                   const int* __restrict__ offs,
+                                                                   const T* __restrict__ W, const T* __restrict__ vc,
+                                                                   T* __restrict__ G, T* __restrict__ Wt,
+                                                                   uint8_t* __restrict__ valid) {
+  constexpr int CAP = P2GCap<T>::value, CP = P2GCap<T>::padded;
+  __shared__ T L[13 * CP];
+  // Workgroups go round-robin over the 8 XCDs (id = 8 * k + xcd); each XCD takes a contiguous
+  // run of tiles, y-tiles fastest, so the bin rows that y-neighbouring tiles share are read
+  // through one L2.
+  const int nwg = nzt * nyt * nseg;
+  const int xcd = blockIdx.x & 7, q = nwg >> 3, r = nwg & 7;
+  const int tile = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (int)(blockIdx.x >> 3);
+  const int yt = tile % nyt, zt = (tile / nyt) % nzt, seg = tile / (nyt * nzt);
+  const int y0 = yt * gy / nyt, y1 = (yt + 1) * gy / nyt;
+  const int z0 = zt * gz / nzt, z1 = (zt + 1) * gz / nzt;
+  const int xs = seg * gx / nseg, xe = (seg + 1) * gx / nseg;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int bx = ga.nx + 1, by = ga.ny + 1, bz = ga.nz + 1;
+  const int z = z0 + lane;
+  const bool zok = z < z1;
+  const int zlo = max(z0 - 2, 0), zhi = min(z1, bz - 1);
+  T acc[P2G_TY], ws[P2G_TY];
+#pragma unroll
+  for (int j = 0; j < P2G_TY; j++) acc[j] = ws[j] = T(0);
+  // Rows in walk order: ri = (b0 - xs + 2) * nb1 + (b1 - b1s). The first chunk of row ri + 1
+  // (records and the lane's bin bounds) is loaded into registers while row ri is summed.
+  const int b1s = max(y0 - 2, 0), nb1 = min(y1, by - 1) - b1s + 1, nrow = (xe - xs + 3) * nb1;
+  constexpr int NT = P2G_TZ * P2G_WAVES, PF = (CAP + NT - 1) / NT;
+  T pre[PF][13];
+  int pr0 = 0, pr1 = 0, pB[5];
+  auto fetch = [&](int ri) {
+    const int b0 = xs - 2 + ri / nb1, b1 = b1s + ri % nb1;
+    if (b0 < 0 || b0 >= bx) {
+      pr0 = pr1 = 0;
+#pragma unroll
+      for (int m = 0; m < 5; m++) pB[m] = 0;
+      return;
+    }
+    const int row = bz * (b1 + by * b0);
+    pr0 = offs[row + zlo];
+    pr1 = offs[row + zhi + 1];
+    // the lane's bins z-2 .. z+1 start at pB[0] .. pB[3] and end at pB[4]
+#pragma unroll
+    for (int m = 0; m < 5; m++) pB[m] = zok ? offs[row + min(max(z - 2 + m, 0), bz)] : pr0;
+    const int cnt = min(CAP, pr1 - pr0);
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+      const int i = (int)threadIdx.x + k * NT;
+      if (i < cnt) {
+#pragma unroll
+        for (int f = 0; f < 12; f++) pre[k][f] = W[(int64_t)f * n + pr0 + i];
+        pre[k][12] = vc[pr0 + i];
+      }
+    }
+  };
+  auto sum_chunk = [&](int o, int cy, int c0, int cnt, const int (&B)[5]) {
+    const int t0 = max(B[0], c0), t1 = min(B[4], c0 + cnt);
+    switch (cy) {
+#define FS_P2G_ROW(CC) \
+  case CC:             \
+    p2g_row<T, NORM, CC>(L, o, t0, t1, c0, B[1], B[2], B[3], acc, ws); \
+    break;
+      FS_P2G_ROW(3)
+      FS_P2G_ROW(2)
+      FS_P2G_ROW(1)
+      FS_P2G_ROW(0)
+      FS_P2G_ROW(-1)
+      FS_P2G_ROW(-2)
+      FS_P2G_ROW(-3)
+      FS_P2G_ROW(-4)
+      FS_P2G_ROW(-5)
+      FS_P2G_ROW(-6)
+      FS_P2G_ROW(-7)
+#undef FS_P2G_ROW
+      default:
+        break;
+    }
+  };
+  fetch(0);
+  for (int ri = 0; ri < nrow; ri++) {
+    const int b0 = xs - 2 + ri / nb1, b1 = b1s + ri % nb1;
+    const int o = (wave + 1 + xs - b0) & 3;  // this wave's plane x = b0 - 1 + o, x - xs = wave (mod 4)
+    const int x = b0 - 1 + o;
+    const bool live = x >= xs && x < xe;
+    const int r0 = pr0, r1 = pr1, cy = y0 - b1 + 1;
+    int B[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) B[m] = pB[m];
+    if (r0 != r1) {
+      const int cnt = min(CAP, r1 - r0);
+      __syncthreads();  // the previous row has been read
+#pragma unroll
+      for (int k = 0; k < PF; k++) {
+        const int i = (int)threadIdx.x + k * NT;
+        if (i < cnt) {
+          const int s = p2g_slot(i);
+#pragma unroll
+          for (int f = 0; f < 13; f++) L[f * CP + s] = pre[k][f];
+        }
+      }
+      __syncthreads();
+    }
+    if (ri + 1 < nrow) fetch(ri + 1);
+    if (r0 != r1) {
+      if (live) sum_chunk(o, cy, r0, min(CAP, r1 - r0), B);
+      for (int c0 = r0 + CAP; c0 < r1; c0 += CAP) {  // a row beyond one chunk: stage the rest in place
+        const int cnt = min(CAP, r1 - c0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < cnt; i += NT) {
+          const int s = p2g_slot(i);
+#pragma unroll
+          for (int f = 0; f < 12; f++) L[f * CP + s] = W[(int64_t)f * n + c0 + i];
+          L[12 * CP + s] = vc[c0 + i];
+        }
+        __syncthreads();
+        if (live) sum_chunk(o, cy, c0, cnt, B);
+      }
+    }
+    if (ri % nb1 == nb1 - 1 && o == 0) {  // plane x = b0 - 1 has seen every slab x-2 .. x+1
+      if (live && zok) {
+#pragma unroll
+        for (int j = 0; j < P2G_TY; j++) {
+          if (y0 + j >= y1) break;
+          const int64_t nid = ((int64_t)x * gy + (y0 + j)) * gz + z;
+          const T a = acc[j], s = ws[j];
+          if (NORM) {
+            G[nid] = s > T(0) ? a / s : T(0);
+            valid[nid] = s > T(0) ? 1 : 0;
+            Wt[nid] = s;
+          } else {
+            G[nid] = a;
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < P2G_TY; j++) acc[j] = ws[j] = T(0);
+    }
+  }
+}
+static_assert(P2G_TY == 8, "the row switch in k_p2g_gather lists the y-offsets 3 .. 1 - P2G_TY");
+
+// Launch of k_p2g_gather over face grid dims (gx, gy, gz). x is cut into segments so that
+// there are about 2048 workgroups (several per CU), but not below 8 planes per segment:
+// every segment walks three slabs more than it has planes.
+template <typename T, bool NORM>
+void launch_p2g_gather(hipStream_t st, GridArgs<T> ga, int gx, int gy, int gz, int n, const int* offs, const T* W,
+                       const T* vc, T* G, T* Wt, uint8_t* valid) {
+  const int nzt = (gz + P2G_TZ - 1) / P2G_TZ, nyt = (gy + P2G_TY - 1) / P2G_TY;
+  const int nseg = std::max(1, std::min((2048 + nzt * nyt - 1) / (nzt * nyt), gx / 8));
+  k_p2g_gather<T, NORM><<<nzt * nyt * nseg, P2G_TZ * P2G_WAVES, 0, st>>>(ga, gx, gy, gz, n, nzt, nyt, nseg, offs, W,
+                                                                        vc, G, Wt, valid);
 }
 
 // NaiveReconstructor::reconstruct (FluidSim/include/FluidSim/cpu/sdf.h:210-225) as a particle
@@ -1034,9 +1136,8 @@ int p2g_t(fs_ctx* c) {
                                                                  c->vel[a].as<T>(), c->p2g_w.as<T>(),
                                                                  c->sorted[0].as<T>());
     int gx = c->fdim[a][0], gy = c->fdim[a][1], gz = c->fdim[a][2];
-    k_p2g_gather<T><<<fs_blocks(p2g_gather_threads(gx, gy, gz), TPB), TPB, 0, c->stream>>>(
-        ga, gx, gy, gz, n, c->offs.as<int>(), c->p2g_w.as<T>(), c->sorted[0].as<T>(), c->g[a].as<T>(),
-        c->gw[a].as<T>(), c->gvalid[a].as<uint8_t>());
+    launch_p2g_gather<T, true>(c->stream, ga, gx, gy, gz, n, c->offs.as<int>(), c->p2g_w.as<T>(),
+                               c->sorted[0].as<T>(), c->g[a].as<T>(), c->gw[a].as<T>(), c->gvalid[a].as<uint8_t>());
     fs_prof_end(c, pf);
     FS_HIP(c, hipGetLastError());
     if (c->cfg.advection == FS_ADVECT_FLIP)
@@ -1600,9 +1701,8 @@ int p2g_sum_t(fs_ctx* c, DBuf* pos, DBuf* coef, DBuf* out) {
                                                                  pos[1].as<T>(), pos[2].as<T>(), coef[a].as<T>(),
                                                                  c->p2g_w.as<T>(), c->sorted[0].as<T>());
     const int gx = c->fdim[a][0], gy = c->fdim[a][1], gz = c->fdim[a][2];
-    k_p2g_gather<T, false><<<fs_blocks(p2g_gather_threads(gx, gy, gz), TPB), TPB, 0, c->stream>>>(
-        ga, gx, gy, gz, n, c->offs.as<int>(), c->p2g_w.as<T>(), c->sorted[0].as<T>(), out[a].as<T>(), nullptr,
-        nullptr);
+    launch_p2g_gather<T, false>(c->stream, ga, gx, gy, gz, n, c->offs.as<int>(), c->p2g_w.as<T>(),
+                                c->sorted[0].as<T>(), out[a].as<T>(), nullptr, nullptr);
     FS_HIP(c, hipGetLastError());
   }
   return FS_OK;

@@ -30,6 +30,15 @@ def dam_scene(n: int, seed: int = 2024):
     return pos, vel, scenes.no_collider(n)
 
 
+def dense_scene(n: int, seed: int = 11, copies: int = 10):
+    """Dam break with `copies` times the usual 8 particles per cell: a bin row holds more
+    particle records than the P2G gather stages at once, so it is summed in several chunks."""
+    mask = scenes.dam_break_mask(n)
+    pos = np.concatenate([scenes.particles_in_cells(mask, seed + k, 1.0 / n) for k in range(copies)])
+    vel = np.random.default_rng(seed).uniform(-0.5, 0.5, size=pos.shape)
+    return pos, vel, scenes.no_collider(n)
+
+
 def setup(sim, pos, vel, col):
     sim.set_collider_sdf(col)
     sim.set_particles(pos, vel)

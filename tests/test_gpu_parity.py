@@ -8,12 +8,12 @@ compared within tolerance (rel. L2 <= 1e-5, iteration count within 10%).
 import numpy as np
 import pytest
 
-from tests._pairs import ALL_ARRAYS, STAGES, assert_same, dam_scene, overlap_scene, setup
+from tests._pairs import ALL_ARRAYS, STAGES, assert_same, dam_scene, dense_scene, overlap_scene, setup
 from tests.conftest import announce
 
 pytestmark = pytest.mark.gpu
 
-SCENES = {"dam": dam_scene, "overlap": overlap_scene}
+SCENES = {"dam": dam_scene, "overlap": overlap_scene, "dense": dense_scene}
 
 
 def make_pair(hip, oracle, n, scene, dot_mode=None, **kw):
@@ -26,8 +26,8 @@ def make_pair(hip, oracle, n, scene, dot_mode=None, **kw):
     return g, o
 
 
-@pytest.mark.parametrize("advection", [1, 0], ids=["flip", "pic"])
-@pytest.mark.parametrize("scene", ["dam", "overlap"])
+@pytest.mark.parametrize("scene,advection", [("dam", 1), ("dam", 0), ("overlap", 1), ("overlap", 0), ("dense", 1)],
+                         ids=["dam-flip", "dam-pic", "overlap-flip", "overlap-pic", "dense-flip"])
 def test_stagewise_bitexact(hip, oracle, scene, advection):
     n, dt = 16, 0.004
     announce(f"stagewise scene={scene} advection={advection} n={n}")
