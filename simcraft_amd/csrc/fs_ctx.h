@@ -72,7 +72,6 @@ struct PcgState {
   DBuf partials2;                // second partial array (max reductions)
   DBuf scal;                     // PcgScalars
   DBuf xf, xb;                   // boundary exchange (forward/factor, backward): [2][nlive][S][8] + zero row
-  int sweep_lag = 0;             // extra chunks a sweep lets its producer get ahead after a wait
   DBuf tickets;                  // uint64[3]: brick tickets of the factor, forward and backward sweeps
   uint64_t ticket_base[3] = {0, 0, 0};
   PcgScalars* host_scal = nullptr;  // pinned

@@ -662,24 +662,28 @@ __global__ __launch_bounds__(256) void k_resid(Lay L, const double* __restrict__
 }
 
 // ---- MIC sweeps ---------------------------------------------------------------------------------
-// One wave per live brick. Lane (a, b) = (lane >> 3, lane & 7) owns column (8I+a, 8J+b) and at
-// slot s handles cell k = s - a - b, so its -x and -y neighbours at the same k were finished one
-// slot earlier by lanes a-1 and b-1 (lane shuffles) or, on the brick face, 7 slots earlier by
-// the neighbouring brick (exchange entries X[f][B][s][w]).
+// One workgroup of two waves per live brick. Lane (a, b) = (lane >> 3, lane & 7) owns column
+// (8I+a, 8J+b) and at slot s handles cell k = s - a - b, so its -x and -y neighbours at the same
+// k were finished one slot earlier by lanes a-1 and b-1 (lane shuffles) or, on the brick face,
+// 7 slots earlier by the neighbouring brick (exchange entries X[f][B][s][w]).
 //
-// The sweeps are bound by instruction issue of their single wave, so every address in the
-// streaming loop is a wave-uniform base in scalar registers plus the lane's fixed offset plus
-// an immediate: chunk c of a brick array starts at element (B*S + c*U)*64 and its element u
-// is at +u*64 + lane. Every lane loads the exchange row of its face; the row ZROW (all zeros,
-// after the entries) stands in for a face without a neighbour or past the last slot, and only
-// the face lanes look at the value. Producer stores and sentinel resets are exec-masked.
+// Chunk c (U slots) of a brick array is one contiguous run of U*64 elements, and the entries a
+// chunk consumes from one face are 64 doubles, one per lane.
+//   Wave 0 (loads and arithmetic) keeps SW_R<T> chunks of loads in flight: 16 bytes per lane
+//   per instruction for the arrays (moved to the lanes that own them through LDS) and one 8-byte
+//   load per lane per face for the entries. It issues no global store, so the waits the
+//   compiler places before using loaded values count loads only, which complete in issue order,
+//   and stay exact (with stores in flight the compiler has to wait for everything).
+//   Wave 1 (stores) takes each finished chunk from an LDS outbox: it hands the boundary values
+//   to the next bricks (one 8-byte store per lane and face), returns the consumed entries to the
+//   sentinel and writes the chunk's outputs with 16-byte stores.
+// The waves meet at two LDS counters: chunks finished by wave 0 and outbox slots released by
+// wave 1. Every wait is bounded by SPIN_LIMIT, after which the wave gives up, counts a timeout
+// and the host reports FS_DEVICE_ERROR.
 //
-// Two register sets alternate over the chunks: the inputs and boundary entries of chunk c+1 are
-// in flight while chunk c runs, and no set is ever copied (a copy would force the loads to
-// complete early). A chunk whose boundary entries were not all written when loaded is never
-// patched in place (that would merge register values and force every load to complete): the
-// pipeline is abandoned, the wave waits for the producer to get `lag` chunks ahead, and the
-// pipeline restarts at that chunk.
+// Exchange entries hold a sentinel until the producer stores the value (agent-scope atomic
+// stores and loads); the consumer uses the value and stores the sentinel back. An entry still
+// unwritten when its chunk starts is polled on a slow path, after which nothing is in flight.
 
 // High word of the sentinel. Arithmetic never returns a signalling NaN, so for computed values
 // the high word alone identifies the sentinel.
@@ -701,180 +705,265 @@ __device__ __forceinline__ float dpp_row(float v) {
 }
 constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHL1 = 0x101;
 
+constexpr int SW_NA = 6;       // input arrays per chunk
+constexpr int SW_CE = U * 64;  // elements per chunk and array
+constexpr int SW_K = 2;        // outbox slots
+typedef float sw_f4 __attribute__((ext_vector_type(4)));
+typedef double sw_d2 __attribute__((ext_vector_type(2)));
+template <typename V>
+using Vec16 = typename std::conditional<sizeof(V) == 8, sw_d2, sw_f4>::type;  // 16 bytes, a register tuple
+template <typename V>
+struct SwCfg {
+  static constexpr int OPS = SW_CE * (int)sizeof(V) / 1024;  // 16-byte loads per lane, array and chunk
+  static constexpr int PER16 = 16 / (int)sizeof(V);          // elements per 16-byte access
+};
+
+// Loads of one chunk, in wave 0's registers.
+template <typename V>
+struct SwSet {
+  Vec16<V> d[SW_NA][SwCfg<V>::OPS];
+  double x[2];
+};
+
+// Shared state of a sweep workgroup.
+template <typename V>
+struct SwLds {
+  __align__(16) V tb[SW_NA][SW_CE];       // the chunk being computed, one element per lane and slot
+  __align__(16) V ob[SW_K][2][SW_CE];     // outbox: outputs of finished chunks
+  double xb[2][64];                       // the chunk's exchange entries
+  int done, freed, B, stop;               // chunks finished by wave 0 / released by wave 1
+};
+
+__device__ __forceinline__ int lds_get(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_put(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// Wait until *p >= v; false if the wait gave up.
+__device__ __forceinline__ bool lds_wait(const int* p, int v, PcgScalars* sc) {
+  long spins = 0;
+  while (lds_get(p) < v) {
+    __builtin_amdgcn_s_sleep(1);
+    if ((++spins & 255) == 0) {
+      const int to = __hip_atomic_load(&sc->wait_timeouts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (to != 0 || spins > SPIN_LIMIT) {
+        if ((threadIdx.x & 63) == 0) atomicAdd(&sc->wait_timeouts, 1);
+        __builtin_amdgcn_s_waitcnt(0);  // leave nothing in flight on this path
+        return false;
+      }
+    }
+  }
+  return true;
+}
+
+// last_block for a single wave: true in exactly one wave of the launch, after every wave has
+// published its partial.
+__device__ inline bool last_wave(uint32_t* ctr, uint32_t nwaves) {
+  uint32_t old = 0;
+  if ((threadIdx.x & 63) == 0) {
+    __threadfence();
+    old = atomicAdd(ctr, 1u);
+  }
+  old = __shfl(old, 0, 64);
+  const bool is_last = old == nwaves - 1;
+  if (is_last) {
+    if ((threadIdx.x & 63) == 0) {
+      __threadfence();
+      atomicExch(ctr, 0u);
+    }
+    __threadfence();
+  }
+  return is_last;
+}
+
+// Slow path: poll this lane's two entries until both are written (or the wave gives up).
+// Ends with nothing in flight.
+__device__ __forceinline__ void sw_poll(const double* p0, const double* p1, double& v0, double& v1, bool& give_up,
+                                        PcgScalars* sc) {
+  long spins = 0;
+  while (((__builtin_amdgcn_ballot_w64(xunset(v0) | xunset(v1)) != 0) & !give_up)) {
+    __builtin_amdgcn_s_sleep(1);
+    v0 = xload(p0);
+    v1 = xload(p1);
+    if ((++spins & 255) == 0) {
+      const int to = __hip_atomic_load(&sc->wait_timeouts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (to != 0 || spins > SPIN_LIMIT) {
+        if ((threadIdx.x & 63) == 0) atomicAdd(&sc->wait_timeouts, 1);
+        give_up = true;
+      }
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+}
+
+// Element offset of chunk c in a brick array: forward chunks run up from slot 0, backward
+// chunks down from slot S-1 (chunk c then starts at slot S - (c+1)U).
+struct RunFwd {
+  __device__ size_t operator()(int c, int) const { return (size_t)c * SW_CE; }
+};
+struct RunBwd {
+  __device__ size_t operator()(int c, int S) const { return (size_t)(S - (c + 1) * U) * 64; }
+};
+
+// Wave 0: issue the loads of chunk c (past the end: the last chunk again, unused).
+template <int NARR, typename V, typename FX, typename RUN>
+__device__ __forceinline__ void sw_load(SwSet<V>& st, const V* const* src, int nc, int c, FX ent, int lane, RUN run,
+                                        int S) {
+  const int cs = min(c, nc - 1);
+  const size_t o = run(cs, S);
+#pragma unroll
+  for (int k = 0; k < NARR; k++) {
+#pragma unroll
+    for (int h = 0; h < SwCfg<V>::OPS; h++)
+      st.d[k][h] = *reinterpret_cast<const Vec16<V>*>(src[k] + o + h * (1024 / (int)sizeof(V)) +
+                                                     lane * SwCfg<V>::PER16);
+  }
+  st.x[0] = xload(ent(0, c));
+  st.x[1] = xload(ent(1, c));
+}
+
+// Wave 0: move a chunk's array loads to LDS, element e of array k at tb[k][e].
+template <int NARR, typename V>
+__device__ __forceinline__ void sw_spill(SwLds<V>& sh, const SwSet<V>& st, int lane) {
+#pragma unroll
+  for (int k = 0; k < NARR; k++) {
+#pragma unroll
+    for (int h = 0; h < SwCfg<V>::OPS; h++)
+      *reinterpret_cast<Vec16<V>*>(&sh.tb[k][h * (1024 / (int)sizeof(V)) + lane * SwCfg<V>::PER16]) = st.d[k][h];
+  }
+}
+
+// One chunk of an LDS array to global memory, 16 bytes per lane.
+template <typename V>
+__device__ __forceinline__ void sw_store(V* dst, const V* lds, int lane) {
+#pragma unroll
+  for (int h = 0; h < SwCfg<V>::OPS; h++) {
+    const int o = h * (1024 / (int)sizeof(V)) + lane * SwCfg<V>::PER16;
+    *reinterpret_cast<Vec16<V>*>(dst + o) = *reinterpret_cast<const Vec16<V>*>(lds + o);
+  }
+}
+
+
+// Wave 0's loop over the chunks of one sweep. BODY(c, slot) computes chunk c from sh.tb / sh.xb
+// into sh.ob[slot]; the loads of chunk c+2 are in flight meanwhile (two register sets, named
+// rather than indexed so that both stay in registers). Whole pairs of chunks run as
+// straight-line code (an exit inside would merge paths and cost exact waits).
+template <int NARR, typename V, typename FX, typename RUN, typename BODY>
+__device__ __forceinline__ void sw_wave0(SwLds<V>& sh, const V* const* src, int nc, int S, FX ent, RUN run,
+                                         PcgScalars* sc, int lane, bool& give_up, BODY body) {
+  SwSet<V> s0, s1;
+  sw_load<NARR>(s0, src, nc, 0, ent, lane, run, S);
+  sw_load<NARR>(s1, src, nc, 1, ent, lane, run, S);
+  auto chunk = [&](int c, SwSet<V>& cur) __attribute__((always_inline)) {
+    double v0 = cur.x[0], v1 = cur.x[1];
+    if ((__builtin_amdgcn_ballot_w64(xunset(v0) | xunset(v1)) != 0) & !give_up)
+      sw_poll(ent(0, c), ent(1, c), v0, v1, give_up, sc);
+    sh.xb[0][lane] = v0;
+    sh.xb[1][lane] = v1;
+    sw_spill<NARR>(sh, cur, lane);
+    sw_load<NARR>(cur, src, nc, c + 2, ent, lane, run, S);
+    const int slot = c % SW_K;
+    if (c >= SW_K && !lds_wait(&sh.freed, c - SW_K + 1, sc)) give_up = true;
+    body(c, slot);
+    lds_put(&sh.done, c + 1);
+  };
+  int c = 0;
+  for (; c + 2 <= nc; c += 2) {
+    chunk(c, s0);
+    chunk(c + 1, s1);
+  }
+  if (c < nc) chunk(c, s0);
+}
+
 // Forward sweep L q = r (project-solver.cc:275-288). UPDATE: first r += (-alpha) t and
 // ||r||_inf (saxpy + LinfNorm, :333-335); the last brick decides convergence.
-// V: storage and arithmetic type of vectors and coefficients; the exchange entries stay FP64.
-template <typename V>
-struct FwdSet {
-  V r[U], t[U], c1[U], c2[U], c3[U], e[U];
-  double xl[U], xd[U];
-};
-
-template <typename V>
-struct FwdK {  // wave-uniform
-  V* r;
-  const V *t, *cl, *cd, *cb, *E;
-  V* q;             // brick bases: element B*S*64
-  double* xL;       // f 0 entries of the -x neighbour, nullptr when there is none
-  double* xD;       // f 1 entries of the -y neighbour
-  double* oL;       // this brick's f 0 entries (read by its +x neighbour)
-  double* oD;       // this brick's f 1 entries
-  const double* Z;  // zero row
-  int S, NC, lag;
-  double nalpha;
-};
-
+// T: storage and arithmetic type of vectors and coefficients; the exchange entries stay FP64.
 template <bool UPDATE, typename T>
-__device__ __forceinline__ void fwd_load(const FwdK<T>& k, FwdSet<T>& S, int c, int lane) {
-  c = min(c, k.NC - 1);  // past the end: reload the last chunk (unused) rather than branch
-  const size_t o = (size_t)c * U * 64;
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const int i = u * 64 + lane;
-    S.r[u] = k.r[o + i];
-    S.t[u] = UPDATE ? k.t[o + i] : T(0);
-    S.c1[u] = k.cl[o + i];
-    S.c2[u] = k.cd[o + i];
-    S.c3[u] = k.cb[o + i];
-    S.e[u] = k.E[o + i];
+__global__ __launch_bounds__(128) void k_fwd(Lay L, T* __restrict__ r, const T* __restrict__ t,
+                                             const T* __restrict__ cl, const T* __restrict__ cd,
+                                             const T* __restrict__ cb, const T* __restrict__ E,
+                                             T* __restrict__ q, double* X, double* __restrict__ part2,
+                                             double* __restrict__ part, uint64_t* ticket, uint64_t tbase,
+                                             const int* __restrict__ order, PcgScalars* sc, int iter, int tol_mode,
+                                             double tol) {
+  enum { AR = 0, AL = 1, AD = 2, AB = 3, AE = 4, AT = 5 };  // t last: the plain sweep loads five arrays
+  constexpr int NARR = UPDATE ? 6 : 5;
+  __shared__ SwLds<T> sh;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, a = lane >> 3, b = lane & 7;
+  if (threadIdx.x == 0) {
+    const uint64_t tk = atomicAdd((unsigned long long*)ticket, 1ull) - tbase;
+    // a ticket outside [0, n) can only follow a failed launch; clamp it so every access
+    // stays in bounds (the exchange polls then time out and the solve reports an error)
+    sh.B = order[tk < (uint64_t)L.nlive ? tk : (uint64_t)(L.nlive - 1)];
+    sh.stop = UPDATE && sc->converged != STOP_RUN;
+    sh.done = 0;
+    sh.freed = 0;
   }
-}
-
-template <typename T>
-__device__ __forceinline__ void fwd_cross(const FwdK<T>& k, FwdSet<T>& S, int c, int a, int b) {
-  c = min(c, k.NC - 1);
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const int s = c * U + u;
-    const double* pL = (k.xL != nullptr && s + 7 < k.S) ? k.xL + (size_t)(s + 7) * 8 : k.Z;
-    const double* pD = (k.xD != nullptr && s + 7 < k.S) ? k.xD + (size_t)(s + 7) * 8 : k.Z;
-    S.xl[u] = xload(pL + b);
-    S.xd[u] = xload(pD + a);
-  }
-}
-
-// Wave-uniform: every boundary entry of the chunk had been written when it was loaded.
-template <typename T>
-__device__ __forceinline__ bool fwd_ready(const FwdSet<T>& S, int a, int b, bool give_up) {
-  bool wait = false;
-#pragma unroll
-  for (int u = 0; u < U; u++) wait |= ((a == 0) & xunset(S.xl[u])) | ((b == 0) & xunset(S.xd[u]));
-  return give_up || __builtin_amdgcn_ballot_w64(wait) == 0;
-}
-
-// Wait until the boundary entries of chunks c .. c+lag are written; true if a wait gave up.
-template <typename T>
-__device__ __forceinline__ bool fwd_wait(const FwdK<T>& k, int c, int a, int b, PcgScalars* sc) {
-  bool ok = true;
-  const int c1 = min(c + k.lag, k.NC - 1);
-  for (int s = c * U; s < (c1 + 1) * U && s + 7 < k.S; s++) {
-    if (k.xL != nullptr && a == 0) ok = ok && xwait(k.xL + (size_t)(s + 7) * 8 + b, sc);
-    if (k.xD != nullptr && b == 0) ok = ok && xwait(k.xD + (size_t)(s + 7) * 8 + a, sc);
-  }
-  return __builtin_amdgcn_ballot_w64(!ok) != 0;
-}
-
-template <bool UPDATE, typename T>
-__device__ __forceinline__ void fwd_run(const FwdK<T>& k, const FwdSet<T>& S, int c, int lane, int a, int b, T& qprev,
-                                        double& mx, double& ss) {
-  const int s0 = c * U;
-  const size_t o = (size_t)s0 * 64;
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const int s = s0 + u;
-    const int i = u * 64 + lane;
-    T rr = S.r[u];
-    if (UPDATE) {
-      rr = rr + (T)k.nalpha * S.t[u];
-      k.r[o + i] = rr;
-      const double av = fabs((double)rr);
-      mx = (mx < av) ? av : mx;
-      ss += (double)rr * (double)rr;
-    }
-    T qL = __shfl_up(qprev, 8, 64);
-    T qD = dpp_row<DPP_ROW_SHR1>(qprev);
-    qL = a == 0 ? (T)S.xl[u] : qL;
-    qD = b == 0 ? (T)S.xd[u] : qD;
-    T tt = rr - S.c1[u] * qL;
-    tt = tt - S.c2[u] * qD;
-    tt = tt - S.c3[u] * qprev;
-    const T qv = tt * S.e[u];
-    k.q[o + i] = qv;
-    if (a == 7) xstore(k.oL + (size_t)s * 8 + b, (double)qv);
-    if (b == 7) xstore(k.oD + (size_t)s * 8 + a, (double)qv);
-    qprev = qv;
-  }
-#pragma unroll
-  for (int u = 0; u < U; u++) {  // hand the consumed entries back
-    const int s = s0 + u;
-    if (s + 7 < k.S) {
-      if (k.xL != nullptr && a == 0) xstore(k.xL + (size_t)(s + 7) * 8 + b, xsent());
-      if (k.xD != nullptr && b == 0) xstore(k.xD + (size_t)(s + 7) * 8 + a, xsent());
-    }
-  }
-}
-
-template <bool UPDATE, typename T>
-__global__ __launch_bounds__(64) void k_fwd(Lay L, T* __restrict__ r, const T* __restrict__ t,
-                                            const T* __restrict__ cl, const T* __restrict__ cd,
-                                            const T* __restrict__ cb, const T* __restrict__ E,
-                                            T* __restrict__ q, double* X, double* __restrict__ part2,
-                                            double* __restrict__ part, uint64_t* ticket, uint64_t tbase,
-                                            const int* __restrict__ order, PcgScalars* sc, int iter, int tol_mode,
-                                            double tol, int lag) {
-  const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
-  const int B = __builtin_amdgcn_readfirstlane(take_ticket(ticket, tbase, order, L.nlive));
-  if (UPDATE && sc->converged != STOP_RUN) return;
-  // readfirstlane: the ticket branch above would otherwise leave nlive in a divergent phi
-  const size_t brick = (size_t)B * L.S * 64, face = (size_t)__builtin_amdgcn_readfirstlane(L.nlive) * L.S * 8;
+  __syncthreads();
+  if (sh.stop) return;
+  const int B = __builtin_amdgcn_readfirstlane(sh.B);
+  const int S = __builtin_amdgcn_readfirstlane(L.S), NC = S / U;
+  const size_t brick = (size_t)B * S * 64, face = (size_t)__builtin_amdgcn_readfirstlane(L.nlive) * S * 8;
   const int2 ij = L.bij[B];
   const int BL = __builtin_amdgcn_readfirstlane(ij.x > 0 ? L.live[(ij.x - 1) * L.nbj + ij.y] : -1);
   const int BD = __builtin_amdgcn_readfirstlane(ij.y > 0 ? L.live[ij.x * L.nbj + ij.y - 1] : -1);
-  FwdK<T> k;
-  k.r = r + brick;
-  k.t = t + brick;
-  k.cl = cl + brick;
-  k.cd = cd + brick;
-  k.cb = cb + brick;
-  k.E = E + brick;
-  k.q = q + brick;
-  k.xL = BL >= 0 ? X + (size_t)BL * L.S * 8 : nullptr;
-  k.xD = BD >= 0 ? X + face + (size_t)BD * L.S * 8 : nullptr;
-  k.oL = X + (size_t)B * L.S * 8;
-  k.oD = X + face + (size_t)B * L.S * 8;
-  k.Z = X + 2 * face;
-  k.S = L.S;
-  k.NC = L.S / U;
-  k.lag = lag;
-  k.nalpha = UPDATE ? -sc->alpha : 0.0;
+  const double* xn[2] = {BL >= 0 ? X + (size_t)BL * S * 8 : nullptr,  // f 0 entries of the -x neighbour
+                         BD >= 0 ? X + face + (size_t)BD * S * 8 : nullptr};
+  const double* Z = X + 2 * face;  // zero row: no neighbour, or past the last slot
+  // this lane's entry of chunk c on face f: slot cU + lane/8 + 7 of the neighbour, index lane % 8
+  const int es = (lane >> 3) + 7, ew = lane & 7;
+  auto live = [&](int f, int c) __attribute__((always_inline)) { return xn[f] != nullptr && c * U + es < S; };
+  auto ent = [&](int f, int c) __attribute__((always_inline)) -> const double* {
+    return live(f, c) ? xn[f] + (size_t)(c * U + es) * 8 + ew : Z + ew;
+  };
+  if (wave == 1) {  // stores
+    double* o0 = X + (size_t)B * S * 8;  // this brick's f 0 entries (read by its +x neighbour)
+    double* o1 = X + face + (size_t)B * S * 8;
+    for (int c = 0; c < NC; c++) {
+      lds_wait(&sh.done, c + 1, sc);
+      const T* qc = &sh.ob[c % SW_K][0][0];
+      // boundary values first: lane e hands over slot cU + e/8, index e % 8
+      xstore(o0 + (size_t)c * U * 8 + lane, (double)qc[(lane >> 3) * 64 + 56 + (lane & 7)]);
+      xstore(o1 + (size_t)c * U * 8 + lane, (double)qc[(lane >> 3) * 64 + (lane & 7) * 8 + 7]);
+      if (live(0, c)) xstore(const_cast<double*>(ent(0, c)), xsent());  // consumed entries back
+      if (live(1, c)) xstore(const_cast<double*>(ent(1, c)), xsent());
+      sw_store(q + brick + RunFwd{}(c, S), qc, lane);
+      if (UPDATE) sw_store(r + brick + RunFwd{}(c, S), &sh.ob[c % SW_K][1][0], lane);
+      lds_put(&sh.freed, c + 1);
+    }
+    return;
+  }
+  const T* src[SW_NA] = {r + brick, cl + brick, cd + brick, cb + brick, E + brick, t + brick};
+  const double nalpha = UPDATE ? -sc->alpha : 0.0;
   bool give_up = false;
-  FwdSet<T> A, Bs;
   T qprev = T(0);
   double mx = 0.0, ss = 0.0;
-  for (int c0 = 0;;) {  // (re)start the pipeline at chunk c0
-    fwd_load<UPDATE, T>(k, A, c0, lane);
-    fwd_cross(k, A, c0, a, b);
-    int bail = -1, c = c0;
-    // Whole pairs of chunks only: an exit between the two halves would merge register states
-    // at the loop latch and force every load to complete there.
-    for (; c + 1 < k.NC; c += 2) {
-      fwd_load<UPDATE, T>(k, Bs, c + 1, lane);
-      fwd_cross(k, Bs, c + 1, a, b);
-      if (!fwd_ready(A, a, b, give_up)) { bail = c; break; }
-      fwd_run<UPDATE, T>(k, A, c, lane, a, b, qprev, mx, ss);
-      fwd_load<UPDATE, T>(k, A, c + 2, lane);
-      fwd_cross(k, A, c + 2, a, b);
-      if (!fwd_ready(Bs, a, b, give_up)) { bail = c + 1; break; }
-      fwd_run<UPDATE, T>(k, Bs, c + 1, lane, a, b, qprev, mx, ss);
+  sw_wave0<NARR>(sh, src, NC, S, ent, RunFwd{}, sc, lane, give_up, [&](int c, int slot) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = u * 64 + lane;
+      T rr = sh.tb[AR][i];
+      if (UPDATE) {
+        rr = rr + (T)nalpha * sh.tb[AT][i];
+        sh.ob[slot][1][i] = rr;
+        const double av = fabs((double)rr);
+        mx = (mx < av) ? av : mx;
+        ss += (double)rr * (double)rr;
+      }
+      const T xq0 = (T)sh.xb[0][u * 8 + b], xq1 = (T)sh.xb[1][u * 8 + a];  // read by every lane: no branch
+      T qL = __shfl_up(qprev, 8, 64);
+      T qD = dpp_row<DPP_ROW_SHR1>(qprev);
+      qL = a == 0 ? xq0 : qL;
+      qD = b == 0 ? xq1 : qD;
+      T tt = rr - sh.tb[AL][i] * qL;
+      tt = tt - sh.tb[AD][i] * qD;
+      tt = tt - sh.tb[AB][i] * qprev;
+      const T qv = tt * sh.tb[AE][i];
+      sh.ob[slot][0][i] = qv;
+      qprev = qv;
     }
-    if (bail < 0 && c < k.NC) {  // odd chunk count: A holds the last chunk
-      if (!fwd_ready(A, a, b, give_up)) bail = c;
-      else fwd_run<UPDATE, T>(k, A, c, lane, a, b, qprev, mx, ss);
-    }
-    if (bail < 0) break;
-    give_up = fwd_wait(k, bail, a, b, sc);
-    c0 = bail;
-  }
+  });
   if (UPDATE) {
     double wm = wave_max(mx);
     double wsum = wave_sum(ss);
@@ -882,7 +971,7 @@ __global__ __launch_bounds__(64) void k_fwd(Lay L, T* __restrict__ r, const T* _
       part2[B] = wm;
       part[B] = wsum;
     }
-    if (last_block(&sc->done_ctr[DC_FWD], L.nlive, 64)) {
+    if (last_wave(&sc->done_ctr[DC_FWD], L.nlive)) {
       double res = max_wave(part2, L.nlive);
       double rr2 = tree256_wave(part, L.nlive);
       if (lane == 0) {
@@ -903,171 +992,86 @@ __global__ __launch_bounds__(64) void k_fwd(Lay L, T* __restrict__ r, const T* _
 }
 
 // Backward sweep L^T z = q (project-solver.cc:289-305) and the fixed-tree z.r; the last brick
-// forms beta = sigma_new / sigma (INIT: sigma = z.r). Slots run from S-1 down to 0; chunk c
-// covers slots S-1-c*U down to S-U-c*U.
-template <typename V>
-struct BwdSet {
-  V q[U], d1[U], d2[U], d3[U], e[U], r[U];
-  double xr[U], xu[U];
-};
-
-template <typename V>
-struct BwdK {  // wave-uniform
-  const V *q, *d1, *d2, *d3, *E, *r;
-  V* z;
-  double* xR;  // f 0 entries of the +x neighbour, nullptr when there is none
-  double* xU;  // f 1 entries of the +y neighbour
-  double* oR;  // this brick's f 0 entries (read by its -x neighbour)
-  double* oU;
-  const double* Z;
-  int S, NC, lag;
-};
-
-// element offset of the lowest slot of chunk c; slot S-1-c*U-u is at +(U-1-u)*64
-template <typename T>
-__device__ __forceinline__ size_t bwd_base(const BwdK<T>& k, int c) { return (size_t)(k.S - (c + 1) * U) * 64; }
-
-template <typename T>
-__device__ __forceinline__ void bwd_load(const BwdK<T>& k, BwdSet<T>& S, int c, int lane) {
-  c = min(c, k.NC - 1);
-  const size_t o = bwd_base(k, c);
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const int i = (U - 1 - u) * 64 + lane;
-    S.q[u] = k.q[o + i];
-    S.d1[u] = k.d1[o + i];
-    S.d2[u] = k.d2[o + i];
-    S.d3[u] = k.d3[o + i];
-    S.e[u] = k.E[o + i];
-    S.r[u] = k.r[o + i];
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ void bwd_cross(const BwdK<T>& k, BwdSet<T>& S, int c, int a, int b) {
-  c = min(c, k.NC - 1);
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const int s = k.S - 1 - (c * U + u);
-    const double* pR = (k.xR != nullptr && s >= 7) ? k.xR + (size_t)(s - 7) * 8 : k.Z;
-    const double* pU = (k.xU != nullptr && s >= 7) ? k.xU + (size_t)(s - 7) * 8 : k.Z;
-    S.xr[u] = xload(pR + b);
-    S.xu[u] = xload(pU + a);
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ bool bwd_ready(const BwdSet<T>& S, int a, int b, bool give_up) {
-  bool wait = false;
-#pragma unroll
-  for (int u = 0; u < U; u++) wait |= ((a == 7) & xunset(S.xr[u])) | ((b == 7) & xunset(S.xu[u]));
-  return give_up || __builtin_amdgcn_ballot_w64(wait) == 0;
-}
-
-template <typename T>
-__device__ __forceinline__ bool bwd_wait(const BwdK<T>& k, int c, int a, int b, PcgScalars* sc) {
-  bool ok = true;
-  const int c1 = min(c + k.lag, k.NC - 1);
-  for (int s = k.S - 1 - c * U; s > k.S - 1 - (c1 + 1) * U && s >= 7; s--) {
-    if (k.xR != nullptr && a == 7) ok = ok && xwait(k.xR + (size_t)(s - 7) * 8 + b, sc);
-    if (k.xU != nullptr && b == 7) ok = ok && xwait(k.xU + (size_t)(s - 7) * 8 + a, sc);
-  }
-  return __builtin_amdgcn_ballot_w64(!ok) != 0;
-}
-
-template <typename T>
-__device__ __forceinline__ void bwd_run(const BwdK<T>& k, const BwdSet<T>& S, int c, int lane, int a, int b, T& zprev,
-                                        double& acc) {
-  const size_t o = bwd_base(k, c);
-  const int s0 = k.S - 1 - c * U;
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const int s = s0 - u;
-    const int i = (U - 1 - u) * 64 + lane;
-    T zR = __shfl_down(zprev, 8, 64);
-    T zU = dpp_row<DPP_ROW_SHL1>(zprev);
-    zR = a == 7 ? (T)S.xr[u] : zR;
-    zU = b == 7 ? (T)S.xu[u] : zU;
-    T tt = S.q[u] - S.d1[u] * zR;
-    tt = tt - S.d2[u] * zU;
-    tt = tt - S.d3[u] * zprev;
-    const T zv = tt * S.e[u];
-    k.z[o + i] = zv;
-    if (a == 0) xstore(k.oR + (size_t)s * 8 + b, (double)zv);
-    if (b == 0) xstore(k.oU + (size_t)s * 8 + a, (double)zv);
-    zprev = zv;
-    acc += (double)zv * (double)S.r[u];
-  }
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const int s = s0 - u;
-    if (s >= 7) {
-      if (k.xR != nullptr && a == 7) xstore(k.xR + (size_t)(s - 7) * 8 + b, xsent());
-      if (k.xU != nullptr && b == 7) xstore(k.xU + (size_t)(s - 7) * 8 + a, xsent());
-    }
-  }
-}
-
+// forms beta = sigma_new / sigma (INIT: sigma = z.r). Slots run from S-1 down to 0: chunk c
+// covers slots S-1-cU down to S-U-cU, and its u-th slot, S-1-cU-u, is element (U-1-u)*64 + lane
+// of the chunk's run.
 template <bool INIT, typename T>
-__global__ __launch_bounds__(64) void k_bwd(Lay L, const T* __restrict__ qv, const T* __restrict__ dr,
-                                            const T* __restrict__ du, const T* __restrict__ df,
-                                            const T* __restrict__ E, const T* __restrict__ r,
-                                            T* __restrict__ z, double* X, double* __restrict__ part,
-                                            uint64_t* ticket, uint64_t tbase, const int* __restrict__ order,
-                                            PcgScalars* sc, int lag) {
-  const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
-  const int B = __builtin_amdgcn_readfirstlane(take_ticket(ticket, tbase, order, L.nlive));
-  if (!INIT && sc->converged != STOP_RUN) return;
-  // readfirstlane: the ticket branch above would otherwise leave nlive in a divergent phi
-  const size_t brick = (size_t)B * L.S * 64, face = (size_t)__builtin_amdgcn_readfirstlane(L.nlive) * L.S * 8;
+__global__ __launch_bounds__(128) void k_bwd(Lay L, const T* __restrict__ qv, const T* __restrict__ dr,
+                                             const T* __restrict__ du, const T* __restrict__ df,
+                                             const T* __restrict__ E, const T* __restrict__ r,
+                                             T* __restrict__ z, double* X, double* __restrict__ part,
+                                             uint64_t* ticket, uint64_t tbase, const int* __restrict__ order,
+                                             PcgScalars* sc) {
+  enum { AQ = 0, A1 = 1, A2 = 2, A3 = 3, AE = 4, AR = 5 };
+  __shared__ SwLds<T> sh;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, a = lane >> 3, b = lane & 7;
+  if (threadIdx.x == 0) {
+    const uint64_t tk = atomicAdd((unsigned long long*)ticket, 1ull) - tbase;
+    sh.B = order[tk < (uint64_t)L.nlive ? tk : (uint64_t)(L.nlive - 1)];
+    sh.stop = !INIT && sc->converged != STOP_RUN;
+    sh.done = 0;
+    sh.freed = 0;
+  }
+  __syncthreads();
+  if (sh.stop) return;
+  const int B = __builtin_amdgcn_readfirstlane(sh.B);
+  const int S = __builtin_amdgcn_readfirstlane(L.S), NC = S / U;
+  const size_t brick = (size_t)B * S * 64, face = (size_t)__builtin_amdgcn_readfirstlane(L.nlive) * S * 8;
   const int2 ij = L.bij[B];
   const int BR = __builtin_amdgcn_readfirstlane(ij.x + 1 < L.nbi ? L.live[(ij.x + 1) * L.nbj + ij.y] : -1);
   const int BU = __builtin_amdgcn_readfirstlane(ij.y + 1 < L.nbj ? L.live[ij.x * L.nbj + ij.y + 1] : -1);
-  BwdK<T> k;
-  k.q = qv + brick;
-  k.d1 = dr + brick;
-  k.d2 = du + brick;
-  k.d3 = df + brick;
-  k.E = E + brick;
-  k.r = r + brick;
-  k.z = z + brick;
-  k.xR = BR >= 0 ? X + (size_t)BR * L.S * 8 : nullptr;
-  k.xU = BU >= 0 ? X + face + (size_t)BU * L.S * 8 : nullptr;
-  k.oR = X + (size_t)B * L.S * 8;
-  k.oU = X + face + (size_t)B * L.S * 8;
-  k.Z = X + 2 * face;
-  k.S = L.S;
-  k.NC = L.S / U;
-  k.lag = lag;
+  const double* xn[2] = {BR >= 0 ? X + (size_t)BR * S * 8 : nullptr,  // f 0 entries of the +x neighbour
+                         BU >= 0 ? X + face + (size_t)BU * S * 8 : nullptr};
+  const double* Z = X + 2 * face;
+  // this lane's entry of chunk c on face f: slot S-1-cU-lane/8 - 7 of the neighbour, index lane % 8
+  auto slot_of = [&](int c) __attribute__((always_inline)) { return S - 1 - c * U - (lane >> 3); };
+  const int ew = lane & 7;
+  auto live = [&](int f, int c) __attribute__((always_inline)) { return xn[f] != nullptr && slot_of(c) >= 7; };
+  auto ent = [&](int f, int c) __attribute__((always_inline)) -> const double* {
+    return live(f, c) ? xn[f] + (size_t)(slot_of(c) - 7) * 8 + ew : Z + ew;
+  };
+  if (wave == 1) {  // stores
+    double* o0 = X + (size_t)B * S * 8;  // this brick's f 0 entries (read by its -x neighbour)
+    double* o1 = X + face + (size_t)B * S * 8;
+    for (int c = 0; c < NC; c++) {
+      lds_wait(&sh.done, c + 1, sc);
+      const T* zc = &sh.ob[c % SW_K][0][0];
+      // boundary values first: lane e hands over slot S-1-cU-e/8, index e % 8
+      const int so = slot_of(c), e0 = (U - 1 - (lane >> 3)) * 64;
+      xstore(o0 + (size_t)so * 8 + ew, (double)zc[e0 + ew]);
+      xstore(o1 + (size_t)so * 8 + ew, (double)zc[e0 + ew * 8]);
+      if (live(0, c)) xstore(const_cast<double*>(ent(0, c)), xsent());
+      if (live(1, c)) xstore(const_cast<double*>(ent(1, c)), xsent());
+      sw_store(z + brick + RunBwd{}(c, S), zc, lane);
+      lds_put(&sh.freed, c + 1);
+    }
+    return;
+  }
+  const T* src[SW_NA] = {qv + brick, dr + brick, du + brick, df + brick, E + brick, r + brick};
   bool give_up = false;
-  BwdSet<T> A, Bs;
   T zprev = T(0);
   double acc = 0.0;
-  for (int c0 = 0;;) {
-    bwd_load(k, A, c0, lane);
-    bwd_cross(k, A, c0, a, b);
-    int bail = -1, c = c0;
-    for (; c + 1 < k.NC; c += 2) {
-      bwd_load(k, Bs, c + 1, lane);
-      bwd_cross(k, Bs, c + 1, a, b);
-      if (!bwd_ready(A, a, b, give_up)) { bail = c; break; }
-      bwd_run(k, A, c, lane, a, b, zprev, acc);
-      bwd_load(k, A, c + 2, lane);
-      bwd_cross(k, A, c + 2, a, b);
-      if (!bwd_ready(Bs, a, b, give_up)) { bail = c + 1; break; }
-      bwd_run(k, Bs, c + 1, lane, a, b, zprev, acc);
+  sw_wave0<SW_NA>(sh, src, NC, S, ent, RunBwd{}, sc, lane, give_up, [&](int c, int slot) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = (U - 1 - u) * 64 + lane;
+      const T xz0 = (T)sh.xb[0][u * 8 + b], xz1 = (T)sh.xb[1][u * 8 + a];  // read by every lane: no branch
+      T zR = __shfl_down(zprev, 8, 64);
+      T zU = dpp_row<DPP_ROW_SHL1>(zprev);
+      zR = a == 7 ? xz0 : zR;
+      zU = b == 7 ? xz1 : zU;
+      T tt = sh.tb[AQ][i] - sh.tb[A1][i] * zR;
+      tt = tt - sh.tb[A2][i] * zU;
+      tt = tt - sh.tb[A3][i] * zprev;
+      const T zv = tt * sh.tb[AE][i];
+      sh.ob[slot][0][i] = zv;
+      zprev = zv;
+      acc += (double)zv * (double)sh.tb[AR][i];
     }
-    if (bail < 0 && c < k.NC) {
-      if (!bwd_ready(A, a, b, give_up)) bail = c;
-      else bwd_run(k, A, c, lane, a, b, zprev, acc);
-    }
-    if (bail < 0) break;
-    give_up = bwd_wait(k, bail, a, b, sc);
-    c0 = bail;
-  }
+  });
   double ws = wave_sum(acc);
   if (lane == 0) part[B] = ws;
-  if (last_block(&sc->done_ctr[DC_BWD], L.nlive, 64)) {
+  if (last_wave(&sc->done_ctr[DC_BWD], L.nlive)) {
     double sig = tree256_wave(part, L.nlive);
     if (lane == 0) {
       if (INIT) {
@@ -1220,7 +1224,6 @@ int pcg_build(fs_ctx* c) {
   k_fill_sentinel<<<fs_blocks((int64_t)nent, 256), 256, 0, c->stream>>>((int64_t)nent, P.xb.as<double>());
   FS_HIP(c, hipMemsetAsync(P.xf.as<double>() + nent, 0, 64 * sizeof(double), c->stream));
   FS_HIP(c, hipMemsetAsync(P.xb.as<double>() + nent, 0, 64 * sizeof(double), c->stream));
-  if (const char* lag = getenv("FS_SWEEP_LAG")) P.sweep_lag = std::max(0, atoi(lag));
   FS_HIP(c, hipGetLastError());
   const size_t ne = (size_t)std::max<int64_t>(P.ne, 1);
   for (DBuf* d : {&P.adiag, &P.al, &P.ad, &P.ab, &P.rhs, &P.e, &P.cl, &P.cd, &P.cb, &P.dr, &P.du, &P.df, &P.x, &P.r,
@@ -1321,10 +1324,10 @@ int launch_fwd(fs_ctx* c, bool update, int iter) {
   Sys<S> y = sys_of<S>(P);
   uint64_t* tk = P.tickets.as<uint64_t>();
   auto go = [&](auto kern) {
-    kern<<<P.nlive, 64, 0, c->stream>>>(L, y.r, y.t, y.cl, y.cd, y.cb, y.e, y.q, P.xf.as<double>(),
+    kern<<<P.nlive, 128, 0, c->stream>>>(L, y.r, y.t, y.cl, y.cd, y.cb, y.e, y.q, P.xf.as<double>(),
                                         P.partials2.as<double>(), P.partials.as<double>(), tk + SW_FWD,
                                         P.ticket_base[SW_FWD], P.fwd_order.as<int>(), P.scal.as<PcgScalars>(), iter,
-                                        c->cfg.tol_mode, c->cfg.cg_tolerance, P.sweep_lag);
+                                        c->cfg.tol_mode, c->cfg.cg_tolerance);
   };
   const double bytes = (update ? 8.0 : 6.0) * sizeof(S) * c->n_active_last;
   const int pf = fs_prof_begin(c, FS_KERNEL_MIC_FWD, bytes);
@@ -1343,9 +1346,9 @@ int launch_bwd(fs_ctx* c, bool init) {
   Sys<S> y = sys_of<S>(P);
   uint64_t* tk = P.tickets.as<uint64_t>();
   auto go = [&](auto kern) {
-    kern<<<P.nlive, 64, 0, c->stream>>>(L, y.q, y.dr, y.du, y.df, y.e, y.r, y.z, P.xb.as<double>(),
+    kern<<<P.nlive, 128, 0, c->stream>>>(L, y.q, y.dr, y.du, y.df, y.e, y.r, y.z, P.xb.as<double>(),
                                         P.partials.as<double>(), tk + SW_BWD, P.ticket_base[SW_BWD],
-                                        P.bwd_order.as<int>(), P.scal.as<PcgScalars>(), P.sweep_lag);
+                                        P.bwd_order.as<int>(), P.scal.as<PcgScalars>());
   };
   const int pf = fs_prof_begin(c, FS_KERNEL_MIC_BWD, 7.0 * sizeof(S) * c->n_active_last);
   if (init) go(k_bwd<true, S>);
