@@ -546,33 +546,61 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const S* __restrict__ x, co
                                               PcgScalars* sc, int iter) {
   __shared__ double sm[4];
   if (PCG && sc->converged != STOP_RUN) return;
+  // A thread's 16 elements (one per group) are independent until the dot product, which adds
+  // them in order: their flags are loaded first, then every operand of MB elements at a time,
+  // unconditionally (an absent neighbour reads the element itself and is discarded by a
+  // select, so the sums keep their order), so the loads overlap instead of waiting one
+  // memory latency per operand and element. Groups past the end reread the last group and are
+  // dropped.
+  constexpr int MB = 4;
+  const int ne = (int)L.ne;
+  uint8_t fl[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) fl[m] = nbm[min((int)blockIdx.x * CH + m * 256, ne - 256) + (int)threadIdx.x];
   double acc = 0.0;
-  for (int m = 0; m < 16; m++) {
-    const int base = blockIdx.x * CH + m * 256;
-    if (base >= L.ne) break;  // L.ne is a multiple of 512
-    const Group g = group_of(L, base);
-    const int e = base + threadIdx.x, lane = threadIdx.x & 63, s = g.s0 + (threadIdx.x >> 6);
-    const int a = lane >> 3, b = lane & 7;
-    const int f = nbm[e];
-    S v = S(0);
-    const S xc = x[e];
-    if (f & NB_SELF) {
-      v = adiag[e] * xc;
-      if (f & NB_L) v = v + al[e] * x[nb_left(L, g, e, s, a, b)];
-      if (f & NB_D) v = v + ad[e] * x[nb_down(L, g, e, s, a, b)];
-      if (f & NB_B) v = v + ab[e] * x[e - 64];
-      if (f & NB_R) {
-        const int n = nb_right(L, g, e, s, a, b);
-        v = v + al[n] * x[n];
-      }
-      if (f & NB_U) {
-        const int n = nb_up(L, g, e, s, a, b);
-        v = v + ad[n] * x[n];
-      }
-      if (f & NB_F) v = v + ab[e + 64] * x[e + 64];
+#pragma unroll
+  for (int m0 = 0; m0 < 16; m0 += MB) {
+    int e[MB], iL[MB], iD[MB], iB[MB], iR[MB], iU[MB], iF[MB];
+    bool in[MB];
+#pragma unroll
+    for (int q = 0; q < MB; q++) {
+      const int raw = (int)blockIdx.x * CH + (m0 + q) * 256;
+      in[q] = raw < ne;
+      const int base = in[q] ? raw : ne - 256;
+      const Group g = group_of(L, base);
+      const int ee = base + (int)threadIdx.x, lane = threadIdx.x & 63, s = g.s0 + (threadIdx.x >> 6);
+      const int a = lane >> 3, b = lane & 7, f = fl[m0 + q];
+      e[q] = ee;
+      iL[q] = (f & NB_L) ? nb_left(L, g, ee, s, a, b) : ee;
+      iD[q] = (f & NB_D) ? nb_down(L, g, ee, s, a, b) : ee;
+      iB[q] = (f & NB_B) ? ee - 64 : ee;
+      iR[q] = (f & NB_R) ? nb_right(L, g, ee, s, a, b) : ee;
+      iU[q] = (f & NB_U) ? nb_up(L, g, ee, s, a, b) : ee;
+      iF[q] = (f & NB_F) ? ee + 64 : ee;
     }
-    y[e] = v;
-    acc += (double)xc * (double)v;
+    S xc[MB], v[MB];
+#pragma unroll
+    for (int q = 0; q < MB; q++) {
+      const int f = fl[m0 + q];
+      xc[q] = x[e[q]];
+      S w = adiag[e[q]] * xc[q];
+      const S cL = al[e[q]], cD = ad[e[q]], cB = ab[e[q]], cR = al[iR[q]], cU = ad[iU[q]], cF = ab[iF[q]];
+      const S xL = x[iL[q]], xD = x[iD[q]], xB = x[iB[q]], xR = x[iR[q]], xU = x[iU[q]], xF = x[iF[q]];
+      w = (f & NB_L) ? w + cL * xL : w;
+      w = (f & NB_D) ? w + cD * xD : w;
+      w = (f & NB_B) ? w + cB * xB : w;
+      w = (f & NB_R) ? w + cR * xR : w;
+      w = (f & NB_U) ? w + cU * xU : w;
+      w = (f & NB_F) ? w + cF * xF : w;
+      v[q] = (f & NB_SELF) ? w : S(0);
+    }
+#pragma unroll
+    for (int q = 0; q < MB; q++) {
+      if (in[q]) {
+        y[e[q]] = v[q];
+        acc += (double)xc[q] * (double)v[q];
+      }
+    }
   }
   if (!PCG) return;
   double bs = block_sum256(acc, sm);
@@ -714,6 +742,7 @@ template <typename V>
 using Vec16 = typename std::conditional<sizeof(V) == 8, sw_d2, sw_f4>::type;  // 16 bytes, a register tuple
 template <typename V>
 struct SwCfg {
+  static constexpr int R = 2;                                // chunks of loads in flight (register sets)
   static constexpr int OPS = SW_CE * (int)sizeof(V) / 1024;  // 16-byte loads per lane, array and chunk
   static constexpr int PER16 = 16 / (int)sizeof(V);          // elements per 16-byte access
 };
@@ -846,15 +875,17 @@ __device__ __forceinline__ void sw_store(V* dst, const V* lds, int lane) {
 
 
 // Wave 0's loop over the chunks of one sweep. BODY(c, slot) computes chunk c from sh.tb / sh.xb
-// into sh.ob[slot]; the loads of chunk c+2 are in flight meanwhile (two register sets, named
-// rather than indexed so that both stay in registers). Whole pairs of chunks run as
+// into sh.ob[slot]; the loads of the next SwCfg::R - 1 chunks are in flight meanwhile (register
+// sets named rather than indexed so that all stay in registers). Whole groups of chunks run as
 // straight-line code (an exit inside would merge paths and cost exact waits).
 template <int NARR, typename V, typename FX, typename RUN, typename BODY>
 __device__ __forceinline__ void sw_wave0(SwLds<V>& sh, const V* const* src, int nc, int S, FX ent, RUN run,
                                          PcgScalars* sc, int lane, bool& give_up, BODY body) {
-  SwSet<V> s0, s1;
+  constexpr int R = SwCfg<V>::R;
+  SwSet<V> s0, s1, s2;
   sw_load<NARR>(s0, src, nc, 0, ent, lane, run, S);
   sw_load<NARR>(s1, src, nc, 1, ent, lane, run, S);
+  if (R == 3) sw_load<NARR>(s2, src, nc, 2, ent, lane, run, S);
   auto chunk = [&](int c, SwSet<V>& cur) __attribute__((always_inline)) {
     double v0 = cur.x[0], v1 = cur.x[1];
     if ((__builtin_amdgcn_ballot_w64(xunset(v0) | xunset(v1)) != 0) & !give_up)
@@ -862,18 +893,52 @@ __device__ __forceinline__ void sw_wave0(SwLds<V>& sh, const V* const* src, int 
     sh.xb[0][lane] = v0;
     sh.xb[1][lane] = v1;
     sw_spill<NARR>(sh, cur, lane);
-    sw_load<NARR>(cur, src, nc, c + 2, ent, lane, run, S);
+    sw_load<NARR>(cur, src, nc, c + R, ent, lane, run, S);
     const int slot = c % SW_K;
     if (c >= SW_K && !lds_wait(&sh.freed, c - SW_K + 1, sc)) give_up = true;
     body(c, slot);
     lds_put(&sh.done, c + 1);
   };
   int c = 0;
-  for (; c + 2 <= nc; c += 2) {
+  for (; c + R <= nc; c += R) {
     chunk(c, s0);
     chunk(c + 1, s1);
+    if (R == 3) chunk(c + 2, s2);
   }
   if (c < nc) chunk(c, s0);
+  if (R == 3 && c + 1 < nc) chunk(c + 1, s1);
+}
+
+// Wave 1: for each finished chunk c (in order) read its outbox slot into registers, release
+// the slot, then issue the chunk's stores via STORE(c, regs). Two register sets alternate so
+// that reading chunk c+1 never has to wait for chunk c's stores to have read their data.
+template <typename V>
+struct SwOut {
+  Vec16<V> d[2][SwCfg<V>::OPS];  // the chunk's one or two output arrays, 16 bytes per lane
+  V bnd[2];                      // this lane's boundary values for the two faces
+};
+template <int NOUT, typename V, typename BND, typename STORE>
+__device__ __forceinline__ void sw_wave1(SwLds<V>& sh, int nc, PcgScalars* sc, int lane, BND bnd, STORE store) {
+  SwOut<V> o0, o1;
+  auto one = [&](int c, SwOut<V>& o) __attribute__((always_inline)) {
+    lds_wait(&sh.done, c + 1, sc);
+    const int slot = c % SW_K;
+    bnd(c, &sh.ob[slot][0][0], o.bnd);
+#pragma unroll
+    for (int k = 0; k < NOUT; k++)
+#pragma unroll
+      for (int h = 0; h < SwCfg<V>::OPS; h++)
+        o.d[k][h] = *reinterpret_cast<const Vec16<V>*>(&sh.ob[slot][k][h * (1024 / (int)sizeof(V)) +
+                                                                         lane * SwCfg<V>::PER16]);
+    lds_put(&sh.freed, c + 1);
+    store(c, o);
+  };
+  int c = 0;
+  for (; c + 2 <= nc; c += 2) {
+    one(c, o0);
+    one(c + 1, o1);
+  }
+  if (c < nc) one(c, o0);
 }
 
 // Forward sweep L q = r (project-solver.cc:275-288). UPDATE: first r += (-alpha) t and
@@ -920,18 +985,23 @@ __global__ __launch_bounds__(128) void k_fwd(Lay L, T* __restrict__ r, const T* 
   if (wave == 1) {  // stores
     double* o0 = X + (size_t)B * S * 8;  // this brick's f 0 entries (read by its +x neighbour)
     double* o1 = X + face + (size_t)B * S * 8;
-    for (int c = 0; c < NC; c++) {
-      lds_wait(&sh.done, c + 1, sc);
-      const T* qc = &sh.ob[c % SW_K][0][0];
-      // boundary values first: lane e hands over slot cU + e/8, index e % 8
-      xstore(o0 + (size_t)c * U * 8 + lane, (double)qc[(lane >> 3) * 64 + 56 + (lane & 7)]);
-      xstore(o1 + (size_t)c * U * 8 + lane, (double)qc[(lane >> 3) * 64 + (lane & 7) * 8 + 7]);
+    // lane e hands over slot cU + e/8, index e % 8 of each face
+    auto bnd = [&](int, const T* qc, T* v) __attribute__((always_inline)) {
+      v[0] = qc[(lane >> 3) * 64 + 56 + (lane & 7)];
+      v[1] = qc[(lane >> 3) * 64 + (lane & 7) * 8 + 7];
+    };
+    sw_wave1<UPDATE ? 2 : 1>(sh, NC, sc, lane, bnd, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
+      xstore(o0 + (size_t)c * U * 8 + lane, (double)o.bnd[0]);  // boundary values first
+      xstore(o1 + (size_t)c * U * 8 + lane, (double)o.bnd[1]);
       if (live(0, c)) xstore(const_cast<double*>(ent(0, c)), xsent());  // consumed entries back
       if (live(1, c)) xstore(const_cast<double*>(ent(1, c)), xsent());
-      sw_store(q + brick + RunFwd{}(c, S), qc, lane);
-      if (UPDATE) sw_store(r + brick + RunFwd{}(c, S), &sh.ob[c % SW_K][1][0], lane);
-      lds_put(&sh.freed, c + 1);
-    }
+#pragma unroll
+      for (int k = 0; k < (UPDATE ? 2 : 1); k++)
+#pragma unroll
+        for (int h = 0; h < SwCfg<T>::OPS; h++)
+          *reinterpret_cast<Vec16<T>*>((k == 0 ? q : r) + brick + RunFwd{}(c, S) + h * (1024 / (int)sizeof(T)) +
+                                       lane * SwCfg<T>::PER16) = o.d[k][h];
+    });
     return;
   }
   const T* src[SW_NA] = {r + brick, cl + brick, cd + brick, cb + brick, E + brick, t + brick};
@@ -940,28 +1010,46 @@ __global__ __launch_bounds__(128) void k_fwd(Lay L, T* __restrict__ r, const T* 
   T qprev = T(0);
   double mx = 0.0, ss = 0.0;
   sw_wave0<NARR>(sh, src, NC, S, ent, RunFwd{}, sc, lane, give_up, [&](int c, int slot) __attribute__((always_inline)) {
+    // operands of UB slots at a time into registers first, so the chain of the slots waits for
+    // LDS only in its lane shuffle
+    constexpr int UB = sizeof(T) == 8 ? 4 : 8;
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int i = u * 64 + lane;
-      T rr = sh.tb[AR][i];
-      if (UPDATE) {
-        rr = rr + (T)nalpha * sh.tb[AT][i];
-        sh.ob[slot][1][i] = rr;
-        const double av = fabs((double)rr);
-        mx = (mx < av) ? av : mx;
-        ss += (double)rr * (double)rr;
+    for (int u0 = 0; u0 < U; u0 += UB) {
+      T rv[UB], tv[UB], lv[UB], dv[UB], bv[UB], ev[UB], x0[UB], x1[UB];
+#pragma unroll
+      for (int k = 0; k < UB; k++) {
+        const int u = u0 + k, i = u * 64 + lane;
+        rv[k] = sh.tb[AR][i];
+        tv[k] = UPDATE ? sh.tb[AT][i] : T(0);
+        lv[k] = sh.tb[AL][i];
+        dv[k] = sh.tb[AD][i];
+        bv[k] = sh.tb[AB][i];
+        ev[k] = sh.tb[AE][i];
+        x0[k] = (T)sh.xb[0][u * 8 + b];  // read by every lane: no branch
+        x1[k] = (T)sh.xb[1][u * 8 + a];
       }
-      const T xq0 = (T)sh.xb[0][u * 8 + b], xq1 = (T)sh.xb[1][u * 8 + a];  // read by every lane: no branch
-      T qL = __shfl_up(qprev, 8, 64);
-      T qD = dpp_row<DPP_ROW_SHR1>(qprev);
-      qL = a == 0 ? xq0 : qL;
-      qD = b == 0 ? xq1 : qD;
-      T tt = rr - sh.tb[AL][i] * qL;
-      tt = tt - sh.tb[AD][i] * qD;
-      tt = tt - sh.tb[AB][i] * qprev;
-      const T qv = tt * sh.tb[AE][i];
-      sh.ob[slot][0][i] = qv;
-      qprev = qv;
+#pragma unroll
+      for (int k = 0; k < UB; k++) {
+        const int i = (u0 + k) * 64 + lane;
+        T rr = rv[k];
+        if (UPDATE) {
+          rr = rr + (T)nalpha * tv[k];
+          sh.ob[slot][1][i] = rr;
+          const double av = fabs((double)rr);
+          mx = (mx < av) ? av : mx;
+          ss += (double)rr * (double)rr;
+        }
+        T qL = __shfl_up(qprev, 8, 64);
+        T qD = dpp_row<DPP_ROW_SHR1>(qprev);
+        qL = a == 0 ? x0[k] : qL;
+        qD = b == 0 ? x1[k] : qD;
+        T tt = rr - lv[k] * qL;
+        tt = tt - dv[k] * qD;
+        tt = tt - bv[k] * qprev;
+        const T qv = tt * ev[k];
+        sh.ob[slot][0][i] = qv;
+        qprev = qv;
+      }
     }
   });
   if (UPDATE) {
@@ -1033,18 +1121,23 @@ __global__ __launch_bounds__(128) void k_bwd(Lay L, const T* __restrict__ qv, co
   if (wave == 1) {  // stores
     double* o0 = X + (size_t)B * S * 8;  // this brick's f 0 entries (read by its -x neighbour)
     double* o1 = X + face + (size_t)B * S * 8;
-    for (int c = 0; c < NC; c++) {
-      lds_wait(&sh.done, c + 1, sc);
-      const T* zc = &sh.ob[c % SW_K][0][0];
-      // boundary values first: lane e hands over slot S-1-cU-e/8, index e % 8
-      const int so = slot_of(c), e0 = (U - 1 - (lane >> 3)) * 64;
-      xstore(o0 + (size_t)so * 8 + ew, (double)zc[e0 + ew]);
-      xstore(o1 + (size_t)so * 8 + ew, (double)zc[e0 + ew * 8]);
+    // lane e hands over slot S-1-cU-e/8, index e % 8 of each face
+    const int e0 = (U - 1 - (lane >> 3)) * 64;
+    auto bnd = [&](int, const T* zc, T* v) __attribute__((always_inline)) {
+      v[0] = zc[e0 + ew];
+      v[1] = zc[e0 + ew * 8];
+    };
+    sw_wave1<1>(sh, NC, sc, lane, bnd, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
+      const int so = slot_of(c);
+      xstore(o0 + (size_t)so * 8 + ew, (double)o.bnd[0]);  // boundary values first
+      xstore(o1 + (size_t)so * 8 + ew, (double)o.bnd[1]);
       if (live(0, c)) xstore(const_cast<double*>(ent(0, c)), xsent());
       if (live(1, c)) xstore(const_cast<double*>(ent(1, c)), xsent());
-      sw_store(z + brick + RunBwd{}(c, S), zc, lane);
-      lds_put(&sh.freed, c + 1);
-    }
+#pragma unroll
+      for (int h = 0; h < SwCfg<T>::OPS; h++)
+        *reinterpret_cast<Vec16<T>*>(z + brick + RunBwd{}(c, S) + h * (1024 / (int)sizeof(T)) +
+                                     lane * SwCfg<T>::PER16) = o.d[0][h];
+    });
     return;
   }
   const T* src[SW_NA] = {qv + brick, dr + brick, du + brick, df + brick, E + brick, r + brick};
@@ -1052,21 +1145,37 @@ __global__ __launch_bounds__(128) void k_bwd(Lay L, const T* __restrict__ qv, co
   T zprev = T(0);
   double acc = 0.0;
   sw_wave0<SW_NA>(sh, src, NC, S, ent, RunBwd{}, sc, lane, give_up, [&](int c, int slot) __attribute__((always_inline)) {
+    constexpr int UB = sizeof(T) == 8 ? 4 : 8;  // operands of UB slots into registers first
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int i = (U - 1 - u) * 64 + lane;
-      const T xz0 = (T)sh.xb[0][u * 8 + b], xz1 = (T)sh.xb[1][u * 8 + a];  // read by every lane: no branch
-      T zR = __shfl_down(zprev, 8, 64);
-      T zU = dpp_row<DPP_ROW_SHL1>(zprev);
-      zR = a == 7 ? xz0 : zR;
-      zU = b == 7 ? xz1 : zU;
-      T tt = sh.tb[AQ][i] - sh.tb[A1][i] * zR;
-      tt = tt - sh.tb[A2][i] * zU;
-      tt = tt - sh.tb[A3][i] * zprev;
-      const T zv = tt * sh.tb[AE][i];
-      sh.ob[slot][0][i] = zv;
-      zprev = zv;
-      acc += (double)zv * (double)sh.tb[AR][i];
+    for (int u0 = 0; u0 < U; u0 += UB) {
+      T qv[UB], v1[UB], v2[UB], v3[UB], ev[UB], rv[UB], x0[UB], x1[UB];
+#pragma unroll
+      for (int k = 0; k < UB; k++) {
+        const int u = u0 + k, i = (U - 1 - u) * 64 + lane;
+        qv[k] = sh.tb[AQ][i];
+        v1[k] = sh.tb[A1][i];
+        v2[k] = sh.tb[A2][i];
+        v3[k] = sh.tb[A3][i];
+        ev[k] = sh.tb[AE][i];
+        rv[k] = sh.tb[AR][i];
+        x0[k] = (T)sh.xb[0][u * 8 + b];  // read by every lane: no branch
+        x1[k] = (T)sh.xb[1][u * 8 + a];
+      }
+#pragma unroll
+      for (int k = 0; k < UB; k++) {
+        const int i = (U - 1 - (u0 + k)) * 64 + lane;
+        T zR = __shfl_down(zprev, 8, 64);
+        T zU = dpp_row<DPP_ROW_SHL1>(zprev);
+        zR = a == 7 ? x0[k] : zR;
+        zU = b == 7 ? x1[k] : zU;
+        T tt = qv[k] - v1[k] * zR;
+        tt = tt - v2[k] * zU;
+        tt = tt - v3[k] * zprev;
+        const T zv = tt * ev[k];
+        sh.ob[slot][0][i] = zv;
+        zprev = zv;
+        acc += (double)zv * (double)rv[k];
+      }
     }
   });
   double ws = wave_sum(acc);
