@@ -690,28 +690,30 @@ __global__ __launch_bounds__(256) void k_resid(Lay L, const double* __restrict__
 }
 
 // ---- MIC sweeps ---------------------------------------------------------------------------------
-// One workgroup of two waves per live brick. Lane (a, b) = (lane >> 3, lane & 7) owns column
+// One workgroup of three waves per live brick. Lane (a, b) = (lane >> 3, lane & 7) owns column
 // (8I+a, 8J+b) and at slot s handles cell k = s - a - b, so its -x and -y neighbours at the same
 // k were finished one slot earlier by lanes a-1 and b-1 (lane shuffles) or, on the brick face,
 // 7 slots earlier by the neighbouring brick (exchange entries X[f][B][s][w]).
 //
 // Chunk c (U slots) of a brick array is one contiguous run of U*64 elements, and the entries a
 // chunk consumes from one face are 64 doubles, one per lane.
-//   Wave 0 (loads and arithmetic) keeps SW_R<T> chunks of loads in flight: 16 bytes per lane
-//   per instruction for the arrays (moved to the lanes that own them through LDS) and one 8-byte
-//   load per lane per face for the entries. It issues no global store, so the waits the
-//   compiler places before using loaded values count loads only, which complete in issue order,
-//   and stay exact (with stores in flight the compiler has to wait for everything).
+//   Wave 0 (loads and arithmetic) keeps SwCfg::R chunks of loads in flight, 16 bytes per lane
+//   per instruction (moved to the lanes that own them through LDS). It issues no other global
+//   access, so the waits the compiler places before using loaded values count these loads only,
+//   which complete in issue order, and stay exact: with stores or polls in flight the compiler
+//   has to wait for everything, which would drain the pipeline once per chunk.
 //   Wave 1 (stores) takes each finished chunk from an LDS outbox: it hands the boundary values
 //   to the next bricks (one 8-byte store per lane and face), returns the consumed entries to the
 //   sentinel and writes the chunk's outputs with 16-byte stores.
-// The waves meet at two LDS counters: chunks finished by wave 0 and outbox slots released by
-// wave 1. Every wait is bounded by SPIN_LIMIT, after which the wave gives up, counts a timeout
-// and the host reports FS_DEVICE_ERROR.
+//   Wave 2 (exchange) reads each chunk's entries ahead of wave 0, polling until the neighbours
+//   have written them; a sweep's chunks mostly wait on their neighbours, and this keeps the wait
+//   out of wave 0's load pipeline.
+// The waves meet at LDS counters: chunks finished by wave 0, outbox slots released by wave 1,
+// chunks whose entries wave 2 has read. Every wait is bounded by SPIN_LIMIT, after which the
+// wave gives up, counts a timeout and the host reports FS_DEVICE_ERROR.
 //
 // Exchange entries hold a sentinel until the producer stores the value (agent-scope atomic
-// stores and loads); the consumer uses the value and stores the sentinel back. An entry still
-// unwritten when its chunk starts is polled on a slow path, after which nothing is in flight.
+// stores and loads); the consumer uses the value and stores the sentinel back.
 
 // High word of the sentinel. Arithmetic never returns a signalling NaN, so for computed values
 // the high word alone identifies the sentinel.
@@ -736,6 +738,7 @@ constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHL1 = 0x101;
 constexpr int SW_NA = 6;       // input arrays per chunk
 constexpr int SW_CE = U * 64;  // elements per chunk and array
 constexpr int SW_K = 2;        // outbox slots
+constexpr int SW_KX = 4;       // exchange-entry slots (wave 2 reads up to this many chunks ahead)
 typedef float sw_f4 __attribute__((ext_vector_type(4)));
 typedef double sw_d2 __attribute__((ext_vector_type(2)));
 template <typename V>
@@ -751,7 +754,6 @@ struct SwCfg {
 template <typename V>
 struct SwSet {
   Vec16<V> d[SW_NA][SwCfg<V>::OPS];
-  double x[2];
 };
 
 // Shared state of a sweep workgroup.
@@ -759,8 +761,8 @@ template <typename V>
 struct SwLds {
   __align__(16) V tb[SW_NA][SW_CE];       // the chunk being computed, one element per lane and slot
   __align__(16) V ob[SW_K][2][SW_CE];     // outbox: outputs of finished chunks
-  double xb[2][64];                       // the chunk's exchange entries
-  int done, freed, B, stop;               // chunks finished by wave 0 / released by wave 1
+  double xr[SW_KX][2][64];                // exchange entries of chunks c % SW_KX, from wave 2
+  int done, freed, xready, B, stop;       // chunks finished by wave 0 / released by wave 1 / read by wave 2
 };
 
 __device__ __forceinline__ int lds_get(const int* p) {
@@ -806,26 +808,6 @@ __device__ inline bool last_wave(uint32_t* ctr, uint32_t nwaves) {
   return is_last;
 }
 
-// Slow path: poll this lane's two entries until both are written (or the wave gives up).
-// Ends with nothing in flight.
-__device__ __forceinline__ void sw_poll(const double* p0, const double* p1, double& v0, double& v1, bool& give_up,
-                                        PcgScalars* sc) {
-  long spins = 0;
-  while (((__builtin_amdgcn_ballot_w64(xunset(v0) | xunset(v1)) != 0) & !give_up)) {
-    __builtin_amdgcn_s_sleep(1);
-    v0 = xload(p0);
-    v1 = xload(p1);
-    if ((++spins & 255) == 0) {
-      const int to = __hip_atomic_load(&sc->wait_timeouts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (to != 0 || spins > SPIN_LIMIT) {
-        if ((threadIdx.x & 63) == 0) atomicAdd(&sc->wait_timeouts, 1);
-        give_up = true;
-      }
-    }
-  }
-  __builtin_amdgcn_s_waitcnt(0);
-}
-
 // Element offset of chunk c in a brick array: forward chunks run up from slot 0, backward
 // chunks down from slot S-1 (chunk c then starts at slot S - (c+1)U).
 struct RunFwd {
@@ -836,9 +818,8 @@ struct RunBwd {
 };
 
 // Wave 0: issue the loads of chunk c (past the end: the last chunk again, unused).
-template <int NARR, typename V, typename FX, typename RUN>
-__device__ __forceinline__ void sw_load(SwSet<V>& st, const V* const* src, int nc, int c, FX ent, int lane, RUN run,
-                                        int S) {
+template <int NARR, typename V, typename RUN>
+__device__ __forceinline__ void sw_load(SwSet<V>& st, const V* const* src, int nc, int c, int lane, RUN run, int S) {
   const int cs = min(c, nc - 1);
   const size_t o = run(cs, S);
 #pragma unroll
@@ -848,8 +829,6 @@ __device__ __forceinline__ void sw_load(SwSet<V>& st, const V* const* src, int n
       st.d[k][h] = *reinterpret_cast<const Vec16<V>*>(src[k] + o + h * (1024 / (int)sizeof(V)) +
                                                      lane * SwCfg<V>::PER16);
   }
-  st.x[0] = xload(ent(0, c));
-  st.x[1] = xload(ent(1, c));
 }
 
 // Wave 0: move a chunk's array loads to LDS, element e of array k at tb[k][e].
@@ -874,29 +853,25 @@ __device__ __forceinline__ void sw_store(V* dst, const V* lds, int lane) {
 }
 
 
-// Wave 0's loop over the chunks of one sweep. BODY(c, slot) computes chunk c from sh.tb / sh.xb
-// into sh.ob[slot]; the loads of the next SwCfg::R - 1 chunks are in flight meanwhile (register
-// sets named rather than indexed so that all stay in registers). Whole groups of chunks run as
-// straight-line code (an exit inside would merge paths and cost exact waits).
-template <int NARR, typename V, typename FX, typename RUN, typename BODY>
-__device__ __forceinline__ void sw_wave0(SwLds<V>& sh, const V* const* src, int nc, int S, FX ent, RUN run,
-                                         PcgScalars* sc, int lane, bool& give_up, BODY body) {
+// Wave 0's loop over the chunks of one sweep. BODY(c, slot, xr) computes chunk c from sh.tb and
+// the chunk's exchange entries xr (from wave 2) into sh.ob[slot]; the loads of the next
+// SwCfg::R - 1 chunks are in flight meanwhile (register sets named rather than indexed so that
+// all stay in registers). Wave 0 issues nothing but these loads, so no wait ever drains them.
+template <int NARR, typename V, typename RUN, typename BODY>
+__device__ __forceinline__ void sw_wave0(SwLds<V>& sh, const V* const* src, int nc, int S, RUN run, PcgScalars* sc,
+                                         int lane, BODY body) {
   constexpr int R = SwCfg<V>::R;
   SwSet<V> s0, s1, s2;
-  sw_load<NARR>(s0, src, nc, 0, ent, lane, run, S);
-  sw_load<NARR>(s1, src, nc, 1, ent, lane, run, S);
-  if (R == 3) sw_load<NARR>(s2, src, nc, 2, ent, lane, run, S);
+  sw_load<NARR>(s0, src, nc, 0, lane, run, S);
+  sw_load<NARR>(s1, src, nc, 1, lane, run, S);
+  if (R == 3) sw_load<NARR>(s2, src, nc, 2, lane, run, S);
   auto chunk = [&](int c, SwSet<V>& cur) __attribute__((always_inline)) {
-    double v0 = cur.x[0], v1 = cur.x[1];
-    if ((__builtin_amdgcn_ballot_w64(xunset(v0) | xunset(v1)) != 0) & !give_up)
-      sw_poll(ent(0, c), ent(1, c), v0, v1, give_up, sc);
-    sh.xb[0][lane] = v0;
-    sh.xb[1][lane] = v1;
     sw_spill<NARR>(sh, cur, lane);
-    sw_load<NARR>(cur, src, nc, c + R, ent, lane, run, S);
+    sw_load<NARR>(cur, src, nc, c + R, lane, run, S);
     const int slot = c % SW_K;
-    if (c >= SW_K && !lds_wait(&sh.freed, c - SW_K + 1, sc)) give_up = true;
-    body(c, slot);
+    if (c >= SW_K) lds_wait(&sh.freed, c - SW_K + 1, sc);
+    lds_wait(&sh.xready, c + 1, sc);
+    body(c, slot, &sh.xr[c % SW_KX][0][0]);
     lds_put(&sh.done, c + 1);
   };
   int c = 0;
@@ -907,6 +882,36 @@ __device__ __forceinline__ void sw_wave0(SwLds<V>& sh, const V* const* src, int 
   }
   if (c < nc) chunk(c, s0);
   if (R == 3 && c + 1 < nc) chunk(c + 1, s1);
+}
+
+// Wave 2: read the exchange entries of each chunk (lane e: entry e of both faces), polling until
+// the producers have written them (bounded by SPIN_LIMIT), into LDS slot c % SW_KX, up to SW_KX
+// chunks ahead of wave 0. Wave 2 issues loads only.
+template <typename V, typename FX>
+__device__ __forceinline__ void sw_wave2(SwLds<V>& sh, int nc, FX ent, PcgScalars* sc, int lane) {
+  bool give_up = false;
+  for (int c = 0; c < nc; c++) {
+    if (c >= SW_KX && !lds_wait(&sh.done, c - SW_KX + 1, sc)) give_up = true;
+    const double* p0 = ent(0, c);
+    const double* p1 = ent(1, c);
+    double v0 = xload(p0), v1 = xload(p1);
+    long spins = 0;
+    while (((__builtin_amdgcn_ballot_w64(xunset(v0) | xunset(v1)) != 0) & !give_up)) {
+      __builtin_amdgcn_s_sleep(1);
+      if (xunset(v0)) v0 = xload(p0);
+      if (xunset(v1)) v1 = xload(p1);
+      if ((++spins & 255) == 0) {
+        const int to = __hip_atomic_load(&sc->wait_timeouts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (to != 0 || spins > SPIN_LIMIT) {
+          if (lane == 0) atomicAdd(&sc->wait_timeouts, 1);
+          give_up = true;
+        }
+      }
+    }
+    sh.xr[c % SW_KX][0][lane] = v0;
+    sh.xr[c % SW_KX][1][lane] = v1;
+    lds_put(&sh.xready, c + 1);
+  }
 }
 
 // Wave 1: for each finished chunk c (in order) read its outbox slot into registers, release
@@ -945,7 +950,7 @@ __device__ __forceinline__ void sw_wave1(SwLds<V>& sh, int nc, PcgScalars* sc, i
 // ||r||_inf (saxpy + LinfNorm, :333-335); the last brick decides convergence.
 // T: storage and arithmetic type of vectors and coefficients; the exchange entries stay FP64.
 template <bool UPDATE, typename T>
-__global__ __launch_bounds__(128) void k_fwd(Lay L, T* __restrict__ r, const T* __restrict__ t,
+__global__ __launch_bounds__(192) void k_fwd(Lay L, T* __restrict__ r, const T* __restrict__ t,
                                              const T* __restrict__ cl, const T* __restrict__ cd,
                                              const T* __restrict__ cb, const T* __restrict__ E,
                                              T* __restrict__ q, double* X, double* __restrict__ part2,
@@ -964,6 +969,7 @@ __global__ __launch_bounds__(128) void k_fwd(Lay L, T* __restrict__ r, const T* 
     sh.stop = UPDATE && sc->converged != STOP_RUN;
     sh.done = 0;
     sh.freed = 0;
+    sh.xready = 0;
   }
   __syncthreads();
   if (sh.stop) return;
@@ -1004,12 +1010,15 @@ __global__ __launch_bounds__(128) void k_fwd(Lay L, T* __restrict__ r, const T* 
     });
     return;
   }
+  if (wave == 2) {  // exchange entries
+    sw_wave2(sh, NC, ent, sc, lane);
+    return;
+  }
   const T* src[SW_NA] = {r + brick, cl + brick, cd + brick, cb + brick, E + brick, t + brick};
   const double nalpha = UPDATE ? -sc->alpha : 0.0;
-  bool give_up = false;
   T qprev = T(0);
   double mx = 0.0, ss = 0.0;
-  sw_wave0<NARR>(sh, src, NC, S, ent, RunFwd{}, sc, lane, give_up, [&](int c, int slot) __attribute__((always_inline)) {
+  sw_wave0<NARR>(sh, src, NC, S, RunFwd{}, sc, lane, [&](int c, int slot, const double* xr) __attribute__((always_inline)) {
     // operands of UB slots at a time into registers first, so the chain of the slots waits for
     // LDS only in its lane shuffle
     constexpr int UB = sizeof(T) == 8 ? 4 : 8;
@@ -1025,8 +1034,8 @@ __global__ __launch_bounds__(128) void k_fwd(Lay L, T* __restrict__ r, const T* 
         dv[k] = sh.tb[AD][i];
         bv[k] = sh.tb[AB][i];
         ev[k] = sh.tb[AE][i];
-        x0[k] = (T)sh.xb[0][u * 8 + b];  // read by every lane: no branch
-        x1[k] = (T)sh.xb[1][u * 8 + a];
+        x0[k] = (T)xr[u * 8 + b];  // read by every lane: no branch
+        x1[k] = (T)xr[64 + u * 8 + a];
       }
 #pragma unroll
       for (int k = 0; k < UB; k++) {
@@ -1084,7 +1093,7 @@ __global__ __launch_bounds__(128) void k_fwd(Lay L, T* __restrict__ r, const T* 
 // covers slots S-1-cU down to S-U-cU, and its u-th slot, S-1-cU-u, is element (U-1-u)*64 + lane
 // of the chunk's run.
 template <bool INIT, typename T>
-__global__ __launch_bounds__(128) void k_bwd(Lay L, const T* __restrict__ qv, const T* __restrict__ dr,
+__global__ __launch_bounds__(192) void k_bwd(Lay L, const T* __restrict__ qv, const T* __restrict__ dr,
                                              const T* __restrict__ du, const T* __restrict__ df,
                                              const T* __restrict__ E, const T* __restrict__ r,
                                              T* __restrict__ z, double* X, double* __restrict__ part,
@@ -1099,6 +1108,7 @@ __global__ __launch_bounds__(128) void k_bwd(Lay L, const T* __restrict__ qv, co
     sh.stop = !INIT && sc->converged != STOP_RUN;
     sh.done = 0;
     sh.freed = 0;
+    sh.xready = 0;
   }
   __syncthreads();
   if (sh.stop) return;
@@ -1140,11 +1150,14 @@ __global__ __launch_bounds__(128) void k_bwd(Lay L, const T* __restrict__ qv, co
     });
     return;
   }
+  if (wave == 2) {  // exchange entries
+    sw_wave2(sh, NC, ent, sc, lane);
+    return;
+  }
   const T* src[SW_NA] = {qv + brick, dr + brick, du + brick, df + brick, E + brick, r + brick};
-  bool give_up = false;
   T zprev = T(0);
   double acc = 0.0;
-  sw_wave0<SW_NA>(sh, src, NC, S, ent, RunBwd{}, sc, lane, give_up, [&](int c, int slot) __attribute__((always_inline)) {
+  sw_wave0<SW_NA>(sh, src, NC, S, RunBwd{}, sc, lane, [&](int c, int slot, const double* xr) __attribute__((always_inline)) {
     constexpr int UB = sizeof(T) == 8 ? 4 : 8;  // operands of UB slots into registers first
 #pragma unroll
     for (int u0 = 0; u0 < U; u0 += UB) {
@@ -1158,8 +1171,8 @@ __global__ __launch_bounds__(128) void k_bwd(Lay L, const T* __restrict__ qv, co
         v3[k] = sh.tb[A3][i];
         ev[k] = sh.tb[AE][i];
         rv[k] = sh.tb[AR][i];
-        x0[k] = (T)sh.xb[0][u * 8 + b];  // read by every lane: no branch
-        x1[k] = (T)sh.xb[1][u * 8 + a];
+        x0[k] = (T)xr[u * 8 + b];  // read by every lane: no branch
+        x1[k] = (T)xr[64 + u * 8 + a];
       }
 #pragma unroll
       for (int k = 0; k < UB; k++) {
@@ -1433,7 +1446,7 @@ int launch_fwd(fs_ctx* c, bool update, int iter) {
   Sys<S> y = sys_of<S>(P);
   uint64_t* tk = P.tickets.as<uint64_t>();
   auto go = [&](auto kern) {
-    kern<<<P.nlive, 128, 0, c->stream>>>(L, y.r, y.t, y.cl, y.cd, y.cb, y.e, y.q, P.xf.as<double>(),
+    kern<<<P.nlive, 192, 0, c->stream>>>(L, y.r, y.t, y.cl, y.cd, y.cb, y.e, y.q, P.xf.as<double>(),
                                         P.partials2.as<double>(), P.partials.as<double>(), tk + SW_FWD,
                                         P.ticket_base[SW_FWD], P.fwd_order.as<int>(), P.scal.as<PcgScalars>(), iter,
                                         c->cfg.tol_mode, c->cfg.cg_tolerance);
@@ -1455,7 +1468,7 @@ int launch_bwd(fs_ctx* c, bool init) {
   Sys<S> y = sys_of<S>(P);
   uint64_t* tk = P.tickets.as<uint64_t>();
   auto go = [&](auto kern) {
-    kern<<<P.nlive, 128, 0, c->stream>>>(L, y.q, y.dr, y.du, y.df, y.e, y.r, y.z, P.xb.as<double>(),
+    kern<<<P.nlive, 192, 0, c->stream>>>(L, y.q, y.dr, y.du, y.df, y.e, y.r, y.z, P.xb.as<double>(),
                                         P.partials.as<double>(), tk + SW_BWD, P.ticket_base[SW_BWD],
                                         P.bwd_order.as<int>(), P.scal.as<PcgScalars>());
   };
