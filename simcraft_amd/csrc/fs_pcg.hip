@@ -690,27 +690,27 @@ __global__ __launch_bounds__(256) void k_resid(Lay L, const double* __restrict__
 }
 
 // ---- MIC sweeps ---------------------------------------------------------------------------------
-// One workgroup of three waves per live brick. Lane (a, b) = (lane >> 3, lane & 7) owns column
+// One workgroup of four waves per live brick. Lane (a, b) = (lane >> 3, lane & 7) owns column
 // (8I+a, 8J+b) and at slot s handles cell k = s - a - b, so its -x and -y neighbours at the same
 // k were finished one slot earlier by lanes a-1 and b-1 (lane shuffles) or, on the brick face,
 // 7 slots earlier by the neighbouring brick (exchange entries X[f][B][s][w]).
 //
 // Chunk c (U slots) of a brick array is one contiguous run of U*64 elements, and the entries a
-// chunk consumes from one face are 64 doubles, one per lane.
-//   Wave 0 (loads and arithmetic) keeps SwCfg::R chunks of loads in flight, 16 bytes per lane
-//   per instruction (moved to the lanes that own them through LDS). It issues no other global
-//   access, so the waits the compiler places before using loaded values count these loads only,
-//   which complete in issue order, and stay exact: with stores or polls in flight the compiler
-//   has to wait for everything, which would drain the pipeline once per chunk.
-//   Wave 1 (stores) takes each finished chunk from an LDS outbox: it hands the boundary values
-//   to the next bricks (one 8-byte store per lane and face), returns the consumed entries to the
-//   sentinel and writes the chunk's outputs with 16-byte stores.
-//   Wave 2 (exchange) reads each chunk's entries ahead of wave 0, polling until the neighbours
-//   have written them; a sweep's chunks mostly wait on their neighbours, and this keeps the wait
-//   out of wave 0's load pipeline.
-// The waves meet at LDS counters: chunks finished by wave 0, outbox slots released by wave 1,
-// chunks whose entries wave 2 has read. Every wait is bounded by SPIN_LIMIT, after which the
-// wave gives up, counts a timeout and the host reports FS_DEVICE_ERROR.
+// chunk consumes from one face are 64 doubles, one per lane. The dependent chain of a sweep is
+// short arithmetic per slot; everything that waits on memory is kept off it:
+//   Wave 3 (array loads) keeps SwCfg::R chunks of 16-byte-per-lane loads in flight and moves
+//   each loaded chunk to an LDS ring (to the lanes that own the elements).
+//   Wave 2 (exchange) reads each chunk's entries ahead, polling until the neighbours have
+//   written them.
+//   Wave 0 (arithmetic) runs the chain from LDS and writes each finished chunk to an outbox.
+//   Wave 1 (stores) hands the outbox's boundary values to the next bricks (one 8-byte store per
+//   lane and face), returns the consumed entries to the sentinel and writes the outputs with
+//   16-byte stores.
+// Waves 3 and 2 issue loads only: loads complete in issue order, so the waits the compiler
+// places before using loaded values stay exact (with stores in flight it has to wait for
+// everything, which would drain the pipeline). The waves meet at LDS counters; every wait is
+// bounded by SPIN_LIMIT, after which the wave gives up, counts a timeout and the host reports
+// FS_DEVICE_ERROR.
 //
 // Exchange entries hold a sentinel until the producer stores the value (agent-scope atomic
 // stores and loads); the consumer uses the value and stores the sentinel back.
@@ -745,7 +745,8 @@ template <typename V>
 using Vec16 = typename std::conditional<sizeof(V) == 8, sw_d2, sw_f4>::type;  // 16 bytes, a register tuple
 template <typename V>
 struct SwCfg {
-  static constexpr int R = 2;                                // chunks of loads in flight (register sets)
+  static constexpr int R = sizeof(V) == 8 ? 2 : 4;           // chunks of loads in flight (loader register sets)
+  static constexpr int RD = sizeof(V) == 8 ? 2 : 4;          // chunks loaded into LDS ahead of the compute wave
   static constexpr int OPS = SW_CE * (int)sizeof(V) / 1024;  // 16-byte loads per lane, array and chunk
   static constexpr int PER16 = 16 / (int)sizeof(V);          // elements per 16-byte access
 };
@@ -759,10 +760,11 @@ struct SwSet {
 // Shared state of a sweep workgroup.
 template <typename V>
 struct SwLds {
-  __align__(16) V tb[SW_NA][SW_CE];       // the chunk being computed, one element per lane and slot
+  __align__(16) V tb[SwCfg<V>::RD][SW_NA][SW_CE];  // chunks c % RD, one element per lane and slot
   __align__(16) V ob[SW_K][2][SW_CE];     // outbox: outputs of finished chunks
   double xr[SW_KX][2][64];                // exchange entries of chunks c % SW_KX, from wave 2
-  int done, freed, xready, B, stop;       // chunks finished by wave 0 / released by wave 1 / read by wave 2
+  int done, freed, xready, loaded, B, stop;  // chunks finished by wave 0 / released by wave 1 /
+                                             // entries read by wave 2 / arrays loaded by wave 3
 };
 
 __device__ __forceinline__ int lds_get(const int* p) {
@@ -817,7 +819,7 @@ struct RunBwd {
   __device__ size_t operator()(int c, int S) const { return (size_t)(S - (c + 1) * U) * 64; }
 };
 
-// Wave 0: issue the loads of chunk c (past the end: the last chunk again, unused).
+// Wave 3: issue the loads of chunk c (past the end: the last chunk again, unused).
 template <int NARR, typename V, typename RUN>
 __device__ __forceinline__ void sw_load(SwSet<V>& st, const V* const* src, int nc, int c, int lane, RUN run, int S) {
   const int cs = min(c, nc - 1);
@@ -831,14 +833,14 @@ __device__ __forceinline__ void sw_load(SwSet<V>& st, const V* const* src, int n
   }
 }
 
-// Wave 0: move a chunk's array loads to LDS, element e of array k at tb[k][e].
+// Wave 3: move a chunk's array loads to an LDS ring slot, element e of array k at tb[k][e].
 template <int NARR, typename V>
-__device__ __forceinline__ void sw_spill(SwLds<V>& sh, const SwSet<V>& st, int lane) {
+__device__ __forceinline__ void sw_spill(V (*tb)[SW_CE], const SwSet<V>& st, int lane) {
 #pragma unroll
   for (int k = 0; k < NARR; k++) {
 #pragma unroll
     for (int h = 0; h < SwCfg<V>::OPS; h++)
-      *reinterpret_cast<Vec16<V>*>(&sh.tb[k][h * (1024 / (int)sizeof(V)) + lane * SwCfg<V>::PER16]) = st.d[k][h];
+      *reinterpret_cast<Vec16<V>*>(&tb[k][h * (1024 / (int)sizeof(V)) + lane * SwCfg<V>::PER16]) = st.d[k][h];
   }
 }
 
@@ -853,35 +855,51 @@ __device__ __forceinline__ void sw_store(V* dst, const V* lds, int lane) {
 }
 
 
-// Wave 0's loop over the chunks of one sweep. BODY(c, slot, xr) computes chunk c from sh.tb and
-// the chunk's exchange entries xr (from wave 2) into sh.ob[slot]; the loads of the next
-// SwCfg::R - 1 chunks are in flight meanwhile (register sets named rather than indexed so that
-// all stay in registers). Wave 0 issues nothing but these loads, so no wait ever drains them.
-template <int NARR, typename V, typename RUN, typename BODY>
-__device__ __forceinline__ void sw_wave0(SwLds<V>& sh, const V* const* src, int nc, int S, RUN run, PcgScalars* sc,
-                                         int lane, BODY body) {
-  constexpr int R = SwCfg<V>::R;
-  SwSet<V> s0, s1, s2;
+// Wave 3 (loads): keeps SwCfg::R chunks of array loads in flight (register sets named rather
+// than indexed so that all stay in registers) and moves each loaded chunk to LDS ring slot
+// c % RD once wave 0 is done with the chunk that used it. Wave 3 issues loads only, so the waits
+// before the moves are exact. Whole groups of R chunks run as straight-line code (an exit
+// inside would merge paths and cost exact waits).
+template <int NARR, typename V, typename RUN>
+__device__ __forceinline__ void sw_wave3(SwLds<V>& sh, const V* const* src, int nc, int S, RUN run, PcgScalars* sc,
+                                         int lane) {
+  constexpr int R = SwCfg<V>::R, RD = SwCfg<V>::RD;
+  SwSet<V> s0, s1, s2, s3;
   sw_load<NARR>(s0, src, nc, 0, lane, run, S);
   sw_load<NARR>(s1, src, nc, 1, lane, run, S);
-  if (R == 3) sw_load<NARR>(s2, src, nc, 2, lane, run, S);
+  if (R >= 3) sw_load<NARR>(s2, src, nc, 2, lane, run, S);
+  if (R >= 4) sw_load<NARR>(s3, src, nc, 3, lane, run, S);
   auto chunk = [&](int c, SwSet<V>& cur) __attribute__((always_inline)) {
-    sw_spill<NARR>(sh, cur, lane);
+    if (c >= RD) lds_wait(&sh.done, c - RD + 1, sc);
+    sw_spill<NARR>(sh.tb[c % RD], cur, lane);
+    lds_put(&sh.loaded, c + 1);
     sw_load<NARR>(cur, src, nc, c + R, lane, run, S);
-    const int slot = c % SW_K;
-    if (c >= SW_K) lds_wait(&sh.freed, c - SW_K + 1, sc);
-    lds_wait(&sh.xready, c + 1, sc);
-    body(c, slot, &sh.xr[c % SW_KX][0][0]);
-    lds_put(&sh.done, c + 1);
   };
   int c = 0;
   for (; c + R <= nc; c += R) {
     chunk(c, s0);
     chunk(c + 1, s1);
-    if (R == 3) chunk(c + 2, s2);
+    if (R >= 3) chunk(c + 2, s2);
+    if (R >= 4) chunk(c + 3, s3);
   }
   if (c < nc) chunk(c, s0);
-  if (R == 3 && c + 1 < nc) chunk(c + 1, s1);
+  if (R >= 3 && c + 1 < nc) chunk(c + 1, s1);
+  if (R >= 4 && c + 2 < nc) chunk(c + 2, s2);
+}
+
+// Wave 0 (arithmetic): for each chunk, once wave 3 has loaded it, wave 2 has read its exchange
+// entries and wave 1 has released the outbox slot, BODY(c, slot, tb, xr) computes it from LDS
+// ring slot tb and entries xr into sh.ob[slot]. Wave 0 issues no global memory access.
+template <typename V, typename BODY>
+__device__ __forceinline__ void sw_wave0(SwLds<V>& sh, int nc, PcgScalars* sc, BODY body) {
+  for (int c = 0; c < nc; c++) {
+    const int slot = c % SW_K;
+    lds_wait(&sh.loaded, c + 1, sc);
+    lds_wait(&sh.xready, c + 1, sc);
+    if (c >= SW_K) lds_wait(&sh.freed, c - SW_K + 1, sc);
+    body(c, slot, sh.tb[c % SwCfg<V>::RD], &sh.xr[c % SW_KX][0][0]);
+    lds_put(&sh.done, c + 1);
+  }
 }
 
 // Wave 2: read the exchange entries of each chunk (lane e: entry e of both faces), polling until
@@ -950,7 +968,7 @@ __device__ __forceinline__ void sw_wave1(SwLds<V>& sh, int nc, PcgScalars* sc, i
 // ||r||_inf (saxpy + LinfNorm, :333-335); the last brick decides convergence.
 // T: storage and arithmetic type of vectors and coefficients; the exchange entries stay FP64.
 template <bool UPDATE, typename T>
-__global__ __launch_bounds__(192) void k_fwd(Lay L, T* __restrict__ r, const T* __restrict__ t,
+__global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* __restrict__ t,
                                              const T* __restrict__ cl, const T* __restrict__ cd,
                                              const T* __restrict__ cb, const T* __restrict__ E,
                                              T* __restrict__ q, double* X, double* __restrict__ part2,
@@ -970,6 +988,7 @@ __global__ __launch_bounds__(192) void k_fwd(Lay L, T* __restrict__ r, const T* 
     sh.done = 0;
     sh.freed = 0;
     sh.xready = 0;
+    sh.loaded = 0;
   }
   __syncthreads();
   if (sh.stop) return;
@@ -1014,11 +1033,15 @@ __global__ __launch_bounds__(192) void k_fwd(Lay L, T* __restrict__ r, const T* 
     sw_wave2(sh, NC, ent, sc, lane);
     return;
   }
-  const T* src[SW_NA] = {r + brick, cl + brick, cd + brick, cb + brick, E + brick, t + brick};
+  if (wave == 3) {  // array loads
+    const T* src[SW_NA] = {r + brick, cl + brick, cd + brick, cb + brick, E + brick, t + brick};
+    sw_wave3<NARR>(sh, src, NC, S, RunFwd{}, sc, lane);
+    return;
+  }
   const double nalpha = UPDATE ? -sc->alpha : 0.0;
   T qprev = T(0);
   double mx = 0.0, ss = 0.0;
-  sw_wave0<NARR>(sh, src, NC, S, RunFwd{}, sc, lane, [&](int c, int slot, const double* xr) __attribute__((always_inline)) {
+  sw_wave0(sh, NC, sc, [&](int c, int slot, const T (*tb)[SW_CE], const double* xr) __attribute__((always_inline)) {
     // operands of UB slots at a time into registers first, so the chain of the slots waits for
     // LDS only in its lane shuffle
     constexpr int UB = sizeof(T) == 8 ? 4 : 8;
@@ -1028,12 +1051,12 @@ __global__ __launch_bounds__(192) void k_fwd(Lay L, T* __restrict__ r, const T* 
 #pragma unroll
       for (int k = 0; k < UB; k++) {
         const int u = u0 + k, i = u * 64 + lane;
-        rv[k] = sh.tb[AR][i];
-        tv[k] = UPDATE ? sh.tb[AT][i] : T(0);
-        lv[k] = sh.tb[AL][i];
-        dv[k] = sh.tb[AD][i];
-        bv[k] = sh.tb[AB][i];
-        ev[k] = sh.tb[AE][i];
+        rv[k] = tb[AR][i];
+        tv[k] = UPDATE ? tb[AT][i] : T(0);
+        lv[k] = tb[AL][i];
+        dv[k] = tb[AD][i];
+        bv[k] = tb[AB][i];
+        ev[k] = tb[AE][i];
         x0[k] = (T)xr[u * 8 + b];  // read by every lane: no branch
         x1[k] = (T)xr[64 + u * 8 + a];
       }
@@ -1093,7 +1116,7 @@ __global__ __launch_bounds__(192) void k_fwd(Lay L, T* __restrict__ r, const T* 
 // covers slots S-1-cU down to S-U-cU, and its u-th slot, S-1-cU-u, is element (U-1-u)*64 + lane
 // of the chunk's run.
 template <bool INIT, typename T>
-__global__ __launch_bounds__(192) void k_bwd(Lay L, const T* __restrict__ qv, const T* __restrict__ dr,
+__global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, const T* __restrict__ dr,
                                              const T* __restrict__ du, const T* __restrict__ df,
                                              const T* __restrict__ E, const T* __restrict__ r,
                                              T* __restrict__ z, double* X, double* __restrict__ part,
@@ -1109,6 +1132,7 @@ __global__ __launch_bounds__(192) void k_bwd(Lay L, const T* __restrict__ qv, co
     sh.done = 0;
     sh.freed = 0;
     sh.xready = 0;
+    sh.loaded = 0;
   }
   __syncthreads();
   if (sh.stop) return;
@@ -1154,10 +1178,14 @@ __global__ __launch_bounds__(192) void k_bwd(Lay L, const T* __restrict__ qv, co
     sw_wave2(sh, NC, ent, sc, lane);
     return;
   }
-  const T* src[SW_NA] = {qv + brick, dr + brick, du + brick, df + brick, E + brick, r + brick};
+  if (wave == 3) {  // array loads
+    const T* src[SW_NA] = {qv + brick, dr + brick, du + brick, df + brick, E + brick, r + brick};
+    sw_wave3<SW_NA>(sh, src, NC, S, RunBwd{}, sc, lane);
+    return;
+  }
   T zprev = T(0);
   double acc = 0.0;
-  sw_wave0<SW_NA>(sh, src, NC, S, RunBwd{}, sc, lane, [&](int c, int slot, const double* xr) __attribute__((always_inline)) {
+  sw_wave0(sh, NC, sc, [&](int c, int slot, const T (*tb)[SW_CE], const double* xr) __attribute__((always_inline)) {
     constexpr int UB = sizeof(T) == 8 ? 4 : 8;  // operands of UB slots into registers first
 #pragma unroll
     for (int u0 = 0; u0 < U; u0 += UB) {
@@ -1165,12 +1193,12 @@ __global__ __launch_bounds__(192) void k_bwd(Lay L, const T* __restrict__ qv, co
 #pragma unroll
       for (int k = 0; k < UB; k++) {
         const int u = u0 + k, i = (U - 1 - u) * 64 + lane;
-        qv[k] = sh.tb[AQ][i];
-        v1[k] = sh.tb[A1][i];
-        v2[k] = sh.tb[A2][i];
-        v3[k] = sh.tb[A3][i];
-        ev[k] = sh.tb[AE][i];
-        rv[k] = sh.tb[AR][i];
+        qv[k] = tb[AQ][i];
+        v1[k] = tb[A1][i];
+        v2[k] = tb[A2][i];
+        v3[k] = tb[A3][i];
+        ev[k] = tb[AE][i];
+        rv[k] = tb[AR][i];
         x0[k] = (T)xr[u * 8 + b];  // read by every lane: no branch
         x1[k] = (T)xr[64 + u * 8 + a];
       }
@@ -1446,7 +1474,7 @@ int launch_fwd(fs_ctx* c, bool update, int iter) {
   Sys<S> y = sys_of<S>(P);
   uint64_t* tk = P.tickets.as<uint64_t>();
   auto go = [&](auto kern) {
-    kern<<<P.nlive, 192, 0, c->stream>>>(L, y.r, y.t, y.cl, y.cd, y.cb, y.e, y.q, P.xf.as<double>(),
+    kern<<<P.nlive, 256, 0, c->stream>>>(L, y.r, y.t, y.cl, y.cd, y.cb, y.e, y.q, P.xf.as<double>(),
                                         P.partials2.as<double>(), P.partials.as<double>(), tk + SW_FWD,
                                         P.ticket_base[SW_FWD], P.fwd_order.as<int>(), P.scal.as<PcgScalars>(), iter,
                                         c->cfg.tol_mode, c->cfg.cg_tolerance);
@@ -1468,7 +1496,7 @@ int launch_bwd(fs_ctx* c, bool init) {
   Sys<S> y = sys_of<S>(P);
   uint64_t* tk = P.tickets.as<uint64_t>();
   auto go = [&](auto kern) {
-    kern<<<P.nlive, 192, 0, c->stream>>>(L, y.q, y.dr, y.du, y.df, y.e, y.r, y.z, P.xb.as<double>(),
+    kern<<<P.nlive, 256, 0, c->stream>>>(L, y.q, y.dr, y.du, y.df, y.e, y.r, y.z, P.xb.as<double>(),
                                         P.partials.as<double>(), tk + SW_BWD, P.ticket_base[SW_BWD],
                                         P.bwd_order.as<int>(), P.scal.as<PcgScalars>());
   };
