@@ -745,7 +745,7 @@ template <typename V>
 using Vec16 = typename std::conditional<sizeof(V) == 8, sw_d2, sw_f4>::type;  // 16 bytes, a register tuple
 template <typename V>
 struct SwCfg {
-  static constexpr int R = sizeof(V) == 8 ? 2 : 4;           // chunks of loads in flight (loader register sets)
+  static constexpr int R = sizeof(V) == 8 ? 3 : 4;           // chunks of loads in flight (loader register sets)
   static constexpr int RD = sizeof(V) == 8 ? 2 : 4;          // chunks loaded into LDS ahead of the compute wave
   static constexpr int OPS = SW_CE * (int)sizeof(V) / 1024;  // 16-byte loads per lane, array and chunk
   static constexpr int PER16 = 16 / (int)sizeof(V);          // elements per 16-byte access
