@@ -58,7 +58,7 @@ struct Dims {
 // applyCollider (fluid-simulator.cc:76-111) is u *= 0 on the walls and u *= (1 - n_a) inside
 // the collider: the reverse multiplies by the same factor.
 template <typename T>
-__global__ void k_collider_adj(int a, Dims D, int nx, int ny, int nz, T h0, T h1, T h2, const T* __restrict__ col,
+__global__ __launch_bounds__(TPB) void k_collider_adj(int a, Dims D, int nx, int ny, int nz, T h0, T h1, T h2, const T* __restrict__ col,
                                T* __restrict__ G) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id >= (int64_t)D.d[0] * D.d[1] * D.d[2]) return;
@@ -81,7 +81,7 @@ __global__ void k_collider_adj(int a, Dims D, int nx, int ny, int nz, T h0, T h1
 // project (project-solver.cc:348-420), reverse on the faces: a face the projection sets to 0
 // (weight > 0 on a domain wall) passes nothing back; every other face passes its adjoint.
 template <typename T>
-__global__ void k_project_faces_adj(int a, Dims D, const T* __restrict__ wts, T* __restrict__ G) {
+__global__ __launch_bounds__(TPB) void k_project_faces_adj(int a, Dims D, const T* __restrict__ wts, T* __restrict__ G) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id >= (int64_t)D.d[0] * D.d[1] * D.d[2]) return;
   int ix[3];
@@ -94,7 +94,7 @@ __global__ void k_project_faces_adj(int a, Dims D, const T* __restrict__ wts, T*
 // u -= (p_i - p_{i-1}) dt/h; only the lower fluid u += p_{i-1} dt/(h theta); only the upper
 // u -= p_i dt/(h (1 - theta)). Only active cells carry a pressure.
 template <typename T>
-__global__ void k_project_cells_adj(int nx, int ny, int nz, T dth, const T* __restrict__ fsd,
+__global__ __launch_bounds__(TPB) void k_project_cells_adj(int nx, int ny, int nz, T dth, const T* __restrict__ fsd,
                                     const uint8_t* __restrict__ act, const T* __restrict__ uw,
                                     const T* __restrict__ vw, const T* __restrict__ ww, const T* __restrict__ gu,
                                     const T* __restrict__ gv, const T* __restrict__ gw, T* __restrict__ pbar) {
@@ -153,7 +153,7 @@ __device__ inline void theta_grad(T a, T b, T& da, T& db) {  // theta = a / (a -
 }
 
 template <typename T>
-__global__ void k_theta_adj(int nx, int ny, int nz, T f, const T* __restrict__ phi, const uint8_t* __restrict__ act,
+__global__ __launch_bounds__(TPB) void k_theta_adj(int nx, int ny, int nz, T f, const T* __restrict__ phi, const uint8_t* __restrict__ act,
                             const T* __restrict__ uw, const T* __restrict__ vw, const T* __restrict__ ww,
                             const T* __restrict__ lam, const T* __restrict__ p, const T* __restrict__ gu,
                             const T* __restrict__ gv, const T* __restrict__ gw, T* __restrict__ phibar) {
@@ -247,7 +247,7 @@ __global__ void k_theta_adj(int nx, int ny, int nz, T f, const T* __restrict__ p
 // +w u (lower face) and -w u (upper face), interior faces only. Reverse:
 // ubar(f) += w(f) (lambda(upper cell) - lambda(lower cell)), active cells only.
 template <typename T>
-__global__ void k_rhs_adj(int a, Dims D, int nx, int ny, int nz, const T* __restrict__ wts,
+__global__ __launch_bounds__(TPB) void k_rhs_adj(int a, Dims D, int nx, int ny, int nz, const T* __restrict__ wts,
                           const uint8_t* __restrict__ act, const T* __restrict__ lam, T* __restrict__ G) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id >= (int64_t)D.d[0] * D.d[1] * D.d[2]) return;
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(TPB) void k_force_adj(int a, Dims D, const uint8_t*
 }
 
 // extrapolate (cpu/fluid-simulator.h:204-253): the valid mask after one iteration.
-__global__ void k_extrap_mask(Dims D, const uint8_t* __restrict__ V, uint8_t* __restrict__ VN) {
+__global__ __launch_bounds__(TPB) void k_extrap_mask(Dims D, const uint8_t* __restrict__ V, uint8_t* __restrict__ VN) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id >= (int64_t)D.d[0] * D.d[1] * D.d[2]) return;
   int i, j, k;
@@ -314,7 +314,7 @@ __device__ inline int valid_count(const uint8_t* V, int64_t id, int i, int j, in
 //   Gbar(n)  = B(n) + [n valid] sum over invalid neighbours c of A(c) / count(c)
 //   GBbar(c) = [c valid] A(c)
 template <typename T>
-__global__ void k_extrap_adj(Dims D, const uint8_t* __restrict__ V, const T* __restrict__ A, const T* __restrict__ B,
+__global__ __launch_bounds__(TPB) void k_extrap_adj(Dims D, const uint8_t* __restrict__ V, const T* __restrict__ A, const T* __restrict__ B,
                              T* __restrict__ Gb, T* __restrict__ GBb) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id >= (int64_t)D.d[0] * D.d[1] * D.d[2]) return;
@@ -339,7 +339,7 @@ __global__ void k_extrap_adj(Dims D, const uint8_t* __restrict__ V, const T* __r
 
 // applyDirichletBoundary (fluid-simulator.cc:113-138) sets the wall faces to 0.
 template <typename T>
-__global__ void k_zero_walls(int a, Dims D, T* __restrict__ G) {
+__global__ __launch_bounds__(TPB) void k_zero_walls(int a, Dims D, T* __restrict__ G) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id >= (int64_t)D.d[0] * D.d[1] * D.d[2]) return;
   int ix[3];
@@ -349,7 +349,7 @@ __global__ void k_zero_walls(int a, Dims D, T* __restrict__ G) {
 
 // P2G reverse node fields: H = gbar / W where W > 0 (else 0), HG = H * g.
 template <typename T>
-__global__ void k_p2g_fields(int64_t n, const T* __restrict__ gb, const T* __restrict__ W, const T* __restrict__ g,
+__global__ __launch_bounds__(TPB) void k_p2g_fields(int64_t n, const T* __restrict__ gb, const T* __restrict__ W, const T* __restrict__ g,
                              T* __restrict__ H, T* __restrict__ HG) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id >= n) return;
@@ -359,17 +359,17 @@ __global__ void k_p2g_fields(int64_t n, const T* __restrict__ gb, const T* __res
 }
 
 template <typename T>
-__global__ void k_scale(int64_t n, T s, const T* __restrict__ a, T* __restrict__ b) {
+__global__ __launch_bounds__(TPB) void k_scale(int64_t n, T s, const T* __restrict__ a, T* __restrict__ b) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id < n) b[id] = s * a[id];
 }
 template <typename T>
-__global__ void k_add(int64_t n, const T* __restrict__ a, T* __restrict__ b) {
+__global__ __launch_bounds__(TPB) void k_add(int64_t n, const T* __restrict__ a, T* __restrict__ b) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id < n) b[id] = b[id] + a[id];
 }
 template <typename T>
-__global__ void k_split3(int64_t n, const double* __restrict__ aos, T* __restrict__ x, T* __restrict__ y,
+__global__ __launch_bounds__(TPB) void k_split3(int64_t n, const double* __restrict__ aos, T* __restrict__ x, T* __restrict__ y,
                          T* __restrict__ z) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -378,7 +378,7 @@ __global__ void k_split3(int64_t n, const double* __restrict__ aos, T* __restric
   z[i] = (T)aos[3 * i + 2];
 }
 template <typename T>
-__global__ void k_join3(int64_t n, const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z,
+__global__ __launch_bounds__(TPB) void k_join3(int64_t n, const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z,
                         double* __restrict__ aos) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -387,12 +387,12 @@ __global__ void k_join3(int64_t n, const T* __restrict__ x, const T* __restrict_
   aos[3 * i + 2] = (double)z[i];
 }
 template <typename T>
-__global__ void k_from_f64(int64_t n, const double* __restrict__ a, T* __restrict__ b) {
+__global__ __launch_bounds__(TPB) void k_from_f64(int64_t n, const double* __restrict__ a, T* __restrict__ b) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) b[i] = (T)a[i];
 }
 template <typename T>
-__global__ void k_to_f64(int64_t n, const T* __restrict__ a, double* __restrict__ b) {
+__global__ __launch_bounds__(TPB) void k_to_f64(int64_t n, const T* __restrict__ a, double* __restrict__ b) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) b[i] = (double)a[i];
 }

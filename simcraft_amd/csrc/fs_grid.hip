@@ -88,6 +88,19 @@ __device__ inline void sample_grid(const Stencil<T>& st, const T* __restrict__ G
       if (y < 0 || y >= gy) continue;
       T wxy = st.w[0][i] * st.w[1][j];
       int base = gz * (y + gy * x);
+      if (st.b[2] >= 0 && st.b[2] + 3 < gz) {  // the four z-nodes in range: one row, read at once
+        T g4[4], h4[4];
+        __builtin_memcpy(g4, G + base + st.b[2], sizeof(g4));
+        if (TWO) __builtin_memcpy(h4, G2 + base + st.b[2], sizeof(h4));
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          T w = wxy * st.w[2][k];
+          acc = acc + g4[k] * w;
+          if (TWO) acc2 = acc2 + h4[k] * w;
+          ws = ws + w;
+        }
+        continue;
+      }
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         int z = st.b[2] + k;
@@ -167,7 +180,7 @@ __device__ inline void handle_collision(const GridArgs<T>& ga, const T* __restri
 
 // FlipAdvectionSolver3D::advect (advect-solver.cc:170-187): Ralston RK3 + collision.
 template <typename T>
-__global__ void k_advect(int n, GridArgs<T> ga, Grids3<T> G, const T* __restrict__ col, T* __restrict__ px,
+__global__ __launch_bounds__(TPB) void k_advect(int n, GridArgs<T> ga, Grids3<T> G, const T* __restrict__ col, T* __restrict__ px,
                          T* __restrict__ py, T* __restrict__ pz, T* __restrict__ vx, T* __restrict__ vy,
                          T* __restrict__ vz, T dt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -196,7 +209,7 @@ __global__ void k_advect(int n, GridArgs<T> ga, Grids3<T> G, const T* __restrict
 
 // FlipAdvectionSolver3D::solveG2P / PicAdvector3D::solveG2P (advect-solver.cc:7-17, 157-169).
 template <typename T, bool FLIP>
-__global__ void k_g2p(int n, GridArgs<T> ga, Grids3<T> G, Grids3<T> L, const T* __restrict__ px,
+__global__ __launch_bounds__(TPB) void k_g2p(int n, GridArgs<T> ga, Grids3<T> G, Grids3<T> L, const T* __restrict__ px,
                       const T* __restrict__ py, const T* __restrict__ pz, T* __restrict__ vx, T* __restrict__ vy,
                       T* __restrict__ vz, T alpha) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -223,7 +236,7 @@ __global__ void k_g2p(int n, GridArgs<T> ga, Grids3<T> G, Grids3<T> L, const T* 
 
 // ---- counting sort of particles into bins (stable: index ascending inside a bin) ----
 template <typename T>
-__global__ void k_keys_nearest(int n, int axis, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
+__global__ __launch_bounds__(TPB) void k_keys_nearest(int n, int axis, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
                                const T* __restrict__ pz, int* __restrict__ keys) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -239,7 +252,7 @@ __global__ void k_keys_nearest(int n, int axis, GridArgs<T> ga, const T* __restr
 
 // ParticleNeighbourSearcher::update key (Spatify/include/Spatify/ns-util.h:50-62)
 template <typename T>
-__global__ void k_keys_morton(int n, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
+__global__ __launch_bounds__(TPB) void k_keys_morton(int n, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
                               const T* __restrict__ pz, int* __restrict__ keys) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -255,20 +268,37 @@ __global__ void k_keys_morton(int n, GridArgs<T> ga, const T* __restrict__ px, c
   keys[i] = morton3D(ix, iy, iz);
 }
 
-__global__ void k_count(int n, const int* __restrict__ keys, int* __restrict__ counts, int* __restrict__ slots) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  slots[i] = atomicAdd(&counts[keys[i]], 1);
+// Particles are stored in a spatially coherent order, so the 64 particles of a wave fall into
+// few bins. Lanes with equal keys are served by one atomic from the group's lowest lane, which
+// takes the whole group's count; each lane's slot is the returned base plus its rank in the group.
+// (Slots only seed the bin's positions: k_sort_bins restores index order inside every bin.)
+__global__ __launch_bounds__(TPB) void k_count(int n, const int* __restrict__ keys, int* __restrict__ counts, int* __restrict__ slots) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
+  const bool ok = i < n;
+  const int key = ok ? keys[i] : -1;
+  uint64_t todo = __builtin_amdgcn_ballot_w64(ok);
+  int slot = 0;
+  while (todo != 0) {
+    const int leader = __builtin_ctzll(todo);
+    const int lk = __shfl(key, leader, 64);
+    const uint64_t grp = __builtin_amdgcn_ballot_w64(key == lk) & todo;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&counts[lk], __builtin_popcountll(grp));
+    base = __shfl(base, leader, 64);
+    if ((grp >> lane) & 1) slot = base + __builtin_popcountll(grp & ((1ull << lane) - 1));
+    todo &= ~grp;
+  }
+  if (ok) slots[i] = slot;
 }
 
-__global__ void k_scatter(int n, const int* __restrict__ keys, const int* __restrict__ slots,
+__global__ __launch_bounds__(TPB) void k_scatter(int n, const int* __restrict__ keys, const int* __restrict__ slots,
                           const int* __restrict__ offs, int* __restrict__ perm) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   perm[offs[keys[i]] + slots[i]] = i;
 }
 
-__global__ void k_sort_bins(int nb, const int* __restrict__ offs, int* __restrict__ perm) {
+__global__ __launch_bounds__(TPB) void k_sort_bins(int nb, const int* __restrict__ offs, int* __restrict__ perm) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   int s = offs[b], e = offs[b + 1];
@@ -289,7 +319,7 @@ __global__ void k_sort_bins(int nb, const int* __restrict__ offs, int* __restric
 // n - b + 1 on each axis, so the node weight CubicKernel::weight(h, p - pn) is
 // (W[ox] * W[4+oy]) * W[8+oz], the same product the reference forms for that pair.
 template <typename T>
-__global__ void k_p2g_records(int n, int axis, GridArgs<T> ga, const int* __restrict__ perm, const T* __restrict__ px,
+__global__ __launch_bounds__(TPB) void k_p2g_records(int n, int axis, GridArgs<T> ga, const int* __restrict__ perm, const T* __restrict__ px,
                               const T* __restrict__ py, const T* __restrict__ pz, const T* __restrict__ vc,
                               T* __restrict__ W, T* __restrict__ sv) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -545,20 +575,28 @@ struct Bits<float> {
 };
 
 // The last particle of a Morton bin in the stable order is the one with the largest index.
-__global__ void k_bin_last(int n, const int* __restrict__ keys, int* __restrict__ binmax) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  atomicMax(&binmax[keys[i]], i);
+// Lanes with equal keys are served by one atomic from the group's highest lane (largest index).
+__global__ __launch_bounds__(TPB) void k_bin_last(int n, const int* __restrict__ keys, int* __restrict__ binmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
+  const bool ok = i < n;
+  const int key = ok ? keys[i] : -1;
+  uint64_t todo = __builtin_amdgcn_ballot_w64(ok);
+  while (todo != 0) {
+    const int lk = __shfl(key, __builtin_ctzll(todo), 64);
+    const uint64_t grp = __builtin_amdgcn_ballot_w64(key == lk) & todo;
+    if (lane == 63 - __builtin_clzll(grp)) atomicMax(&binmax[lk], i);
+    todo &= ~grp;
+  }
 }
 
 template <typename T>
-__global__ void k_fill_bits(int n, typename Bits<T>::U* __restrict__ d) {
+__global__ __launch_bounds__(TPB) void k_fill_bits(int n, typename Bits<T>::U* __restrict__ d) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = Bits<T>::inf;
 }
 
 template <typename T>
-__global__ void k_recon_splat(int n, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
+__global__ __launch_bounds__(TPB) void k_recon_splat(int n, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
                               const T* __restrict__ pz, const int* __restrict__ keys, const int* __restrict__ binmax,
                               T r2, T s_max, typename Bits<T>::U* __restrict__ dmin) {
   using UB = typename Bits<T>::U;
@@ -607,7 +645,7 @@ __global__ void k_recon_splat(int n, GridArgs<T> ga, const T* __restrict__ px, c
 
 // sdf.grid.fill(1e9); sdfValid.fill(false); then min(sdf, |d| - r) for the kept particles.
 template <typename T>
-__global__ void k_recon_final(int ncell, const typename Bits<T>::U* __restrict__ dmin, T radius, T* __restrict__ sdf,
+__global__ __launch_bounds__(TPB) void k_recon_final(int ncell, const typename Bits<T>::U* __restrict__ dmin, T radius, T* __restrict__ sdf,
                               uint8_t* __restrict__ valid) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
@@ -625,7 +663,7 @@ __global__ void k_recon_final(int ncell, const typename Bits<T>::U* __restrict__
 
 // extrapolateFluidSdf iteration (cpu/fluid-simulator.h:124-164): valid cells are not copied.
 template <typename T>
-__global__ void k_extrap_sdf(int nx, int ny, int nz, const T* __restrict__ cur, T* __restrict__ buf,
+__global__ __launch_bounds__(TPB) void k_extrap_sdf(int nx, int ny, int nz, const T* __restrict__ cur, T* __restrict__ buf,
                              const uint8_t* __restrict__ val, uint8_t* __restrict__ vbuf) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nx * ny * nz) return;
@@ -655,7 +693,7 @@ __global__ void k_extrap_sdf(int nx, int ny, int nz, const T* __restrict__ cur, 
 // smoothFluidSurface iteration (FluidSim/src/cpu/fluid-simulator.cc:40-71): min(avg6, self).
 // FLAG also records whether the cell took the average (the reverse pass needs the branch).
 template <typename T, bool FLAG = false>
-__global__ void k_smooth(int nx, int ny, int nz, const T* __restrict__ cur, T* __restrict__ buf,
+__global__ __launch_bounds__(TPB) void k_smooth(int nx, int ny, int nz, const T* __restrict__ cur, T* __restrict__ buf,
                          uint8_t* __restrict__ took_avg = nullptr) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nx * ny * nz) return;
@@ -679,7 +717,7 @@ __global__ void k_smooth(int nx, int ny, int nz, const T* __restrict__ cur, T* _
 
 // applyDirichletBoundary (fluid-simulator.cc:113-138) for face grid `axis`.
 template <typename T>
-__global__ void k_dirichlet(int axis, int gx, int gy, int gz, T* __restrict__ G, uint8_t* __restrict__ valid) {
+__global__ __launch_bounds__(TPB) void k_dirichlet(int axis, int gx, int gy, int gz, T* __restrict__ G, uint8_t* __restrict__ valid) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= gx * gy * gz) return;
   int i, j, k;
@@ -694,7 +732,7 @@ __global__ void k_dirichlet(int axis, int gx, int gy, int gz, T* __restrict__ G,
 
 // extrapolate<GridType> iteration (cpu/fluid-simulator.h:209-251).
 template <typename T>
-__global__ void k_extrap_vel(int gx, int gy, int gz, const T* __restrict__ g, T* __restrict__ gbuf,
+__global__ __launch_bounds__(TPB) void k_extrap_vel(int gx, int gy, int gz, const T* __restrict__ g, T* __restrict__ gbuf,
                              const uint8_t* __restrict__ val, uint8_t* __restrict__ vbuf) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= gx * gy * gz) return;
@@ -724,7 +762,7 @@ __global__ void k_extrap_vel(int gx, int gy, int gz, const T* __restrict__ g, T*
 
 // applyForce (fluid-simulator.cc:13-18)
 template <typename T>
-__global__ void k_force(int gx, int gy, int gz, T* __restrict__ V, const uint8_t* __restrict__ valid, T gdt) {
+__global__ __launch_bounds__(TPB) void k_force(int gx, int gy, int gz, T* __restrict__ V, const uint8_t* __restrict__ valid, T gdt) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= gx * gy * gz) return;
   int i, j, k;
@@ -735,7 +773,7 @@ __global__ void k_force(int gx, int gy, int gz, T* __restrict__ V, const uint8_t
 // Face solid weights of FvmSolver3D::buildSystem (project-solver.cc:77-127): faces between
 // two air cells keep 0; otherwise 1 - fractionInside of the collider SDF at the 4 corners.
 template <typename T>
-__global__ void k_face_weights(int axis, GridArgs<T> ga, int gx, int gy, int gz, const T* __restrict__ fs,
+__global__ __launch_bounds__(TPB) void k_face_weights(int axis, GridArgs<T> ga, int gx, int gy, int gz, const T* __restrict__ fs,
                                const T* __restrict__ col, T* __restrict__ W) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= gx * gy * gz) return;
@@ -782,7 +820,7 @@ __global__ void k_face_weights(int axis, GridArgs<T> ga, int gx, int gy, int gz,
 // Cell loop of FvmSolver3D::buildSystem (project-solver.cc:129-226): active marking and the
 // 7-point ghost-fluid Poisson row. Neighbour order left, right, down, up, back, front.
 template <typename T>
-__global__ void k_assemble(GridArgs<T> ga, T dt, const T* __restrict__ fs, const T* __restrict__ uW,
+__global__ __launch_bounds__(TPB) void k_assemble(GridArgs<T> ga, T dt, const T* __restrict__ fs, const T* __restrict__ uW,
                            const T* __restrict__ vW, const T* __restrict__ wW, const T* __restrict__ U,
                            const T* __restrict__ V, const T* __restrict__ Wv, T* __restrict__ Adiag,
                            T* __restrict__ aL, T* __restrict__ aR, T* __restrict__ aU, T* __restrict__ aD,
@@ -883,7 +921,7 @@ __global__ void k_assemble(GridArgs<T> ga, T dt, const T* __restrict__ fs, const
 
 // FvmSolver3D::project (project-solver.cc:348-421) for face grid `axis`.
 template <typename T>
-__global__ void k_project(int axis, GridArgs<T> ga, int gx, int gy, int gz, T dt, const T* __restrict__ fs,
+__global__ __launch_bounds__(TPB) void k_project(int axis, GridArgs<T> ga, int gx, int gy, int gz, T dt, const T* __restrict__ fs,
                           const T* __restrict__ pg, const T* __restrict__ Wt, T* __restrict__ G) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= gx * gy * gz) return;
@@ -917,7 +955,7 @@ __global__ void k_project(int axis, GridArgs<T> ga, int gx, int gy, int gz, T dt
 
 // applyCollider (fluid-simulator.cc:76-111) for face grid `axis`.
 template <typename T>
-__global__ void k_collider(int axis, GridArgs<T> ga, int gx, int gy, int gz, const T* __restrict__ col,
+__global__ __launch_bounds__(TPB) void k_collider(int axis, GridArgs<T> ga, int gx, int gy, int gz, const T* __restrict__ col,
                            T* __restrict__ G) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= gx * gy * gz) return;
@@ -942,7 +980,7 @@ __global__ void k_collider(int axis, GridArgs<T> ga, int gx, int gy, int gz, con
 // max |u| over a face grid (CFL, fluid-simulator.cc:184-203): fl(h/|u|) is monotone in |u|,
 // so min over faces of h/|u| is h / max|u|.
 template <typename T>
-__global__ void k_maxabs(int n, const T* __restrict__ g, unsigned long long* __restrict__ out) {
+__global__ __launch_bounds__(TPB) void k_maxabs(int n, const T* __restrict__ g, unsigned long long* __restrict__ out) {
   __shared__ double sm[TPB];
   double m = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -960,18 +998,18 @@ __global__ void k_maxabs(int n, const T* __restrict__ g, unsigned long long* __r
 }
 
 template <typename T>
-__global__ void k_to_double(int64_t n, const T* __restrict__ a, double* __restrict__ b) {
+__global__ __launch_bounds__(TPB) void k_to_double(int64_t n, const T* __restrict__ a, double* __restrict__ b) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) b[i] = (double)a[i];
 }
 template <typename T>
-__global__ void k_from_double(int64_t n, const double* __restrict__ a, T* __restrict__ b) {
+__global__ __launch_bounds__(TPB) void k_from_double(int64_t n, const double* __restrict__ a, T* __restrict__ b) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) b[i] = (T)a[i];
 }
 
 template <typename T>
-__global__ void k_split_aos(int64_t n, const double* __restrict__ aos, T* __restrict__ x, T* __restrict__ y,
+__global__ __launch_bounds__(TPB) void k_split_aos(int64_t n, const double* __restrict__ aos, T* __restrict__ x, T* __restrict__ y,
                             T* __restrict__ z) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -980,7 +1018,7 @@ __global__ void k_split_aos(int64_t n, const double* __restrict__ aos, T* __rest
   z[i] = (T)aos[3 * i + 2];
 }
 template <typename T>
-__global__ void k_join_aos(int64_t n, const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z,
+__global__ __launch_bounds__(TPB) void k_join_aos(int64_t n, const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z,
                            double* __restrict__ aos) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1266,7 +1304,7 @@ int cfl_t(fs_ctx* c, double* out) {
 
 // ---- reverse mode: the fluid-surface chain ---------------------------------------------------
 template <typename T>
-__global__ void k_accumulate(int64_t n, const T* __restrict__ a, T* __restrict__ b) {
+__global__ __launch_bounds__(TPB) void k_accumulate(int64_t n, const T* __restrict__ a, T* __restrict__ b) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) b[i] = b[i] + a[i];
 }
@@ -1284,7 +1322,7 @@ __device__ inline int v_count(const uint8_t* V, int id, int i, int j, int k, int
 // the average. curbar(n) = [n kept itself] outbar(n) + sum over neighbours c that averaged of
 // outbar(c) / count(c).
 template <typename T>
-__global__ void k_smooth_adj(int nx, int ny, int nz, const uint8_t* __restrict__ F, const T* __restrict__ ob,
+__global__ __launch_bounds__(TPB) void k_smooth_adj(int nx, int ny, int nz, const uint8_t* __restrict__ F, const T* __restrict__ ob,
                              T* __restrict__ cb) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nx * ny * nz) return;
@@ -1307,7 +1345,7 @@ __global__ void k_smooth_adj(int nx, int ny, int nz, const uint8_t* __restrict__
 //   curbar(n) = B(n) + [n valid] sum over invalid neighbours c of A(c) / count(c)
 //   bufbar(c) = [c valid or count(c) = 0] A(c)
 template <typename T>
-__global__ void k_extrap_sdf_adj(int nx, int ny, int nz, const uint8_t* __restrict__ V, const T* __restrict__ A,
+__global__ __launch_bounds__(TPB) void k_extrap_sdf_adj(int nx, int ny, int nz, const uint8_t* __restrict__ V, const T* __restrict__ A,
                                  const T* __restrict__ B, T* __restrict__ cb, T* __restrict__ bb) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nx * ny * nz) return;
@@ -1331,7 +1369,7 @@ __global__ void k_extrap_sdf_adj(int nx, int ny, int nz, const uint8_t* __restri
 // particle p (the one whose squared distance equals the cell's minimum), so p receives
 // abar(c) (x_p - x_c) / |x_c - x_p|. Same candidate cells and inclusion tests as k_recon_splat.
 template <typename T>
-__global__ void k_recon_adj(int n, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
+__global__ __launch_bounds__(TPB) void k_recon_adj(int n, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
                             const T* __restrict__ pz, const int* __restrict__ keys, const int* __restrict__ binmax,
                             T r2, T s_max, const typename Bits<T>::U* __restrict__ dmin, const T* __restrict__ abar,
                             T* __restrict__ gx, T* __restrict__ gy, T* __restrict__ gz) {
@@ -1556,7 +1594,7 @@ struct PArr {
 // velocity (alpha * vbar), the transfer coefficients vbar / S of the transposed gather, and
 // adds the position derivative to xbar.
 template <typename T, bool FLIP>
-__global__ void k_g2p_adj(int n, GridArgs<T> ga, Grids3<T> G, Grids3<T> L, PArr<T> pos, PArr<T> vbar, PArr<T> xbar,
+__global__ __launch_bounds__(TPB) void k_g2p_adj(int n, GridArgs<T> ga, Grids3<T> G, Grids3<T> L, PArr<T> pos, PArr<T> vbar, PArr<T> xbar,
                           PArr<T> vbar1, PArr<T> coef, T alpha) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1591,7 +1629,7 @@ __global__ void k_g2p_adj(int n, GridArgs<T> ga, Grids3<T> G, Grids3<T> L, PArr<
 // With H = gbar / W (0 where W = 0) and HG = H * g per node:
 // vbar_a += sum w H, xbar += v_a * sum dw H - sum dw HG.
 template <typename T>
-__global__ void k_p2g_adj(int n, GridArgs<T> ga, Grids3<T> H, Grids3<T> HG, PArr<T> pos, PArr<T> vel, PArr<T> vbar,
+__global__ __launch_bounds__(TPB) void k_p2g_adj(int n, GridArgs<T> ga, Grids3<T> H, Grids3<T> HG, PArr<T> pos, PArr<T> vel, PArr<T> vbar,
                           PArr<T> xbar) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1617,7 +1655,7 @@ __global__ void k_p2g_adj(int n, GridArgs<T> ga, Grids3<T> H, Grids3<T> HG, PArr
 // projection v -= min(v.n, 0) n likewise; a clamped coordinate and a velocity component the
 // wall zeroes carry no derivative. In: xbar, vbar of the outputs; out: of the inputs.
 template <typename T>
-__global__ void k_advect_adj(int n, GridArgs<T> ga, const T* __restrict__ col, PArr<T> x0, PArr<T> v0, PArr<T> xbar,
+__global__ __launch_bounds__(TPB) void k_advect_adj(int n, GridArgs<T> ga, const T* __restrict__ col, PArr<T> x0, PArr<T> v0, PArr<T> xbar,
                              PArr<T> vbar, T dt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;

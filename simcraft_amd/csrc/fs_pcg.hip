@@ -553,10 +553,10 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const S* __restrict__ x, co
   // memory latency per operand and element. Groups past the end reread the last group and are
   // dropped.
   constexpr int MB = 4;
-  const int ne = (int)L.ne;
+  const int ne = (int)L.ne, chunk = blockIdx.x;
   uint8_t fl[16];
 #pragma unroll
-  for (int m = 0; m < 16; m++) fl[m] = nbm[min((int)blockIdx.x * CH + m * 256, ne - 256) + (int)threadIdx.x];
+  for (int m = 0; m < 16; m++) fl[m] = nbm[min(chunk * CH + m * 256, ne - 256) + (int)threadIdx.x];
   double acc = 0.0;
 #pragma unroll
   for (int m0 = 0; m0 < 16; m0 += MB) {
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const S* __restrict__ x, co
     bool in[MB];
 #pragma unroll
     for (int q = 0; q < MB; q++) {
-      const int raw = (int)blockIdx.x * CH + (m0 + q) * 256;
+      const int raw = chunk * CH + (m0 + q) * 256;
       in[q] = raw < ne;
       const int base = in[q] ? raw : ne - 256;
       const Group g = group_of(L, base);
@@ -604,7 +604,7 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const S* __restrict__ x, co
   }
   if (!PCG) return;
   double bs = block_sum256(acc, sm);
-  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+  if (threadIdx.x == 0) part[chunk] = bs;
   if (last_block(&sc->done_ctr[DC_SPMV], gridDim.x, 256)) {
     double sdotz = tree256(part, nchunks, sm);
     if (threadIdx.x == 0) {
