@@ -557,6 +557,11 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const S* __restrict__ x, co
   uint8_t fl[16];
 #pragma unroll
   for (int m = 0; m < 16; m++) fl[m] = nbm[min(chunk * CH + m * 256, ne - 256) + (int)threadIdx.x];
+  // with S >= 64 a chunk (64 slots) spans at most two bricks, whose neighbour bricks are looked
+  // up once; smaller layouts look them up per group
+  const int SL = L.S * 64;
+  const bool two = SL >= CH;
+  const Group ga = group_of(L, min(chunk * CH, ne - 256)), gb = group_of(L, min(chunk * CH + CH - 256, ne - 256));
   double acc = 0.0;
 #pragma unroll
   for (int m0 = 0; m0 < 16; m0 += MB) {
@@ -567,7 +572,13 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const S* __restrict__ x, co
       const int raw = chunk * CH + (m0 + q) * 256;
       in[q] = raw < ne;
       const int base = in[q] ? raw : ne - 256;
-      const Group g = group_of(L, base);
+      Group g;
+      if (two) {
+        g = base < (ga.B + 1) * SL ? ga : gb;
+        g.s0 = (base - g.B * SL) >> 6;
+      } else {
+        g = group_of(L, base);
+      }
       const int ee = base + (int)threadIdx.x, lane = threadIdx.x & 63, s = g.s0 + (threadIdx.x >> 6);
       const int a = lane >> 3, b = lane & 7, f = fl[m0 + q];
       e[q] = ee;
@@ -578,20 +589,37 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const S* __restrict__ x, co
       iU[q] = (f & NB_U) ? nb_up(L, g, ee, s, a, b) : ee;
       iF[q] = (f & NB_F) ? ee + 64 : ee;
     }
-    S xc[MB], v[MB];
+    // all loads of the MB elements first (written as a separate loop, the compiler issues them
+    // before any arithmetic instead of waiting for each element's operands in turn)
+    S xc[MB], dg[MB], cL[MB], cD[MB], cB[MB], cR[MB], cU[MB], cF[MB], xL[MB], xD[MB], xB[MB], xR[MB], xU[MB],
+        xF[MB], v[MB];
+#pragma unroll
+    for (int q = 0; q < MB; q++) {
+      xc[q] = x[e[q]];
+      dg[q] = adiag[e[q]];
+      cL[q] = al[e[q]];
+      cD[q] = ad[e[q]];
+      cB[q] = ab[e[q]];
+      cR[q] = al[iR[q]];
+      cU[q] = ad[iU[q]];
+      cF[q] = ab[iF[q]];
+      xL[q] = x[iL[q]];
+      xD[q] = x[iD[q]];
+      xB[q] = x[iB[q]];
+      xR[q] = x[iR[q]];
+      xU[q] = x[iU[q]];
+      xF[q] = x[iF[q]];
+    }
 #pragma unroll
     for (int q = 0; q < MB; q++) {
       const int f = fl[m0 + q];
-      xc[q] = x[e[q]];
-      S w = adiag[e[q]] * xc[q];
-      const S cL = al[e[q]], cD = ad[e[q]], cB = ab[e[q]], cR = al[iR[q]], cU = ad[iU[q]], cF = ab[iF[q]];
-      const S xL = x[iL[q]], xD = x[iD[q]], xB = x[iB[q]], xR = x[iR[q]], xU = x[iU[q]], xF = x[iF[q]];
-      w = (f & NB_L) ? w + cL * xL : w;
-      w = (f & NB_D) ? w + cD * xD : w;
-      w = (f & NB_B) ? w + cB * xB : w;
-      w = (f & NB_R) ? w + cR * xR : w;
-      w = (f & NB_U) ? w + cU * xU : w;
-      w = (f & NB_F) ? w + cF * xF : w;
+      S w = dg[q] * xc[q];
+      w = (f & NB_L) ? w + cL[q] * xL[q] : w;
+      w = (f & NB_D) ? w + cD[q] * xD[q] : w;
+      w = (f & NB_B) ? w + cB[q] * xB[q] : w;
+      w = (f & NB_R) ? w + cR[q] * xR[q] : w;
+      w = (f & NB_U) ? w + cU[q] * xU[q] : w;
+      w = (f & NB_F) ? w + cF[q] * xF[q] : w;
       v[q] = (f & NB_SELF) ? w : S(0);
     }
 #pragma unroll
