@@ -791,8 +791,9 @@ struct SwLds {
   __align__(16) V tb[SwCfg<V>::RD][SW_NA][SW_CE];  // chunks c % RD, one element per lane and slot
   __align__(16) V ob[SW_K][2][SW_CE];     // outbox: outputs of finished chunks
   double xr[SW_KX][2][64];                // exchange entries of chunks c % SW_KX, from wave 2
-  int done, freed, xready, loaded, B, stop;  // chunks finished by wave 0 / released by wave 1 /
-                                             // entries read by wave 2 / arrays loaded by wave 3
+  int prog, freed, xready, loaded, B, stop;  // slots finished by wave 0 / chunks released by wave 1 /
+                                             // slots whose entries wave 2 has read / chunks loaded
+                                             // by wave 3
 };
 
 __device__ __forceinline__ int lds_get(const int* p) {
@@ -816,6 +817,38 @@ __device__ __forceinline__ bool lds_wait(const int* p, int v, PcgScalars* sc) {
     }
   }
   return true;
+}
+
+// Slot counters (prog, xready) are posted once per slot, so they use relaxed accesses: the LDS
+// performs one wave's operations in issue order, so a counter stored after plain LDS stores is
+// seen by another wave only once those stores are performed, and that wave's later reads (issued
+// after it has read the counter) see them. A compiler-only fence keeps the program order, and
+// no wait for the stores to complete is needed.
+__device__ __forceinline__ int lds_peek(const int* p) {
+  const int v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  return v;
+}
+__device__ __forceinline__ void lds_post(int* p, int v) {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// Wait until *p >= v (bounded like lds_wait); returns the value read, or v if the wait gave up.
+__device__ __forceinline__ int lds_until(const int* p, int v, PcgScalars* sc) {
+  long spins = 0;
+  int x;
+  while ((x = __builtin_amdgcn_readfirstlane(lds_peek(p))) < v) {
+    __builtin_amdgcn_s_sleep(1);
+    if ((++spins & 255) == 0) {
+      const int to = __hip_atomic_load(&sc->wait_timeouts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (to != 0 || spins > SPIN_LIMIT) {
+        if ((threadIdx.x & 63) == 0) atomicAdd(&sc->wait_timeouts, 1);
+        __builtin_amdgcn_s_waitcnt(0);  // leave nothing in flight on this path
+        return v;
+      }
+    }
+  }
+  return x;
 }
 
 // last_block for a single wave: true in exactly one wave of the launch, after every wave has
@@ -898,7 +931,7 @@ __device__ __forceinline__ void sw_wave3(SwLds<V>& sh, const V* const* src, int 
   if (R >= 3) sw_load<NARR>(s2, src, nc, 2, lane, run, S);
   if (R >= 4) sw_load<NARR>(s3, src, nc, 3, lane, run, S);
   auto chunk = [&](int c, SwSet<V>& cur) __attribute__((always_inline)) {
-    if (c >= RD) lds_wait(&sh.done, c - RD + 1, sc);
+    if (c >= RD) lds_until(&sh.prog, (c - RD + 1) * U, sc);
     sw_spill<NARR>(sh.tb[c % RD], cur, lane);
     lds_put(&sh.loaded, c + 1);
     sw_load<NARR>(cur, src, nc, c + R, lane, run, S);
@@ -915,34 +948,61 @@ __device__ __forceinline__ void sw_wave3(SwLds<V>& sh, const V* const* src, int 
   if (R >= 4 && c + 2 < nc) chunk(c + 2, s2);
 }
 
-// Wave 0 (arithmetic): for each chunk, once wave 3 has loaded it, wave 2 has read its exchange
-// entries and wave 1 has released the outbox slot, BODY(c, slot, tb, xr) computes it from LDS
-// ring slot tb and entries xr into sh.ob[slot]. Wave 0 issues no global memory access.
+// Wave 0 (arithmetic): for each chunk, once wave 3 has loaded it and wave 1 has released the
+// outbox slot, BODY(c, slot, tb, xr, xw) computes it from LDS ring slot tb and entries xr into
+// sh.ob[slot], posting prog after every slot. The exchange entries are tracked per slot through
+// xw: a batch of slots whose entries are all read runs straight through, otherwise each slot
+// waits for its own. Wave 0 issues no global memory access.
+template <typename V>
+struct SwXw {
+  SwLds<V>& sh;
+  PcgScalars* sc;
+  int known;  // slots whose entries wave 2 has read, as last seen
+  // true if the entries of every slot below `need` are in xr (one counter read at most)
+  __device__ __forceinline__ bool all(int need) {
+    if (known < need) known = __builtin_amdgcn_readfirstlane(lds_peek(&sh.xready));
+    return known >= need;
+  }
+  __device__ __forceinline__ void wait(int need) {
+    if (known < need) known = lds_until(&sh.xready, need, sc);
+  }
+  __device__ __forceinline__ void post(int v) { lds_post(&sh.prog, v); }
+};
 template <typename V, typename BODY>
 __device__ __forceinline__ void sw_wave0(SwLds<V>& sh, int nc, PcgScalars* sc, BODY body) {
+  SwXw<V> xw{sh, sc, 0};
   for (int c = 0; c < nc; c++) {
     const int slot = c % SW_K;
     lds_wait(&sh.loaded, c + 1, sc);
-    lds_wait(&sh.xready, c + 1, sc);
     if (c >= SW_K) lds_wait(&sh.freed, c - SW_K + 1, sc);
-    body(c, slot, sh.tb[c % SwCfg<V>::RD], &sh.xr[c % SW_KX][0][0]);
-    lds_put(&sh.done, c + 1);
+    body(c, slot, sh.tb[c % SwCfg<V>::RD], &sh.xr[c % SW_KX][0][0], xw);
   }
 }
 
-// Wave 2: read the exchange entries of each chunk (lane e: entry e of both faces), polling until
-// the producers have written them (bounded by SPIN_LIMIT), into LDS slot c % SW_KX, up to SW_KX
-// chunks ahead of wave 0. Wave 2 issues loads only.
+// Wave 2: read the exchange entries of each chunk (lane e: entry e of both faces, slot e / 8 of
+// the chunk) into LDS slot c % SW_KX, up to SW_KX chunks ahead of wave 0, polling until the
+// producers have written them (bounded by SPIN_LIMIT); xready counts the leading slots whose
+// sixteen entries are all in. Wave 2 issues loads only.
 template <typename V, typename FX>
 __device__ __forceinline__ void sw_wave2(SwLds<V>& sh, int nc, FX ent, PcgScalars* sc, int lane) {
   bool give_up = false;
   for (int c = 0; c < nc; c++) {
-    if (c >= SW_KX && !lds_wait(&sh.done, c - SW_KX + 1, sc)) give_up = true;
+    if (c >= SW_KX && !lds_wait(&sh.prog, (c - SW_KX + 1) * U, sc)) give_up = true;
     const double* p0 = ent(0, c);
     const double* p1 = ent(1, c);
     double v0 = xload(p0), v1 = xload(p1);
+    int pub = 0;
     long spins = 0;
-    while (((__builtin_amdgcn_ballot_w64(xunset(v0) | xunset(v1)) != 0) & !give_up)) {
+    while (true) {
+      const uint64_t miss = __builtin_amdgcn_ballot_w64(xunset(v0) | xunset(v1));
+      const int nv = (give_up | (miss == 0)) ? U : (int)(__builtin_ctzll(miss) >> 3);
+      if (nv > pub) {
+        sh.xr[c % SW_KX][0][lane] = v0;
+        sh.xr[c % SW_KX][1][lane] = v1;
+        lds_post(&sh.xready, c * U + nv);
+        pub = nv;
+      }
+      if (pub == U) break;
       __builtin_amdgcn_s_sleep(1);
       if (xunset(v0)) v0 = xload(p0);
       if (xunset(v1)) v1 = xload(p1);
@@ -954,27 +1014,33 @@ __device__ __forceinline__ void sw_wave2(SwLds<V>& sh, int nc, FX ent, PcgScalar
         }
       }
     }
-    sh.xr[c % SW_KX][0][lane] = v0;
-    sh.xr[c % SW_KX][1][lane] = v1;
-    lds_put(&sh.xready, c + 1);
   }
 }
 
-// Wave 1: for each finished chunk c (in order) read its outbox slot into registers, release
-// the slot, then issue the chunk's stores via STORE(c, regs). Two register sets alternate so
-// that reading chunk c+1 never has to wait for chunk c's stores to have read their data.
+// Wave 1: hands each finished slot's boundary values to the next bricks as soon as wave 0 posts
+// it (PUB(c, v, lo, hi): lanes of slots [lo, hi) of chunk c store v), then, once chunk c is
+// complete, reads its outbox slot into registers, releases the slot and issues the chunk's
+// stores via STORE(c, regs). Two register sets alternate so that reading chunk c+1 never has to
+// wait for chunk c's stores to have read their data.
 template <typename V>
 struct SwOut {
   Vec16<V> d[2][SwCfg<V>::OPS];  // the chunk's one or two output arrays, 16 bytes per lane
-  V bnd[2];                      // this lane's boundary values for the two faces
 };
-template <int NOUT, typename V, typename BND, typename STORE>
-__device__ __forceinline__ void sw_wave1(SwLds<V>& sh, int nc, PcgScalars* sc, int lane, BND bnd, STORE store) {
+template <int NOUT, typename V, typename BND, typename PUB, typename STORE>
+__device__ __forceinline__ void sw_wave1(SwLds<V>& sh, int nc, PcgScalars* sc, int lane, BND bnd, PUB pub,
+                                         STORE store) {
   SwOut<V> o0, o1;
+  int seen = 0;  // slots finished by wave 0, as last read
   auto one = [&](int c, SwOut<V>& o) __attribute__((always_inline)) {
-    lds_wait(&sh.done, c + 1, sc);
     const int slot = c % SW_K;
-    bnd(c, &sh.ob[slot][0][0], o.bnd);
+    for (int lo = 0; lo < U;) {
+      if (seen <= c * U + lo) seen = lds_until(&sh.prog, c * U + lo + 1, sc);
+      const int hi = min(seen - c * U, U);
+      V v[2];
+      bnd(&sh.ob[slot][0][0], v);
+      pub(c, v, lo, hi);
+      lo = hi;
+    }
 #pragma unroll
     for (int k = 0; k < NOUT; k++)
 #pragma unroll
@@ -1013,7 +1079,7 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
     // stays in bounds (the exchange polls then time out and the solve reports an error)
     sh.B = order[tk < (uint64_t)L.nlive ? tk : (uint64_t)(L.nlive - 1)];
     sh.stop = UPDATE && sc->converged != STOP_RUN;
-    sh.done = 0;
+    sh.prog = 0;
     sh.freed = 0;
     sh.xready = 0;
     sh.loaded = 0;
@@ -1039,13 +1105,17 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
     double* o0 = X + (size_t)B * S * 8;  // this brick's f 0 entries (read by its +x neighbour)
     double* o1 = X + face + (size_t)B * S * 8;
     // lane e hands over slot cU + e/8, index e % 8 of each face
-    auto bnd = [&](int, const T* qc, T* v) __attribute__((always_inline)) {
+    auto bnd = [&](const T* qc, T* v) __attribute__((always_inline)) {
       v[0] = qc[(lane >> 3) * 64 + 56 + (lane & 7)];
       v[1] = qc[(lane >> 3) * 64 + (lane & 7) * 8 + 7];
     };
-    sw_wave1<UPDATE ? 2 : 1>(sh, NC, sc, lane, bnd, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
-      xstore(o0 + (size_t)c * U * 8 + lane, (double)o.bnd[0]);  // boundary values first
-      xstore(o1 + (size_t)c * U * 8 + lane, (double)o.bnd[1]);
+    auto pub = [&](int c, const T* v, int lo, int hi) __attribute__((always_inline)) {
+      if ((lane >> 3) >= lo && (lane >> 3) < hi) {
+        xstore(o0 + (size_t)c * U * 8 + lane, (double)v[0]);
+        xstore(o1 + (size_t)c * U * 8 + lane, (double)v[1]);
+      }
+    };
+    sw_wave1<UPDATE ? 2 : 1>(sh, NC, sc, lane, bnd, pub, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
       if (live(0, c)) xstore(const_cast<double*>(ent(0, c)), xsent());  // consumed entries back
       if (live(1, c)) xstore(const_cast<double*>(ent(1, c)), xsent());
 #pragma unroll
@@ -1069,46 +1139,62 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
   const double nalpha = UPDATE ? -sc->alpha : 0.0;
   T qprev = T(0);
   double mx = 0.0, ss = 0.0;
-  sw_wave0(sh, NC, sc, [&](int c, int slot, const T (*tb)[SW_CE], const double* xr) __attribute__((always_inline)) {
+  sw_wave0(sh, NC, sc, [&](int c, int slot, const T (*tb)[SW_CE], const double* xr, SwXw<T>& xw)
+                           __attribute__((always_inline)) {
     // operands of UB slots at a time into registers first, so the chain of the slots waits for
     // LDS only in its lane shuffle
     constexpr int UB = sizeof(T) == 8 ? 4 : 8;
 #pragma unroll
     for (int u0 = 0; u0 < U; u0 += UB) {
-      T rv[UB], tv[UB], lv[UB], dv[UB], bv[UB], ev[UB], x0[UB], x1[UB];
+      T rv[UB], lv[UB], dv[UB], bv[UB], ev[UB];
 #pragma unroll
       for (int k = 0; k < UB; k++) {
-        const int u = u0 + k, i = u * 64 + lane;
-        rv[k] = tb[AR][i];
-        tv[k] = UPDATE ? tb[AT][i] : T(0);
+        const int i = (u0 + k) * 64 + lane;
+        T rr = tb[AR][i];
+        if (UPDATE) {
+          rr = rr + (T)nalpha * tb[AT][i];
+          sh.ob[slot][1][i] = rr;
+        }
+        rv[k] = rr;
         lv[k] = tb[AL][i];
         dv[k] = tb[AD][i];
         bv[k] = tb[AB][i];
         ev[k] = tb[AE][i];
-        x0[k] = (T)xr[u * 8 + b];  // read by every lane: no branch
-        x1[k] = (T)xr[64 + u * 8 + a];
       }
-#pragma unroll
-      for (int k = 0; k < UB; k++) {
-        const int i = (u0 + k) * 64 + lane;
-        T rr = rv[k];
-        if (UPDATE) {
-          rr = rr + (T)nalpha * tv[k];
-          sh.ob[slot][1][i] = rr;
-          const double av = fabs((double)rr);
-          mx = (mx < av) ? av : mx;
-          ss += (double)rr * (double)rr;
-        }
+      auto step = [&](int k, T x0, T x1) __attribute__((always_inline)) {
+        const int u = u0 + k;
         T qL = __shfl_up(qprev, 8, 64);
         T qD = dpp_row<DPP_ROW_SHR1>(qprev);
-        qL = a == 0 ? x0[k] : qL;
-        qD = b == 0 ? x1[k] : qD;
-        T tt = rr - lv[k] * qL;
+        qL = a == 0 ? x0 : qL;
+        qD = b == 0 ? x1 : qD;
+        T tt = rv[k] - lv[k] * qL;
         tt = tt - dv[k] * qD;
         tt = tt - bv[k] * qprev;
         const T qv = tt * ev[k];
-        sh.ob[slot][0][i] = qv;
+        sh.ob[slot][0][u * 64 + lane] = qv;
         qprev = qv;
+        xw.post(c * U + u + 1);
+        if (UPDATE) {  // ||r||_inf and r.r of the updated residual, off the chain
+          const double av = fabs((double)rv[k]);
+          mx = (mx < av) ? av : mx;
+          ss += (double)rv[k] * (double)rv[k];
+        }
+      };
+      if (xw.all(c * U + u0 + UB)) {
+        T x0[UB], x1[UB];  // read by every lane: no branch
+#pragma unroll
+        for (int k = 0; k < UB; k++) {
+          x0[k] = (T)xr[(u0 + k) * 8 + b];
+          x1[k] = (T)xr[64 + (u0 + k) * 8 + a];
+        }
+#pragma unroll
+        for (int k = 0; k < UB; k++) step(k, x0[k], x1[k]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < UB; k++) {
+          xw.wait(c * U + u0 + k + 1);
+          step(k, (T)xr[(u0 + k) * 8 + b], (T)xr[64 + (u0 + k) * 8 + a]);
+        }
       }
     }
   });
@@ -1157,7 +1243,7 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
     const uint64_t tk = atomicAdd((unsigned long long*)ticket, 1ull) - tbase;
     sh.B = order[tk < (uint64_t)L.nlive ? tk : (uint64_t)(L.nlive - 1)];
     sh.stop = !INIT && sc->converged != STOP_RUN;
-    sh.done = 0;
+    sh.prog = 0;
     sh.freed = 0;
     sh.xready = 0;
     sh.loaded = 0;
@@ -1185,15 +1271,19 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
     double* o1 = X + face + (size_t)B * S * 8;
     // lane e hands over slot S-1-cU-e/8, index e % 8 of each face
     const int e0 = (U - 1 - (lane >> 3)) * 64;
-    auto bnd = [&](int, const T* zc, T* v) __attribute__((always_inline)) {
+    auto bnd = [&](const T* zc, T* v) __attribute__((always_inline)) {
       v[0] = zc[e0 + ew];
       v[1] = zc[e0 + ew * 8];
     };
-    sw_wave1<1>(sh, NC, sc, lane, bnd, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
-      const int so = slot_of(c);
-      xstore(o0 + (size_t)so * 8 + ew, (double)o.bnd[0]);  // boundary values first
-      xstore(o1 + (size_t)so * 8 + ew, (double)o.bnd[1]);
-      if (live(0, c)) xstore(const_cast<double*>(ent(0, c)), xsent());
+    auto pub = [&](int c, const T* v, int lo, int hi) __attribute__((always_inline)) {
+      if ((lane >> 3) >= lo && (lane >> 3) < hi) {
+        const int so = slot_of(c);
+        xstore(o0 + (size_t)so * 8 + ew, (double)v[0]);
+        xstore(o1 + (size_t)so * 8 + ew, (double)v[1]);
+      }
+    };
+    sw_wave1<1>(sh, NC, sc, lane, bnd, pub, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
+      if (live(0, c)) xstore(const_cast<double*>(ent(0, c)), xsent());  // consumed entries back
       if (live(1, c)) xstore(const_cast<double*>(ent(1, c)), xsent());
 #pragma unroll
       for (int h = 0; h < SwCfg<T>::OPS; h++)
@@ -1213,37 +1303,52 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
   }
   T zprev = T(0);
   double acc = 0.0;
-  sw_wave0(sh, NC, sc, [&](int c, int slot, const T (*tb)[SW_CE], const double* xr) __attribute__((always_inline)) {
+  sw_wave0(sh, NC, sc, [&](int c, int slot, const T (*tb)[SW_CE], const double* xr, SwXw<T>& xw)
+                           __attribute__((always_inline)) {
     constexpr int UB = sizeof(T) == 8 ? 4 : 8;  // operands of UB slots into registers first
 #pragma unroll
     for (int u0 = 0; u0 < U; u0 += UB) {
-      T qv[UB], v1[UB], v2[UB], v3[UB], ev[UB], rv[UB], x0[UB], x1[UB];
+      T qv[UB], v1[UB], v2[UB], v3[UB], ev[UB], rv[UB];
 #pragma unroll
       for (int k = 0; k < UB; k++) {
-        const int u = u0 + k, i = (U - 1 - u) * 64 + lane;
+        const int i = (U - 1 - (u0 + k)) * 64 + lane;
         qv[k] = tb[AQ][i];
         v1[k] = tb[A1][i];
         v2[k] = tb[A2][i];
         v3[k] = tb[A3][i];
         ev[k] = tb[AE][i];
         rv[k] = tb[AR][i];
-        x0[k] = (T)xr[u * 8 + b];  // read by every lane: no branch
-        x1[k] = (T)xr[64 + u * 8 + a];
       }
-#pragma unroll
-      for (int k = 0; k < UB; k++) {
-        const int i = (U - 1 - (u0 + k)) * 64 + lane;
+      auto step = [&](int k, T x0, T x1) __attribute__((always_inline)) {
+        const int u = u0 + k;
         T zR = __shfl_down(zprev, 8, 64);
         T zU = dpp_row<DPP_ROW_SHL1>(zprev);
-        zR = a == 7 ? x0[k] : zR;
-        zU = b == 7 ? x1[k] : zU;
+        zR = a == 7 ? x0 : zR;
+        zU = b == 7 ? x1 : zU;
         T tt = qv[k] - v1[k] * zR;
         tt = tt - v2[k] * zU;
         tt = tt - v3[k] * zprev;
         const T zv = tt * ev[k];
-        sh.ob[slot][0][i] = zv;
+        sh.ob[slot][0][(U - 1 - u) * 64 + lane] = zv;
         zprev = zv;
+        xw.post(c * U + u + 1);
         acc += (double)zv * (double)rv[k];
+      };
+      if (xw.all(c * U + u0 + UB)) {
+        T x0[UB], x1[UB];  // read by every lane: no branch
+#pragma unroll
+        for (int k = 0; k < UB; k++) {
+          x0[k] = (T)xr[(u0 + k) * 8 + b];
+          x1[k] = (T)xr[64 + (u0 + k) * 8 + a];
+        }
+#pragma unroll
+        for (int k = 0; k < UB; k++) step(k, x0[k], x1[k]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < UB; k++) {
+          xw.wait(c * U + u0 + k + 1);
+          step(k, (T)xr[(u0 + k) * 8 + b], (T)xr[64 + (u0 + k) * 8 + a]);
+        }
       }
     }
   });
