@@ -51,7 +51,6 @@ struct PcgScalars {
   int32_t conv_iter;
   int32_t status;
   int32_t wait_timeouts;
-  uint32_t done_ctr[8];  // last-block counters, one per kernel kind (reset by the last block)
 };
 
 struct PcgState {

@@ -33,7 +33,6 @@ constexpr int TF_DO = 1, TF_GL = 2, TF_L1 = 4, TF_L2 = 8, TF_GD = 16, TF_D1 = 32
               TF_B1 = 256, TF_B2 = 512;
 constexpr long SPIN_LIMIT = 1000000;    // bounded wait (~1 s), then the solve reports a device error
 enum { STOP_RUN = 0, STOP_CONVERGED = 1, STOP_ABORT = 2 };
-enum { DC_SPMV = 0, DC_FWD = 1, DC_BWD = 2, DC_INIT = 3, DC_FACTOR = 4, DC_RES = 5 };
 
 struct Lay {
   int nx, ny, nz, nbi, nbj, S, nlive;
@@ -124,50 +123,6 @@ __device__ inline double block_max256(double v, double* sm) {
   __syncthreads();
   return r;
 }
-// Fixed tree over n partials with a 256-thread block: thread t sums p[t + 256m] for m
-// ascending, then block_sum256.
-__device__ inline double tree256(const double* p, int n, double* sm) {
-  double a = 0.0;
-  for (int m = threadIdx.x; m < n; m += 256) a += p[m];
-  return block_sum256(a, sm);
-}
-// The same tree evaluated by one 64-lane wave (lane l plays threads l, l+64, l+128, l+192).
-__device__ inline double tree256_wave(const double* p, int n) {
-  double w[4];
-#pragma unroll
-  for (int v = 0; v < 4; v++) {
-    double a = 0.0;
-    for (int m = threadIdx.x + 64 * v; m < n; m += 256) a += p[m];
-    w[v] = wave_sum(a);
-  }
-  return (w[0] + w[1]) + (w[2] + w[3]);
-}
-__device__ inline double max_wave(const double* p, int n) {
-  double m = 0.0;
-  for (int q = threadIdx.x; q < n; q += 64) m = (m < p[q]) ? p[q] : m;
-  return wave_max(m);
-}
-
-// Last-block election: returns true in exactly one block (all threads), after every block
-// has published its partial.
-__device__ inline bool last_block(uint32_t* ctr, uint32_t nblocks, int nthreads) {
-  __shared__ int is_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    uint32_t old = atomicAdd(ctr, 1u);
-    is_last = (old == nblocks - 1) ? 1 : 0;
-    if (is_last) {
-      __threadfence();
-      atomicExch(ctr, 0u);
-    }
-  }
-  __syncthreads();
-  (void)nthreads;
-  if (is_last) __threadfence();
-  return is_last != 0;
-}
-
 // ---- boundary exchange between bricks ---------------------------------------------------------
 // A brick hands its boundary row to the next brick through a small exchange array. Every entry
 // holds a sentinel until the producer stores the value with an agent-scope atomic store; the
@@ -219,6 +174,96 @@ __device__ inline double xpoll(const double* p, double v, PcgScalars* sc) {
     }
   }
   return v;
+}
+
+// High word of the sentinel. Arithmetic never returns a signalling NaN, so for computed values
+// the high word alone identifies the sentinel.
+constexpr uint32_t XSENT_HI = (uint32_t)(XSENT >> 32);
+__device__ __forceinline__ bool xunset(double v) { return (uint32_t)__double2hiint(v) == XSENT_HI; }
+
+// ---- cross-block reductions --------------------------------------------------------------------
+// Every block (in the sweeps: every brick) stores its partials with agent-scope atomic stores into
+// entries that otherwise hold the sentinel, so a stored partial is its own ready flag and no
+// fence is needed: an agent-scope fence writes back and invalidates the L2 on this GPU, which,
+// done by every block, costs more than the kernel's own work. The block dispatched last (the
+// brick with the last ticket) waits until every entry is set, reduces them in the fixed order and
+// stores the sentinel back, so the entries are unset again for the next kernel.
+
+// Block-wide (256 threads) wait until p[0, n) are all set; false if the wait gave up.
+__device__ inline bool gather_wait256(const double* p, int n, PcgScalars* sc) {
+  long spins = 0;
+  while (true) {
+    int mine = 1;
+    for (int m = threadIdx.x; m < n; m += 256) mine &= xunset(xload(p + m)) ? 0 : 1;
+    if (__syncthreads_and(mine)) return true;
+    __builtin_amdgcn_s_sleep(2);
+    int give = 0;
+    if ((++spins & 63) == 0 && threadIdx.x == 0) {
+      const int to = __hip_atomic_load(&sc->wait_timeouts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      give = (to != 0 || spins > SPIN_LIMIT) ? 1 : 0;
+      if (give) atomicAdd(&sc->wait_timeouts, 1);
+    }
+    if (__syncthreads_or(give)) return false;
+  }
+}
+// Fixed tree over the n gathered entries (thread t sums p[t + 256m] for m ascending, then
+// block_sum256), which it returns to the sentinel.
+__device__ inline double gather_tree256(double* p, int n, double* sm) {
+  double a = 0.0;
+  for (int m = threadIdx.x; m < n; m += 256) {
+    a += xload(p + m);
+    xstore(p + m, xsent());
+  }
+  return block_sum256(a, sm);
+}
+// max over gathered entries (order-free: the comparison never takes a NaN), returned to the sentinel
+__device__ inline double gather_max256(double* p, int n, double* sm) {
+  double mx = 0.0;
+  for (int m = threadIdx.x; m < n; m += 256) {
+    const double v = xload(p + m);
+    mx = (mx < v) ? v : mx;
+    xstore(p + m, xsent());
+  }
+  return block_max256(mx, sm);
+}
+// One wave's wait until p[0, n) are all set; false if the wait gave up.
+__device__ inline bool gather_wait_wave(const double* p, int n, PcgScalars* sc) {
+  const int lane = threadIdx.x & 63;
+  long spins = 0;
+  while (true) {
+    bool mine = true;
+    for (int m = lane; m < n; m += 64) mine &= !xunset(xload(p + m));
+    if (__builtin_amdgcn_ballot_w64(!mine) == 0) return true;
+    __builtin_amdgcn_s_sleep(2);
+    if ((++spins & 63) == 0) {
+      const int to = __hip_atomic_load(&sc->wait_timeouts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (to != 0 || spins > SPIN_LIMIT) {
+        if (lane == 0) atomicAdd(&sc->wait_timeouts, 1);
+        return false;
+      }
+    }
+  }
+}
+// The same tree evaluated by one 64-lane wave (lane l plays threads l, l+64, l+128, l+192).
+__device__ inline double gather_tree256_wave(double* p, int n) {
+  double w[4];
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    double a = 0.0;
+    for (int m = (threadIdx.x & 63) + 64 * v; m < n; m += 256) a += xload(p + m);
+    w[v] = wave_sum(a);
+  }
+  for (int m = threadIdx.x & 63; m < n; m += 64) xstore(p + m, xsent());
+  return (w[0] + w[1]) + (w[2] + w[3]);
+}
+__device__ inline double gather_max_wave(double* p, int n) {
+  double mx = 0.0;
+  for (int m = threadIdx.x & 63; m < n; m += 64) {
+    const double v = xload(p + m);
+    mx = (mx < v) ? v : mx;
+    xstore(p + m, xsent());
+  }
+  return wave_max(mx);
 }
 
 __global__ void k_fill_sentinel(int64_t n, double* __restrict__ x) {
@@ -481,13 +526,14 @@ __global__ __launch_bounds__(256) void k_init(Lay L, const double* __restrict__ 
   double bm = block_max256(mx, sm);
   double bs = block_sum256(ss, sm);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = bs;
-    part2[blockIdx.x] = bm;
+    xstore(part + blockIdx.x, bs);
+    xstore(part2 + blockIdx.x, bm);
   }
-  if (last_block(&sc->done_ctr[DC_INIT], gridDim.x, 256)) {
-    double tot = tree256(part, nchunks, sm);
-    double mxx = 0.0;
-    for (int q = 0; q < nchunks; q++) mxx = (mxx < part2[q]) ? part2[q] : mxx;
+  if (blockIdx.x == gridDim.x - 1) {
+    gather_wait256(part, nchunks, sc);
+    gather_wait256(part2, nchunks, sc);
+    const double tot = gather_tree256(part, nchunks, sm);
+    const double mxx = gather_max256(part2, nchunks, sm);
     if (threadIdx.x == 0) {
       sc->bnorm2sq = tot;
       sc->rnorm2sq = tot;
@@ -534,6 +580,37 @@ __device__ inline int nb_up(const Lay& L, const Group& g, int e, int s, int a, i
   return b < 7 ? e + 65 : (g.BU * L.S + s - 7) * 64 + a * 8;
 }
 
+// Byte offsets of the four cross-brick neighbours of a brick's face lanes, relative to the
+// lane's own element (see nb_left .. nb_up): a = 0 lanes read the -x brick at slot s+7, lane 56+b;
+// a = 7 lanes the +x brick at slot s-7, lane b; b = 0 / 7 lanes the -y / +y brick at slot s+7 /
+// s-7, lanes 8a+7 / 8a. An absent brick gives 0 (its flags are never set).
+struct FaceOff {
+  uint32_t dL, dR, dD, dU;
+};
+template <typename S>
+__device__ __forceinline__ FaceOff face_off(const Lay& L, int B) {
+  const int SL = L.S * 64, sz = (int)sizeof(S);
+  const int2 ij = L.bij[B];
+  const int BL = ij.x > 0 ? L.live[(ij.x - 1) * L.nbj + ij.y] : -1;
+  const int BR = ij.x + 1 < L.nbi ? L.live[(ij.x + 1) * L.nbj + ij.y] : -1;
+  const int BD = ij.y > 0 ? L.live[ij.x * L.nbj + ij.y - 1] : -1;
+  const int BU = ij.y + 1 < L.nbj ? L.live[ij.x * L.nbj + ij.y + 1] : -1;
+  // byte offsets modulo 2^32: added to the lane's own offset they wrap to the neighbour's
+  auto off = [&](int NB, int d) { return NB >= 0 ? (uint32_t)(((int64_t)(NB - B) * SL + d) * sz) : 0u; };
+  FaceOff f;
+  f.dL = off(BL, 7 * 64 + 56);
+  f.dR = off(BR, -7 * 64 - 56);
+  f.dD = off(BD, 7 * 64 + 7);
+  f.dU = off(BU, -7 * 64 - 7);
+  return f;
+}
+// element at byte offset `off` of a brick array (an unsigned 32-bit offset from a scalar base,
+// so the address needs no 64-bit arithmetic; layouts are limited to 2^32 bytes per array)
+template <typename T>
+__device__ __forceinline__ T ld_at(const T* __restrict__ base, uint32_t off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+
 // z = A s with the fixed-tree s.z (applyCompressedMatrix + dotProduct(s, z)); last block
 // forms alpha = sigma / s.z.
 // S is the storage and arithmetic type of the vectors and the stencil; the dot product is
@@ -546,69 +623,68 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const S* __restrict__ x, co
                                               PcgScalars* sc, int iter) {
   __shared__ double sm[4];
   if (PCG && sc->converged != STOP_RUN) return;
-  // A thread's 16 elements (one per group) are independent until the dot product, which adds
-  // them in order: their flags are loaded first, then every operand of MB elements at a time,
-  // unconditionally (an absent neighbour reads the element itself and is discarded by a
-  // select, so the sums keep their order), so the loads overlap instead of waiting one
-  // memory latency per operand and element. Groups past the end reread the last group and are
-  // dropped.
-  constexpr int MB = 4;
-  const int ne = (int)L.ne, chunk = blockIdx.x;
+  // A thread's 16 elements (one per group of 256) are independent until the dot product, which
+  // adds them in order. Every neighbour is a byte offset from the element's own: a constant
+  // inside the brick, a per-brick constant (FaceOff) across a face, zero when the flag is clear
+  // (the element itself is read and the term dropped by a select, so the sums keep their
+  // order). The operands of MB elements are loaded before any of their arithmetic, so the loads
+  // overlap. Groups past the end reread the last group and are dropped.
+  constexpr int MB = 4, sz = (int)sizeof(S);
+  const int ne = (int)L.ne, chunk = blockIdx.x, tid = threadIdx.x;
+  const int lane = tid & 63, a = lane >> 3, b = lane & 7;
   uint8_t fl[16];
 #pragma unroll
-  for (int m = 0; m < 16; m++) fl[m] = nbm[min(chunk * CH + m * 256, ne - 256) + (int)threadIdx.x];
-  // with S >= 64 a chunk (64 slots) spans at most two bricks, whose neighbour bricks are looked
-  // up once; smaller layouts look them up per group
+  for (int m = 0; m < 16; m++) fl[m] = nbm[min(chunk * CH + m * 256, ne - 256) + tid];
+  // with S >= 64 a chunk (64 slots) spans at most two bricks, whose faces are looked up once;
+  // smaller layouts look them up per group
   const int SL = L.S * 64;
   const bool two = SL >= CH;
-  const Group ga = group_of(L, min(chunk * CH, ne - 256)), gb = group_of(L, min(chunk * CH + CH - 256, ne - 256));
+  const int B0 = min(chunk * CH, ne - 256) / SL;
+  const FaceOff f0 = face_off<S>(L, B0), f1 = face_off<S>(L, min(B0 + 1, L.nlive - 1));
   double acc = 0.0;
 #pragma unroll
   for (int m0 = 0; m0 < 16; m0 += MB) {
-    int e[MB], iL[MB], iD[MB], iB[MB], iR[MB], iU[MB], iF[MB];
+    uint32_t e4[MB], oL[MB], oD[MB], oB[MB], oR[MB], oU[MB], oF[MB];
     bool in[MB];
 #pragma unroll
     for (int q = 0; q < MB; q++) {
       const int raw = chunk * CH + (m0 + q) * 256;
       in[q] = raw < ne;
       const int base = in[q] ? raw : ne - 256;
-      Group g;
+      FaceOff g;
       if (two) {
-        g = base < (ga.B + 1) * SL ? ga : gb;
-        g.s0 = (base - g.B * SL) >> 6;
+        g = base < (B0 + 1) * SL ? f0 : f1;
       } else {
-        g = group_of(L, base);
+        g = face_off<S>(L, base / SL);
       }
-      const int ee = base + (int)threadIdx.x, lane = threadIdx.x & 63, s = g.s0 + (threadIdx.x >> 6);
-      const int a = lane >> 3, b = lane & 7, f = fl[m0 + q];
-      e[q] = ee;
-      iL[q] = (f & NB_L) ? nb_left(L, g, ee, s, a, b) : ee;
-      iD[q] = (f & NB_D) ? nb_down(L, g, ee, s, a, b) : ee;
-      iB[q] = (f & NB_B) ? ee - 64 : ee;
-      iR[q] = (f & NB_R) ? nb_right(L, g, ee, s, a, b) : ee;
-      iU[q] = (f & NB_U) ? nb_up(L, g, ee, s, a, b) : ee;
-      iF[q] = (f & NB_F) ? ee + 64 : ee;
+      const int f = fl[m0 + q];
+      const uint32_t e = (uint32_t)(base + tid) * sz;
+      e4[q] = e;
+      oL[q] = (f & NB_L) ? e + (a > 0 ? (uint32_t)(-72 * sz) : g.dL) : e;
+      oD[q] = (f & NB_D) ? e + (b > 0 ? (uint32_t)(-65 * sz) : g.dD) : e;
+      oB[q] = (f & NB_B) ? e - 64u * sz : e;
+      oR[q] = (f & NB_R) ? e + (a < 7 ? 72u * sz : g.dR) : e;
+      oU[q] = (f & NB_U) ? e + (b < 7 ? 65u * sz : g.dU) : e;
+      oF[q] = (f & NB_F) ? e + 64u * sz : e;
     }
-    // all loads of the MB elements first (written as a separate loop, the compiler issues them
-    // before any arithmetic instead of waiting for each element's operands in turn)
     S xc[MB], dg[MB], cL[MB], cD[MB], cB[MB], cR[MB], cU[MB], cF[MB], xL[MB], xD[MB], xB[MB], xR[MB], xU[MB],
         xF[MB], v[MB];
 #pragma unroll
     for (int q = 0; q < MB; q++) {
-      xc[q] = x[e[q]];
-      dg[q] = adiag[e[q]];
-      cL[q] = al[e[q]];
-      cD[q] = ad[e[q]];
-      cB[q] = ab[e[q]];
-      cR[q] = al[iR[q]];
-      cU[q] = ad[iU[q]];
-      cF[q] = ab[iF[q]];
-      xL[q] = x[iL[q]];
-      xD[q] = x[iD[q]];
-      xB[q] = x[iB[q]];
-      xR[q] = x[iR[q]];
-      xU[q] = x[iU[q]];
-      xF[q] = x[iF[q]];
+      xc[q] = ld_at(x, e4[q]);
+      dg[q] = ld_at(adiag, e4[q]);
+      cL[q] = ld_at(al, e4[q]);
+      cD[q] = ld_at(ad, e4[q]);
+      cB[q] = ld_at(ab, e4[q]);
+      cR[q] = ld_at(al, oR[q]);
+      cU[q] = ld_at(ad, oU[q]);
+      cF[q] = ld_at(ab, oF[q]);
+      xL[q] = ld_at(x, oL[q]);
+      xD[q] = ld_at(x, oD[q]);
+      xB[q] = ld_at(x, oB[q]);
+      xR[q] = ld_at(x, oR[q]);
+      xU[q] = ld_at(x, oU[q]);
+      xF[q] = ld_at(x, oF[q]);
     }
 #pragma unroll
     for (int q = 0; q < MB; q++) {
@@ -625,16 +701,17 @@ __global__ __launch_bounds__(256) void k_spmv(Lay L, const S* __restrict__ x, co
 #pragma unroll
     for (int q = 0; q < MB; q++) {
       if (in[q]) {
-        y[e[q]] = v[q];
+        *reinterpret_cast<S*>(reinterpret_cast<char*>(y) + e4[q]) = v[q];
         acc += (double)xc[q] * (double)v[q];
       }
     }
   }
   if (!PCG) return;
   double bs = block_sum256(acc, sm);
-  if (threadIdx.x == 0) part[chunk] = bs;
-  if (last_block(&sc->done_ctr[DC_SPMV], gridDim.x, 256)) {
-    double sdotz = tree256(part, nchunks, sm);
+  if (threadIdx.x == 0) xstore(part + chunk, bs);
+  if (chunk == (int)gridDim.x - 1) {
+    gather_wait256(part, nchunks, sc);
+    const double sdotz = gather_tree256(part, nchunks, sm);
     if (threadIdx.x == 0) {
       sc->sdotz = sdotz;
       if (sdotz == 0.0) {
@@ -692,13 +769,14 @@ __global__ __launch_bounds__(256) void k_resid(Lay L, const double* __restrict__
   double bm = block_max256(mx, sm);
   double bs = block_sum256(ss, sm);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = bs;
-    part2[blockIdx.x] = bm;
+    xstore(part + blockIdx.x, bs);
+    xstore(part2 + blockIdx.x, bm);
   }
-  if (last_block(&sc->done_ctr[DC_INIT], gridDim.x, 256)) {
-    double tot = tree256(part, nchunks, sm);
-    double mxx = 0.0;
-    for (int q = 0; q < nchunks; q++) mxx = (mxx < part2[q]) ? part2[q] : mxx;
+  if (blockIdx.x == gridDim.x - 1) {
+    gather_wait256(part, nchunks, sc);
+    gather_wait256(part2, nchunks, sc);
+    const double tot = gather_tree256(part, nchunks, sm);
+    const double mxx = gather_max256(part2, nchunks, sm);
     if (threadIdx.x == 0) {
       sc->residual = mxx;
       sc->rnorm2sq = tot;
@@ -743,10 +821,7 @@ __global__ __launch_bounds__(256) void k_resid(Lay L, const double* __restrict__
 // Exchange entries hold a sentinel until the producer stores the value (agent-scope atomic
 // stores and loads); the consumer uses the value and stores the sentinel back.
 
-// High word of the sentinel. Arithmetic never returns a signalling NaN, so for computed values
-// the high word alone identifies the sentinel.
-constexpr uint32_t XSENT_HI = (uint32_t)(XSENT >> 32);
-__device__ __forceinline__ bool xunset(double v) { return (uint32_t)__double2hiint(v) == XSENT_HI; }
+
 
 // v of lane-1 / lane+1 inside 16-lane rows (DPP row_shr:1 / row_shl:1); a lane whose source is
 // outside its row gets 0. Used for the b-1 / b+1 neighbour, which lanes b = 0 / 7 never take.
@@ -794,6 +869,7 @@ struct SwLds {
   int prog, freed, xready, loaded, B, stop;  // slots finished by wave 0 / chunks released by wave 1 /
                                              // slots whose entries wave 2 has read / chunks loaded
                                              // by wave 3
+  int last;                                  // this brick holds the last ticket (it reduces)
 };
 
 __device__ __forceinline__ int lds_get(const int* p) {
@@ -849,26 +925,6 @@ __device__ __forceinline__ int lds_until(const int* p, int v, PcgScalars* sc) {
     }
   }
   return x;
-}
-
-// last_block for a single wave: true in exactly one wave of the launch, after every wave has
-// published its partial.
-__device__ inline bool last_wave(uint32_t* ctr, uint32_t nwaves) {
-  uint32_t old = 0;
-  if ((threadIdx.x & 63) == 0) {
-    __threadfence();
-    old = atomicAdd(ctr, 1u);
-  }
-  old = __shfl(old, 0, 64);
-  const bool is_last = old == nwaves - 1;
-  if (is_last) {
-    if ((threadIdx.x & 63) == 0) {
-      __threadfence();
-      atomicExch(ctr, 0u);
-    }
-    __threadfence();
-  }
-  return is_last;
 }
 
 // Element offset of chunk c in a brick array: forward chunks run up from slot 0, backward
@@ -1078,6 +1134,7 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
     // a ticket outside [0, n) can only follow a failed launch; clamp it so every access
     // stays in bounds (the exchange polls then time out and the solve reports an error)
     sh.B = order[tk < (uint64_t)L.nlive ? tk : (uint64_t)(L.nlive - 1)];
+    sh.last = tk == (uint64_t)(L.nlive - 1);
     sh.stop = UPDATE && sc->converged != STOP_RUN;
     sh.prog = 0;
     sh.freed = 0;
@@ -1202,12 +1259,14 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
     double wm = wave_max(mx);
     double wsum = wave_sum(ss);
     if (lane == 0) {
-      part2[B] = wm;
-      part[B] = wsum;
+      xstore(part2 + B, wm);
+      xstore(part + B, wsum);
     }
-    if (last_wave(&sc->done_ctr[DC_FWD], L.nlive)) {
-      double res = max_wave(part2, L.nlive);
-      double rr2 = tree256_wave(part, L.nlive);
+    if (__builtin_amdgcn_readfirstlane(sh.last)) {
+      gather_wait_wave(part2, L.nlive, sc);
+      gather_wait_wave(part, L.nlive, sc);
+      const double res = gather_max_wave(part2, L.nlive);
+      const double rr2 = gather_tree256_wave(part, L.nlive);
       if (lane == 0) {
         sc->residual = res;
         sc->rnorm2sq = rr2;
@@ -1242,6 +1301,7 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
   if (threadIdx.x == 0) {
     const uint64_t tk = atomicAdd((unsigned long long*)ticket, 1ull) - tbase;
     sh.B = order[tk < (uint64_t)L.nlive ? tk : (uint64_t)(L.nlive - 1)];
+    sh.last = tk == (uint64_t)(L.nlive - 1);
     sh.stop = !INIT && sc->converged != STOP_RUN;
     sh.prog = 0;
     sh.freed = 0;
@@ -1353,9 +1413,10 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
     }
   });
   double ws = wave_sum(acc);
-  if (lane == 0) part[B] = ws;
-  if (last_wave(&sc->done_ctr[DC_BWD], L.nlive)) {
-    double sig = tree256_wave(part, L.nlive);
+  if (lane == 0) xstore(part + B, ws);
+  if (__builtin_amdgcn_readfirstlane(sh.last)) {
+    gather_wait_wave(part, L.nlive, sc);
+    const double sig = gather_tree256_wave(part, L.nlive);
     if (lane == 0) {
       if (INIT) {
         sc->sigma = sig;
@@ -1476,8 +1537,8 @@ int pcg_build(fs_ctx* c) {
       }
   P.nlive = (int)bij.size();
   P.ne = (int64_t)P.nlive * P.S * 64;
-  if (P.ne >= ((int64_t)1 << 31)) {
-    c->set_error("pressure system too large for 32-bit element indexing");
+  if (P.ne * (int64_t)sizeof(double) >= ((int64_t)1 << 32)) {
+    c->set_error("pressure system too large for 32-bit byte offsets");
     return FS_BAD_ARG;
   }
   P.nchunks = (int)((P.ne + CH - 1) / CH);
@@ -1522,6 +1583,9 @@ int pcg_build(fs_ctx* c) {
   size_t np_ = std::max<size_t>(std::max(P.nchunks, P.nlive), 1);
   FS_TRY(realloc_if(c, P.partials, np_ * sizeof(double)));
   FS_TRY(realloc_if(c, P.partials2, np_ * sizeof(double)));
+  // reduction entries start unset (every reduction returns the entries it used to the sentinel)
+  k_fill_sentinel<<<fs_blocks((int64_t)np_, 256), 256, 0, c->stream>>>((int64_t)np_, P.partials.as<double>());
+  k_fill_sentinel<<<fs_blocks((int64_t)np_, 256), 256, 0, c->stream>>>((int64_t)np_, P.partials2.as<double>());
   if (P.nlive > 0) {
     Lay L = lay_of(c);
     // Aneighbour order Left, Right, Up, Down, Front, Back. The packed system is FP64 (exact
