@@ -974,10 +974,10 @@ __device__ __forceinline__ void sw_store(V* dst, const V* lds, int lane) {
 
 // Wave 3 (loads): keeps SwCfg::R chunks of array loads in flight (register sets named rather
 // than indexed so that all stay in registers) and moves each loaded chunk to LDS ring slot
-// c % RD once wave 0 is done with the chunk that used it. Wave 3 issues loads only, so the waits
-// before the moves are exact. Whole groups of R chunks run as straight-line code (an exit
-// inside would merge paths and cost exact waits).
-template <int NARR, typename V, typename RUN>
+// c % RD once wave 0 (GUARD: and wave 1) is done with the chunk that used it. Wave 3 issues loads
+// only, so the waits before the moves are exact. Whole groups of R chunks run as straight-line
+// code (an exit inside would merge paths and cost exact waits).
+template <int NARR, bool GUARD, typename V, typename RUN>
 __device__ __forceinline__ void sw_wave3(SwLds<V>& sh, const V* const* src, int nc, int S, RUN run, PcgScalars* sc,
                                          int lane) {
   constexpr int R = SwCfg<V>::R, RD = SwCfg<V>::RD;
@@ -988,6 +988,7 @@ __device__ __forceinline__ void sw_wave3(SwLds<V>& sh, const V* const* src, int 
   if (R >= 4) sw_load<NARR>(s3, src, nc, 3, lane, run, S);
   auto chunk = [&](int c, SwSet<V>& cur) __attribute__((always_inline)) {
     if (c >= RD) lds_until(&sh.prog, (c - RD + 1) * U, sc);
+    if (GUARD && c >= RD) lds_wait(&sh.freed, c - RD + 1, sc);  // wave 1 has read the ring slot too
     sw_spill<NARR>(sh.tb[c % RD], cur, lane);
     lds_put(&sh.loaded, c + 1);
     sw_load<NARR>(cur, src, nc, c + R, lane, run, S);
@@ -1075,16 +1076,17 @@ __device__ __forceinline__ void sw_wave2(SwLds<V>& sh, int nc, FX ent, PcgScalar
 
 // Wave 1: hands each finished slot's boundary values to the next bricks as soon as wave 0 posts
 // it (PUB(c, v, lo, hi): lanes of slots [lo, hi) of chunk c store v), then, once chunk c is
-// complete, reads its outbox slot into registers, releases the slot and issues the chunk's
+// complete, reads its outbox slot into registers, folds the chunk into the sweep's reductions
+// (FOLD(c, outbox), off the arithmetic wave's chain), releases the slot and issues the chunk's
 // stores via STORE(c, regs). Two register sets alternate so that reading chunk c+1 never has to
 // wait for chunk c's stores to have read their data.
 template <typename V>
 struct SwOut {
   Vec16<V> d[2][SwCfg<V>::OPS];  // the chunk's one or two output arrays, 16 bytes per lane
 };
-template <int NOUT, typename V, typename BND, typename PUB, typename STORE>
+template <int NOUT, typename V, typename BND, typename PUB, typename FOLD, typename STORE>
 __device__ __forceinline__ void sw_wave1(SwLds<V>& sh, int nc, PcgScalars* sc, int lane, BND bnd, PUB pub,
-                                         STORE store) {
+                                         FOLD fold, STORE store) {
   SwOut<V> o0, o1;
   int seen = 0;  // slots finished by wave 0, as last read
   auto one = [&](int c, SwOut<V>& o) __attribute__((always_inline)) {
@@ -1103,6 +1105,7 @@ __device__ __forceinline__ void sw_wave1(SwLds<V>& sh, int nc, PcgScalars* sc, i
       for (int h = 0; h < SwCfg<V>::OPS; h++)
         o.d[k][h] = *reinterpret_cast<const Vec16<V>*>(&sh.ob[slot][k][h * (1024 / (int)sizeof(V)) +
                                                                          lane * SwCfg<V>::PER16]);
+    fold(c, &sh.ob[slot][0][0]);  // per-chunk reductions, before the slot is released
     lds_put(&sh.freed, c + 1);
     store(c, o);
   };
@@ -1172,7 +1175,20 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
         xstore(o1 + (size_t)c * U * 8 + lane, (double)v[1]);
       }
     };
-    sw_wave1<UPDATE ? 2 : 1>(sh, NC, sc, lane, bnd, pub, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
+    // UPDATE: ||r||_inf and r.r of the updated residual (outbox 1), per lane in slot order
+    double mx = 0.0, ss = 0.0;
+    auto fold = [&](int, const T* ob) __attribute__((always_inline)) {
+      if (UPDATE) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const double rr = (double)ob[SW_CE + u * 64 + lane];
+          const double av = fabs(rr);
+          mx = (mx < av) ? av : mx;
+          ss += rr * rr;
+        }
+      }
+    };
+    sw_wave1<UPDATE ? 2 : 1>(sh, NC, sc, lane, bnd, pub, fold, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
       if (live(0, c)) xstore(const_cast<double*>(ent(0, c)), xsent());  // consumed entries back
       if (live(1, c)) xstore(const_cast<double*>(ent(1, c)), xsent());
 #pragma unroll
@@ -1182,6 +1198,33 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
           *reinterpret_cast<Vec16<T>*>((k == 0 ? q : r) + brick + RunFwd{}(c, S) + h * (1024 / (int)sizeof(T)) +
                                        lane * SwCfg<T>::PER16) = o.d[k][h];
     });
+    if (UPDATE) {
+      const double wm = wave_max(mx);
+      const double wsum = wave_sum(ss);
+      if (lane == 0) {
+        xstore(part2 + B, wm);
+        xstore(part + B, wsum);
+      }
+      if (__builtin_amdgcn_readfirstlane(sh.last)) {
+        gather_wait_wave(part2, L.nlive, sc);
+        gather_wait_wave(part, L.nlive, sc);
+        const double res = gather_max_wave(part2, L.nlive);
+        const double rr2 = gather_tree256_wave(part, L.nlive);
+        if (lane == 0) {
+          sc->residual = res;
+          sc->rnorm2sq = rr2;
+          bool conv = tol_mode == FS_TOL_REL_L2 ? (sqrt(rr2) <= tol * sqrt(sc->bnorm2sq)) : (res < tol);
+          if (!(res <= 1.7976931348623157e308)) {
+            sc->converged = STOP_ABORT;
+            sc->status = FS_NONFINITE;
+            sc->conv_iter = iter;
+          } else if (conv) {
+            sc->converged = STOP_CONVERGED;
+            sc->conv_iter = iter;
+          }
+        }
+      }
+    }
     return;
   }
   if (wave == 2) {  // exchange entries
@@ -1190,12 +1233,11 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
   }
   if (wave == 3) {  // array loads
     const T* src[SW_NA] = {r + brick, cl + brick, cd + brick, cb + brick, E + brick, t + brick};
-    sw_wave3<NARR>(sh, src, NC, S, RunFwd{}, sc, lane);
+    sw_wave3<NARR, false>(sh, src, NC, S, RunFwd{}, sc, lane);
     return;
   }
   const double nalpha = UPDATE ? -sc->alpha : 0.0;
   T qprev = T(0);
-  double mx = 0.0, ss = 0.0;
   sw_wave0(sh, NC, sc, [&](int c, int slot, const T (*tb)[SW_CE], const double* xr, SwXw<T>& xw)
                            __attribute__((always_inline)) {
     // operands of UB slots at a time into registers first, so the chain of the slots waits for
@@ -1231,11 +1273,6 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
         sh.ob[slot][0][u * 64 + lane] = qv;
         qprev = qv;
         xw.post(c * U + u + 1);
-        if (UPDATE) {  // ||r||_inf and r.r of the updated residual, off the chain
-          const double av = fabs((double)rv[k]);
-          mx = (mx < av) ? av : mx;
-          ss += (double)rv[k] * (double)rv[k];
-        }
       };
       if (xw.all(c * U + u0 + UB)) {
         T x0[UB], x1[UB];  // read by every lane: no branch
@@ -1255,33 +1292,6 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
       }
     }
   });
-  if (UPDATE) {
-    double wm = wave_max(mx);
-    double wsum = wave_sum(ss);
-    if (lane == 0) {
-      xstore(part2 + B, wm);
-      xstore(part + B, wsum);
-    }
-    if (__builtin_amdgcn_readfirstlane(sh.last)) {
-      gather_wait_wave(part2, L.nlive, sc);
-      gather_wait_wave(part, L.nlive, sc);
-      const double res = gather_max_wave(part2, L.nlive);
-      const double rr2 = gather_tree256_wave(part, L.nlive);
-      if (lane == 0) {
-        sc->residual = res;
-        sc->rnorm2sq = rr2;
-        bool conv = tol_mode == FS_TOL_REL_L2 ? (sqrt(rr2) <= tol * sqrt(sc->bnorm2sq)) : (res < tol);
-        if (!(res <= 1.7976931348623157e308)) {
-          sc->converged = STOP_ABORT;
-          sc->status = FS_NONFINITE;
-          sc->conv_iter = iter;
-        } else if (conv) {
-          sc->converged = STOP_CONVERGED;
-          sc->conv_iter = iter;
-        }
-      }
-    }
-  }
 }
 
 // Backward sweep L^T z = q (project-solver.cc:289-305) and the fixed-tree z.r; the last brick
@@ -1342,7 +1352,18 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
         xstore(o1 + (size_t)so * 8 + ew, (double)v[1]);
       }
     };
-    sw_wave1<1>(sh, NC, sc, lane, bnd, pub, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
+    // z.r per lane in slot order: z from the outbox, r from the chunk's ring slot (which wave 3
+    // refills only after this wave has released the chunk)
+    double acc = 0.0;
+    auto fold = [&](int c, const T* ob) __attribute__((always_inline)) {
+      const T* rc = sh.tb[c % SwCfg<T>::RD][AR];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int i = (U - 1 - u) * 64 + lane;
+        acc += (double)ob[i] * (double)rc[i];
+      }
+    };
+    sw_wave1<1>(sh, NC, sc, lane, bnd, pub, fold, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
       if (live(0, c)) xstore(const_cast<double*>(ent(0, c)), xsent());  // consumed entries back
       if (live(1, c)) xstore(const_cast<double*>(ent(1, c)), xsent());
 #pragma unroll
@@ -1350,6 +1371,20 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
         *reinterpret_cast<Vec16<T>*>(z + brick + RunBwd{}(c, S) + h * (1024 / (int)sizeof(T)) +
                                      lane * SwCfg<T>::PER16) = o.d[0][h];
     });
+    const double ws = wave_sum(acc);
+    if (lane == 0) xstore(part + B, ws);
+    if (__builtin_amdgcn_readfirstlane(sh.last)) {
+      gather_wait_wave(part, L.nlive, sc);
+      const double sig = gather_tree256_wave(part, L.nlive);
+      if (lane == 0) {
+        if (INIT) {
+          sc->sigma = sig;
+        } else {
+          sc->beta = sig / sc->sigma;
+          sc->sigma = sig;
+        }
+      }
+    }
     return;
   }
   if (wave == 2) {  // exchange entries
@@ -1358,17 +1393,16 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
   }
   if (wave == 3) {  // array loads
     const T* src[SW_NA] = {qv + brick, dr + brick, du + brick, df + brick, E + brick, r + brick};
-    sw_wave3<SW_NA>(sh, src, NC, S, RunBwd{}, sc, lane);
+    sw_wave3<SW_NA, true>(sh, src, NC, S, RunBwd{}, sc, lane);
     return;
   }
   T zprev = T(0);
-  double acc = 0.0;
   sw_wave0(sh, NC, sc, [&](int c, int slot, const T (*tb)[SW_CE], const double* xr, SwXw<T>& xw)
                            __attribute__((always_inline)) {
     constexpr int UB = sizeof(T) == 8 ? 4 : 8;  // operands of UB slots into registers first
 #pragma unroll
     for (int u0 = 0; u0 < U; u0 += UB) {
-      T qv[UB], v1[UB], v2[UB], v3[UB], ev[UB], rv[UB];
+      T qv[UB], v1[UB], v2[UB], v3[UB], ev[UB];
 #pragma unroll
       for (int k = 0; k < UB; k++) {
         const int i = (U - 1 - (u0 + k)) * 64 + lane;
@@ -1377,7 +1411,6 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
         v2[k] = tb[A2][i];
         v3[k] = tb[A3][i];
         ev[k] = tb[AE][i];
-        rv[k] = tb[AR][i];
       }
       auto step = [&](int k, T x0, T x1) __attribute__((always_inline)) {
         const int u = u0 + k;
@@ -1392,7 +1425,6 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
         sh.ob[slot][0][(U - 1 - u) * 64 + lane] = zv;
         zprev = zv;
         xw.post(c * U + u + 1);
-        acc += (double)zv * (double)rv[k];
       };
       if (xw.all(c * U + u0 + UB)) {
         T x0[UB], x1[UB];  // read by every lane: no branch
@@ -1412,20 +1444,6 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
       }
     }
   });
-  double ws = wave_sum(acc);
-  if (lane == 0) xstore(part + B, ws);
-  if (__builtin_amdgcn_readfirstlane(sh.last)) {
-    gather_wait_wave(part, L.nlive, sc);
-    const double sig = gather_tree256_wave(part, L.nlive);
-    if (lane == 0) {
-      if (INIT) {
-        sc->sigma = sig;
-      } else {
-        sc->beta = sig / sc->sigma;
-        sc->sigma = sig;
-      }
-    }
-  }
 }
 
 // pg += alpha s (saxpy, :332); s = z + beta s (scaleAndAdd, :341) unless the solve stopped.
