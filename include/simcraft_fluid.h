@@ -197,6 +197,9 @@ enum {
 };
 int fs_kernel_profile(fs_ctx* ctx, int32_t kind, int64_t* launches, double* total_ms, double* total_bytes);
 int fs_kernel_profile_reset(fs_ctx* ctx);
+/* Turns the bracketing on (default) or off at run time, so that a timed run can leave the
+ * events (and the synchronisations that drain them) out; only for profile_stages >= 2. */
+int fs_set_kernel_timing(fs_ctx* ctx, int32_t on);
 
 /* ---- Reverse mode (no counterpart in the reference, which has no derivative code) ----
  * Vector-Jacobian products of a rollout of fs_substep calls, for differentiable simulation.

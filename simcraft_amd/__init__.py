@@ -235,6 +235,10 @@ class FluidSim:
     def kernel_profile_reset(self) -> None:
         self._check(self._lib.fs_kernel_profile_reset(self._ctx))
 
+    def set_kernel_timing(self, on: bool) -> None:
+        """Switch the per-launch event bracketing on or off (profile_stages >= 2)."""
+        self._check(self._lib.fs_set_kernel_timing(self._ctx, 1 if on else 0))
+
     # -- reverse mode (no counterpart in the reference) ------------------------------------------
     def rollout_begin(self, checkpoint_every: int = 8) -> None:
         """Record the following substeps for rollout_vjp (checkpoints every `checkpoint_every`)."""

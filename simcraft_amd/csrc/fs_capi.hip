@@ -37,7 +37,7 @@ int fs_zero(fs_ctx* c, DBuf& b) {
 }
 
 int fs_prof_begin(fs_ctx* c, int kind, double bytes) {
-  if (c->cfg.profile_stages < 2 || !c->pev_ok) return -1;
+  if (c->cfg.profile_stages < 2 || !c->pev_ok || !c->ktime_on) return -1;
   if (c->pev_used + 2 > fs_ctx::NPROF_EV) {
     if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
     fs_prof_flush(c);
@@ -635,6 +635,12 @@ int fs_kernel_profile(fs_ctx* c, int32_t kind, int64_t* launches, double* total_
   if (launches) *launches = c->kprof_launches[kind];
   if (total_ms) *total_ms = c->kprof_ms[kind];
   if (total_bytes) *total_bytes = c->kprof_bytes[kind];
+  return FS_OK;
+}
+
+int fs_set_kernel_timing(fs_ctx* c, int32_t on) {
+  FS_TRY(check_ctx(c));
+  c->ktime_on = on != 0;
   return FS_OK;
 }
 

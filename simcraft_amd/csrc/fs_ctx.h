@@ -151,6 +151,7 @@ struct fs_ctx {
   double pev_bytes[NPROF_EV / 2];
   int pev_used = 0;
   bool pev_ok = false;
+  bool ktime_on = true;  // fs_set_kernel_timing
   int64_t kprof_launches[FS_KERNEL_COUNT] = {};
   double kprof_ms[FS_KERNEL_COUNT] = {};
   double kprof_bytes[FS_KERNEL_COUNT] = {};
