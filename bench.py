@@ -13,8 +13,8 @@ The JSON line carries:
                 (The events, and the synchronisations that drain them, cost several percent of
                 a step, so they stay out of the timed region.)
   cpu_baseline  the FP64 CPU oracle (a restatement of the reference, which does not build) on
-                a bounded sample: one substep of the same scene at 128^3 with the PCG capped at a
-                few iterations, scaled by the cell/particle count and the device's iterations.
+                a bounded sample: one substep of the same scene at 160^3, scaled by the
+                cell/particle count and the device's PCG iteration count.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
 """
@@ -93,8 +93,8 @@ def cpu_baseline(simcraft_amd, scenes, wl: dict, tol_mode: int, gpu_iters: float
         "cores": cores,
         "kind": "port",
         "sample": (f"one substep of the same scene at {n}^3 (cells and particles 1/{scale:g} of {n_full}^3), "
-                   f"timed per stage and scaled x{scale:g}; its PCG capped at {pcg_iters} iterations "
-                   f"({t_pcg:.2f}s) and scaled to the device's mean {gpu_iters:.1f} iterations at {n_full}^3; "
+                   f"timed per stage and scaled x{scale:g}; its PCG ran {pcg_iters} iterations (cap {cap}, "
+                   f"{t_pcg:.2f}s) and is scaled to the device's mean {gpu_iters:.1f} iterations at {n_full}^3; "
                    f"P2G with the reference's parallel particle loop"),
         "seconds_per_step": step_s,
         "stage_seconds_sample": {k: round(v, 3) for k, v in times.items()},
@@ -176,8 +176,8 @@ def main() -> int:
     ap.add_argument("--workload", default="dam_break_sphere_256_mixed")
     ap.add_argument("--tol-mode", choices=["abs", "rel"], default="abs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-pcg-iters", type=int, default=5)
-    ap.add_argument("--cpu-sample-n", type=int, default=128)
+    ap.add_argument("--cpu-pcg-iters", type=int, default=300)
+    ap.add_argument("--cpu-sample-n", type=int, default=160)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-n", type=int, default=128)
     ap.add_argument("--adjoint-steps", type=int, default=40)
