@@ -864,7 +864,7 @@ struct SwSet {
 template <typename V>
 struct SwLds {
   __align__(16) V tb[SwCfg<V>::RD][SW_NA][SW_CE];  // chunks c % RD, one element per lane and slot
-  __align__(16) V ob[SW_K][2][SW_CE];     // outbox: outputs of finished chunks
+  __align__(16) V ob[SW_K][SW_CE];        // outbox: outputs of finished chunks
   double xr[SW_KX][2][64];                // exchange entries of chunks c % SW_KX, from wave 2
   int prog, freed, xready, loaded, B, stop;  // slots finished by wave 0 / chunks released by wave 1 /
                                              // slots whose entries wave 2 has read / chunks loaded
@@ -973,13 +973,14 @@ __device__ __forceinline__ void sw_store(V* dst, const V* lds, int lane) {
 
 
 // Wave 3 (loads): keeps SwCfg::R chunks of array loads in flight (register sets named rather
-// than indexed so that all stay in registers) and moves each loaded chunk to LDS ring slot
+// than indexed so that all stay in registers), applies PREP to each loaded chunk (elementwise
+// work that the chain's operands need) and moves its first NSPILL arrays to LDS ring slot
 // c % RD once wave 0 (GUARD: and wave 1) is done with the chunk that used it. Wave 3 issues loads
 // only, so the waits before the moves are exact. Whole groups of R chunks run as straight-line
 // code (an exit inside would merge paths and cost exact waits).
-template <int NARR, bool GUARD, typename V, typename RUN>
+template <int NARR, int NSPILL, bool GUARD, typename V, typename RUN, typename PREP>
 __device__ __forceinline__ void sw_wave3(SwLds<V>& sh, const V* const* src, int nc, int S, RUN run, PcgScalars* sc,
-                                         int lane) {
+                                         int lane, PREP prep) {
   constexpr int R = SwCfg<V>::R, RD = SwCfg<V>::RD;
   SwSet<V> s0, s1, s2, s3;
   sw_load<NARR>(s0, src, nc, 0, lane, run, S);
@@ -989,7 +990,8 @@ __device__ __forceinline__ void sw_wave3(SwLds<V>& sh, const V* const* src, int 
   auto chunk = [&](int c, SwSet<V>& cur) __attribute__((always_inline)) {
     if (c >= RD) lds_until(&sh.prog, (c - RD + 1) * U, sc);
     if (GUARD && c >= RD) lds_wait(&sh.freed, c - RD + 1, sc);  // wave 1 has read the ring slot too
-    sw_spill<NARR>(sh.tb[c % RD], cur, lane);
+    prep(cur);
+    sw_spill<NSPILL>(sh.tb[c % RD], cur, lane);
     lds_put(&sh.loaded, c + 1);
     sw_load<NARR>(cur, src, nc, c + R, lane, run, S);
   };
@@ -1082,9 +1084,10 @@ __device__ __forceinline__ void sw_wave2(SwLds<V>& sh, int nc, FX ent, PcgScalar
 // wait for chunk c's stores to have read their data.
 template <typename V>
 struct SwOut {
-  Vec16<V> d[2][SwCfg<V>::OPS];  // the chunk's one or two output arrays, 16 bytes per lane
+  Vec16<V> d[2][SwCfg<V>::OPS];  // the chunk's outputs, 16 bytes per lane: [0] from the outbox,
+                                 // [1] (if any) read from the ring by FOLD
 };
-template <int NOUT, typename V, typename BND, typename PUB, typename FOLD, typename STORE>
+template <typename V, typename BND, typename PUB, typename FOLD, typename STORE>
 __device__ __forceinline__ void sw_wave1(SwLds<V>& sh, int nc, PcgScalars* sc, int lane, BND bnd, PUB pub,
                                          FOLD fold, STORE store) {
   SwOut<V> o0, o1;
@@ -1095,17 +1098,15 @@ __device__ __forceinline__ void sw_wave1(SwLds<V>& sh, int nc, PcgScalars* sc, i
       if (seen <= c * U + lo) seen = lds_until(&sh.prog, c * U + lo + 1, sc);
       const int hi = min(seen - c * U, U);
       V v[2];
-      bnd(&sh.ob[slot][0][0], v);
+      bnd(&sh.ob[slot][0], v);
       pub(c, v, lo, hi);
       lo = hi;
     }
 #pragma unroll
-    for (int k = 0; k < NOUT; k++)
-#pragma unroll
-      for (int h = 0; h < SwCfg<V>::OPS; h++)
-        o.d[k][h] = *reinterpret_cast<const Vec16<V>*>(&sh.ob[slot][k][h * (1024 / (int)sizeof(V)) +
-                                                                         lane * SwCfg<V>::PER16]);
-    fold(c, &sh.ob[slot][0][0]);  // per-chunk reductions, before the slot is released
+    for (int h = 0; h < SwCfg<V>::OPS; h++)
+      o.d[0][h] = *reinterpret_cast<const Vec16<V>*>(&sh.ob[slot][h * (1024 / (int)sizeof(V)) +
+                                                                 lane * SwCfg<V>::PER16]);
+    fold(c, &sh.ob[slot][0], o);  // per-chunk reductions and ring reads, before the slot is released
     lds_put(&sh.freed, c + 1);
     store(c, o);
   };
@@ -1161,6 +1162,7 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
   auto ent = [&](int f, int c) __attribute__((always_inline)) -> const double* {
     return live(f, c) ? xn[f] + (size_t)(c * U + es) * 8 + ew : Z + ew;
   };
+  const double nalpha = UPDATE ? -sc->alpha : 0.0;
   if (wave == 1) {  // stores
     double* o0 = X + (size_t)B * S * 8;  // this brick's f 0 entries (read by its +x neighbour)
     double* o1 = X + face + (size_t)B * S * 8;
@@ -1175,20 +1177,25 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
         xstore(o1 + (size_t)c * U * 8 + lane, (double)v[1]);
       }
     };
-    // UPDATE: ||r||_inf and r.r of the updated residual (outbox 1), per lane in slot order
+    // UPDATE: the updated residual is in the chunk's ring slot (formed by wave 3): ||r||_inf and
+    // r.r per lane in slot order, and its 16-byte-per-lane store values
     double mx = 0.0, ss = 0.0;
-    auto fold = [&](int, const T* ob) __attribute__((always_inline)) {
+    auto fold = [&](int c, const T*, SwOut<T>& o) __attribute__((always_inline)) {
       if (UPDATE) {
+        const T* rc = sh.tb[c % SwCfg<T>::RD][AR];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-          const double rr = (double)ob[SW_CE + u * 64 + lane];
+          const double rr = (double)rc[u * 64 + lane];
           const double av = fabs(rr);
           mx = (mx < av) ? av : mx;
           ss += rr * rr;
         }
+#pragma unroll
+        for (int h = 0; h < SwCfg<T>::OPS; h++)
+          o.d[1][h] = *reinterpret_cast<const Vec16<T>*>(&rc[h * (1024 / (int)sizeof(T)) + lane * SwCfg<T>::PER16]);
       }
     };
-    sw_wave1<UPDATE ? 2 : 1>(sh, NC, sc, lane, bnd, pub, fold, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
+    sw_wave1(sh, NC, sc, lane, bnd, pub, fold, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
       if (live(0, c)) xstore(const_cast<double*>(ent(0, c)), xsent());  // consumed entries back
       if (live(1, c)) xstore(const_cast<double*>(ent(1, c)), xsent());
 #pragma unroll
@@ -1233,10 +1240,16 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
   }
   if (wave == 3) {  // array loads
     const T* src[SW_NA] = {r + brick, cl + brick, cd + brick, cb + brick, E + brick, t + brick};
-    sw_wave3<NARR, false>(sh, src, NC, S, RunFwd{}, sc, lane);
+    // UPDATE: r + (-alpha) t into the r slot as each chunk arrives (t is not staged)
+    const T na = (T)nalpha;
+    sw_wave3<NARR, 5, UPDATE>(sh, src, NC, S, RunFwd{}, sc, lane, [&](SwSet<T>& st) __attribute__((always_inline)) {
+      if (UPDATE) {
+#pragma unroll
+        for (int h = 0; h < SwCfg<T>::OPS; h++) st.d[AR][h] = st.d[AR][h] + na * st.d[AT][h];
+      }
+    });
     return;
   }
-  const double nalpha = UPDATE ? -sc->alpha : 0.0;
   T qprev = T(0);
   sw_wave0(sh, NC, sc, [&](int c, int slot, const T (*tb)[SW_CE], const double* xr, SwXw<T>& xw)
                            __attribute__((always_inline)) {
@@ -1249,12 +1262,7 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
 #pragma unroll
       for (int k = 0; k < UB; k++) {
         const int i = (u0 + k) * 64 + lane;
-        T rr = tb[AR][i];
-        if (UPDATE) {
-          rr = rr + (T)nalpha * tb[AT][i];
-          sh.ob[slot][1][i] = rr;
-        }
-        rv[k] = rr;
+        rv[k] = tb[AR][i];  // UPDATE: already r + (-alpha) t
         lv[k] = tb[AL][i];
         dv[k] = tb[AD][i];
         bv[k] = tb[AB][i];
@@ -1270,7 +1278,7 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
         tt = tt - dv[k] * qD;
         tt = tt - bv[k] * qprev;
         const T qv = tt * ev[k];
-        sh.ob[slot][0][u * 64 + lane] = qv;
+        sh.ob[slot][u * 64 + lane] = qv;
         qprev = qv;
         xw.post(c * U + u + 1);
       };
@@ -1355,7 +1363,7 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
     // z.r per lane in slot order: z from the outbox, r from the chunk's ring slot (which wave 3
     // refills only after this wave has released the chunk)
     double acc = 0.0;
-    auto fold = [&](int c, const T* ob) __attribute__((always_inline)) {
+    auto fold = [&](int c, const T* ob, SwOut<T>&) __attribute__((always_inline)) {
       const T* rc = sh.tb[c % SwCfg<T>::RD][AR];
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -1363,7 +1371,7 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
         acc += (double)ob[i] * (double)rc[i];
       }
     };
-    sw_wave1<1>(sh, NC, sc, lane, bnd, pub, fold, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
+    sw_wave1(sh, NC, sc, lane, bnd, pub, fold, [&](int c, const SwOut<T>& o) __attribute__((always_inline)) {
       if (live(0, c)) xstore(const_cast<double*>(ent(0, c)), xsent());  // consumed entries back
       if (live(1, c)) xstore(const_cast<double*>(ent(1, c)), xsent());
 #pragma unroll
@@ -1393,7 +1401,7 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
   }
   if (wave == 3) {  // array loads
     const T* src[SW_NA] = {qv + brick, dr + brick, du + brick, df + brick, E + brick, r + brick};
-    sw_wave3<SW_NA, true>(sh, src, NC, S, RunBwd{}, sc, lane);
+    sw_wave3<SW_NA, SW_NA, true>(sh, src, NC, S, RunBwd{}, sc, lane, [](SwSet<T>&) {});
     return;
   }
   T zprev = T(0);
@@ -1422,7 +1430,7 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
         tt = tt - v2[k] * zU;
         tt = tt - v3[k] * zprev;
         const T zv = tt * ev[k];
-        sh.ob[slot][0][(U - 1 - u) * 64 + lane] = zv;
+        sh.ob[slot][(U - 1 - u) * 64 + lane] = zv;
         zprev = zv;
         xw.post(c * U + u + 1);
       };
