@@ -866,11 +866,16 @@ struct SwLds {
   __align__(16) V tb[SwCfg<V>::RD][SW_NA][SW_CE];  // chunks c % RD, one element per lane and slot
   __align__(16) V ob[SW_K][SW_CE];        // outbox: outputs of finished chunks
   double xr[SW_KX][2][64];                // exchange entries of chunks c % SW_KX, from wave 2
-  int prog, freed, xready, loaded, B, stop;  // slots finished by wave 0 / chunks released by wave 1 /
-                                             // slots whose entries wave 2 has read / chunks loaded
-                                             // by wave 3
+  __align__(8) int loaded;  // chunks loaded by wave 3 and (next word) released by wave 1: one
+  int freed;                // 8-byte word, so that wave 0 reads both at once
+  int prog, xready, B, stop;       // slots finished by wave 0 / slots whose entries wave 2 has read
   int last;                                  // this brick holds the last ticket (it reduces)
 };
+
+static_assert(offsetof(SwLds<float>, freed) == offsetof(SwLds<float>, loaded) + 4 &&
+                  offsetof(SwLds<double>, freed) == offsetof(SwLds<double>, loaded) + 4 &&
+                  offsetof(SwLds<float>, loaded) % 8 == 0 && offsetof(SwLds<double>, loaded) % 8 == 0,
+              "loaded and freed share one 8-byte word");
 
 __device__ __forceinline__ int lds_get(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -893,6 +898,25 @@ __device__ __forceinline__ bool lds_wait(const int* p, int v, PcgScalars* sc) {
     }
   }
   return true;
+}
+
+// Wait until p[0] >= v0 and p[1] >= v1 (two counters of one 8-byte word, read together).
+__device__ __forceinline__ void lds_wait2(const int* p, int v0, int v1, PcgScalars* sc) {
+  long spins = 0;
+  while (true) {
+    const unsigned long long w = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_ACQUIRE,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (((int)(unsigned)w >= v0) & ((int)(unsigned)(w >> 32) >= v1)) return;
+    __builtin_amdgcn_s_sleep(1);
+    if ((++spins & 255) == 0) {
+      const int to = __hip_atomic_load(&sc->wait_timeouts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (to != 0 || spins > SPIN_LIMIT) {
+        if ((threadIdx.x & 63) == 0) atomicAdd(&sc->wait_timeouts, 1);
+        __builtin_amdgcn_s_waitcnt(0);  // leave nothing in flight on this path
+        return;
+      }
+    }
+  }
 }
 
 // Slot counters (prog, xready) are posted once per slot, so they use relaxed accesses: the LDS
@@ -1032,8 +1056,7 @@ __device__ __forceinline__ void sw_wave0(SwLds<V>& sh, int nc, PcgScalars* sc, B
   SwXw<V> xw{sh, sc, 0};
   for (int c = 0; c < nc; c++) {
     const int slot = c % SW_K;
-    lds_wait(&sh.loaded, c + 1, sc);
-    if (c >= SW_K) lds_wait(&sh.freed, c - SW_K + 1, sc);
+    lds_wait2(&sh.loaded, c + 1, c - SW_K + 1, sc);  // chunk loaded, outbox slot released
     body(c, slot, sh.tb[c % SwCfg<V>::RD], &sh.xr[c % SW_KX][0][0], xw);
   }
 }
