@@ -40,6 +40,20 @@ GridArgs<T> grid_args(const fs_ctx* c) {
   return a;
 }
 
+// One thread per node of an nx*ny*nz grid without index division: blocks of 64 x 4 threads,
+// x -> k (64 per block), y -> j (4 per block), block z -> i. Returns false past the grid's edge.
+constexpr int G3X = 64, G3Y = 4;
+static_assert(G3X * G3Y == TPB, "3D stencil blocks hold TPB threads");
+inline dim3 g3_grid(int nx, int ny, int nz) { return dim3((nz + G3X - 1) / G3X, (ny + G3Y - 1) / G3Y, nx); }
+inline dim3 g3_block() { return dim3(G3X, G3Y, 1); }
+__device__ inline bool g3_index(int ny, int nz, int& i, int& j, int& k, int& id) {
+  k = (int)blockIdx.x * G3X + (int)threadIdx.x;
+  j = (int)blockIdx.y * G3Y + (int)threadIdx.y;
+  i = (int)blockIdx.z;
+  id = (i * ny + j) * nz + k;
+  return k < nz && j < ny;
+}
+
 __device__ inline void unlin3(int id, int ny, int nz, int& i, int& j, int& k) {
   k = id % nz;
   int t = id / nz;
@@ -665,10 +679,8 @@ __global__ __launch_bounds__(TPB) void k_recon_final(int ncell, const typename B
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_extrap_sdf(int nx, int ny, int nz, const T* __restrict__ cur, T* __restrict__ buf,
                              const uint8_t* __restrict__ val, uint8_t* __restrict__ vbuf) {
-  int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= nx * ny * nz) return;
-  int i, j, k;
-  unlin3(id, ny, nz, i, j, k);
+  int i, j, k, id;
+  if (!g3_index(ny, nz, i, j, k, id)) return;
   if (val[id]) {
     vbuf[id] = 1;
     return;
@@ -695,10 +707,8 @@ __global__ __launch_bounds__(TPB) void k_extrap_sdf(int nx, int ny, int nz, cons
 template <typename T, bool FLAG = false>
 __global__ __launch_bounds__(TPB) void k_smooth(int nx, int ny, int nz, const T* __restrict__ cur, T* __restrict__ buf,
                          uint8_t* __restrict__ took_avg = nullptr) {
-  int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= nx * ny * nz) return;
-  int i, j, k;
-  unlin3(id, ny, nz, i, j, k);
+  int i, j, k, id;
+  if (!g3_index(ny, nz, i, j, k, id)) return;
   T sum = T(0);
   int count = 0;
   const int sx = ny * nz, sy = nz;
@@ -734,10 +744,8 @@ __global__ __launch_bounds__(TPB) void k_dirichlet(int axis, int gx, int gy, int
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_extrap_vel(int gx, int gy, int gz, const T* __restrict__ g, T* __restrict__ gbuf,
                              const uint8_t* __restrict__ val, uint8_t* __restrict__ vbuf) {
-  int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= gx * gy * gz) return;
-  int i, j, k;
-  unlin3(id, gy, gz, i, j, k);
+  int i, j, k, id;
+  if (!g3_index(gy, gz, i, j, k, id)) return;
   if (val[id]) {
     vbuf[id] = 1;
     return;
@@ -1136,7 +1144,7 @@ int reconstruct_t(fs_ctx* c, double radius) {
 template <typename T>
 int extrap_sdf_t(fs_ctx* c, int iters) {
   for (int it = 0; it < iters; it++) {
-    k_extrap_sdf<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->nx, c->ny, c->nz, c->fsdf.as<T>(),
+    k_extrap_sdf<T><<<g3_grid(c->nx, c->ny, c->nz), g3_block(), 0, c->stream>>>(c->nx, c->ny, c->nz, c->fsdf.as<T>(),
                                                                       c->fsdfbuf.as<T>(), c->sdfvalid.as<uint8_t>(),
                                                                       c->sdfvalidbuf.as<uint8_t>());
     std::swap(c->fsdf, c->fsdfbuf);
@@ -1149,7 +1157,7 @@ int extrap_sdf_t(fs_ctx* c, int iters) {
 template <typename T>
 int smooth_t(fs_ctx* c, int iters) {
   for (int it = 0; it < iters; it++) {
-    k_smooth<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->nx, c->ny, c->nz, c->fsdf.as<T>(),
+    k_smooth<T><<<g3_grid(c->nx, c->ny, c->nz), g3_block(), 0, c->stream>>>(c->nx, c->ny, c->nz, c->fsdf.as<T>(),
                                                                   c->fsdfbuf.as<T>());
     std::swap(c->fsdf, c->fsdfbuf);
   }
@@ -1198,7 +1206,7 @@ template <typename T>
 int extrap_vel_t(fs_ctx* c, int iters) {
   for (int a = 0; a < 3; a++)
     for (int it = 0; it < iters; it++) {
-      k_extrap_vel<T><<<fs_blocks(c->n_face[a], TPB), TPB, 0, c->stream>>>(
+      k_extrap_vel<T><<<g3_grid(c->fdim[a][0], c->fdim[a][1], c->fdim[a][2]), g3_block(), 0, c->stream>>>(
           c->fdim[a][0], c->fdim[a][1], c->fdim[a][2], c->g[a].as<T>(), c->gbuf[a].as<T>(),
           c->gvalid[a].as<uint8_t>(), c->gvalidbuf[a].as<uint8_t>());
       std::swap(c->g[a], c->gbuf[a]);
@@ -1473,12 +1481,12 @@ int adj_sdf_t(fs_ctx* c, AdjStep& t, DBuf& phibar, DBuf& psibar, DBuf* xbar) {
   // forward: extrapolation (values and masks), then smoothing with its branch flags
   DBuf *cur = &tp.sA, *buf = &tp.sB;
   for (int it = 0; it < ie; it++) {
-    k_extrap_sdf<T><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(c->nx, c->ny, c->nz, cur->as<T>(), buf->as<T>(),
+    k_extrap_sdf<T><<<g3_grid(c->nx, c->ny, c->nz), g3_block(), 0, c->stream>>>(c->nx, c->ny, c->nz, cur->as<T>(), buf->as<T>(),
                                                                tp.sV[it].as<uint8_t>(), tp.sV[it + 1].as<uint8_t>());
     std::swap(cur, buf);
   }
   for (int it = 0; it < is; it++) {
-    k_smooth<T, true><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(c->nx, c->ny, c->nz, cur->as<T>(), buf->as<T>(),
+    k_smooth<T, true><<<g3_grid(c->nx, c->ny, c->nz), g3_block(), 0, c->stream>>>(c->nx, c->ny, c->nz, cur->as<T>(), buf->as<T>(),
                                                                 tp.sF[it].as<uint8_t>());
     std::swap(cur, buf);
   }
