@@ -609,13 +609,27 @@ __global__ __launch_bounds__(TPB) void k_fill_bits(int n, typename Bits<T>::U* _
   if (i < n) d[i] = Bits<T>::inf;
 }
 
+// The per-cell minima are kept in RC_COPIES arrays of ncell entries: one per XCD (XCC_ID 0..7),
+// updated with workgroup-scope atomics, which the XCD's own L2 performs (all CUs of an XCD share
+// it), and one for any other XCC_ID, updated at agent scope; k_recon_final takes the minimum of
+// the copies. Agent-scope atomics on one array would all go out to memory.
+constexpr int RC_COPIES = 9;
+__device__ __forceinline__ int xcc_id() {
+  unsigned v;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+  return (int)v;
+}
+
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_recon_splat(int n, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
                               const T* __restrict__ pz, const int* __restrict__ keys, const int* __restrict__ binmax,
-                              T r2, T s_max, typename Bits<T>::U* __restrict__ dmin) {
+                              T r2, T s_max, typename Bits<T>::U* __restrict__ dmin, int ncell) {
   using UB = typename Bits<T>::U;
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= n) return;
+  const int xc8 = xcc_id();
+  const bool local = xc8 >= 0 && xc8 < 8;  // uniform over the workgroup
+  UB* const dcopy = dmin + (size_t)(local ? xc8 : 8) * ncell;
   if (binmax[keys[id]] == id) return;  // forNeighbours stops one short of the inclusive bin end
   const T p[3] = {px[id], py[id], pz[id]};
   const int dims[3] = {ga.nx, ga.ny, ga.nz};
@@ -649,9 +663,15 @@ __global__ __launch_bounds__(TPB) void k_recon_splat(int n, GridArgs<T> ga, cons
         const T dz = zc - p[2];
         const T d2 = dxy + dz * dz;
         if (d2 > s_max || !in_range(2, zc)) continue;
-        UB* cell = dmin + lin3(i, j, k, ga.ny, ga.nz);
+        UB* cell = dcopy + lin3(i, j, k, ga.ny, ga.nz);
         const UB b = Bits<T>::of(d2);
-        if (b < __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(cell, b);
+        if (local) {
+          if (b < __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+            __hip_atomic_fetch_min(cell, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+          if (b < __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            __hip_atomic_fetch_min(cell, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
     }
   }
@@ -659,11 +679,17 @@ __global__ __launch_bounds__(TPB) void k_recon_splat(int n, GridArgs<T> ga, cons
 
 // sdf.grid.fill(1e9); sdfValid.fill(false); then min(sdf, |d| - r) for the kept particles.
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_recon_final(int ncell, const typename Bits<T>::U* __restrict__ dmin, T radius, T* __restrict__ sdf,
+__global__ __launch_bounds__(TPB) void k_recon_final(int ncell, typename Bits<T>::U* __restrict__ dmin, T radius, T* __restrict__ sdf,
                               uint8_t* __restrict__ valid) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
-  const typename Bits<T>::U b = dmin[c];
+  typename Bits<T>::U b = dmin[c];
+#pragma unroll
+  for (int q = 1; q < RC_COPIES; q++) {
+    const typename Bits<T>::U o = dmin[(size_t)q * ncell + c];
+    b = o < b ? o : b;
+  }
+  dmin[c] = b;  // the merged minimum, which the reverse pass reads
   T out = T(1e9);
   uint8_t v = 0;
   if (b != Bits<T>::inf) {
@@ -1116,9 +1142,9 @@ int reconstruct_t(fs_ctx* c, double radius) {
   int n = (int)c->np;
   int nb = morton3D(c->nx - 1, c->ny - 1, c->nz - 1) + 1;
   FS_TRY(ensure_bins(c, (size_t)nb));
-  if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)c->n_cell * sizeof(double)));
+  if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)c->n_cell * RC_COPIES * sizeof(double)));
   UB* dmin = c->recon_tmp.as<UB>();
-  k_fill_bits<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->n_cell, dmin);
+  k_fill_bits<T><<<fs_blocks(c->n_cell * RC_COPIES, TPB), TPB, 0, c->stream>>>(c->n_cell * RC_COPIES, dmin);
   T rad = (T)radius;
   T r2 = T(2) * rad;
   T smax;
@@ -1132,7 +1158,7 @@ int reconstruct_t(fs_ctx* c, double radius) {
     k_bin_last<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->counts.as<int>());
     k_recon_splat<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
                                                                c->pos[2].as<T>(), c->keys.as<int>(),
-                                                               c->counts.as<int>(), r2, smax, dmin);
+                                                               c->counts.as<int>(), r2, smax, dmin, c->n_cell);
   }
   k_recon_final<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->n_cell, dmin, rad, c->fsdf.as<T>(),
                                                                      c->sdfvalid.as<uint8_t>());
@@ -1462,11 +1488,11 @@ int adj_sdf_t(fs_ctx* c, AdjStep& t, DBuf& phibar, DBuf& psibar, DBuf* xbar) {
   GridArgs<T> ga = grid_args<T>(c);
   const int nb = morton3D(c->nx - 1, c->ny - 1, c->nz - 1) + 1;
   FS_TRY(ensure_bins(c, (size_t)nb));
-  if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)nc * sizeof(double)));
+  if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)nc * RC_COPIES * sizeof(double)));
   UB* dmin = c->recon_tmp.as<UB>();
   const T rad = (T)(c->cfg.recon_radius_factor * c->h[0]), r2 = T(2) * rad;
   const T smx = sizeof(T) == 8 ? (T)smax_for((double)r2) : (T)smax_for_f((float)r2);
-  k_fill_bits<T><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(nc, dmin);
+  k_fill_bits<T><<<fs_blocks(nc * RC_COPIES, TPB), TPB, 0, c->stream>>>(nc * RC_COPIES, dmin);
   if (n > 0) {
     FS_HIP(c, hipMemsetAsync(c->counts.p, 0xFF, (size_t)nb * sizeof(int), c->stream));
     k_keys_morton<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, t.x1[0].as<T>(), t.x1[1].as<T>(),
@@ -1474,7 +1500,7 @@ int adj_sdf_t(fs_ctx* c, AdjStep& t, DBuf& phibar, DBuf& psibar, DBuf* xbar) {
     k_bin_last<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->counts.as<int>());
     k_recon_splat<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, t.x1[0].as<T>(), t.x1[1].as<T>(),
                                                                t.x1[2].as<T>(), c->keys.as<int>(),
-                                                               c->counts.as<int>(), r2, smx, dmin);
+                                                               c->counts.as<int>(), r2, smx, dmin, nc);
   }
   k_recon_final<T><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(nc, dmin, rad, tp.sA.as<T>(), tp.sV[0].as<uint8_t>());
   FS_HIP(c, hipMemcpyAsync(tp.sB.p, t.sdfbuf0.p, cb, hipMemcpyDeviceToDevice, c->stream));
