@@ -84,7 +84,8 @@ void release_all(fs_ctx* c) {
       fs_free(*b);
   for (DBuf* b : {&c->fsdf, &c->fsdfbuf, &c->sdfvalid, &c->sdfvalidbuf, &c->collider, &c->pg, &c->adiag, &c->rhs,
                   &c->active, &c->keys, &c->slots, &c->perm, &c->counts, &c->offs, &c->scan_tmp, &c->red,
-                  &c->recon_tmp, &c->sorted[0], &c->sorted[1], &c->sorted[2], &c->sorted[3], &c->p2g_w})
+                  &c->recon_tmp, &c->sorted[0], &c->sorted[1], &c->sorted[2], &c->sorted[3], &c->p2g_w,
+                  &c->p2g_tiles})
     fs_free(*b);
   for (int q = 0; q < 6; q++) fs_free(c->an[q]);
   fs_pcg_release(c);
@@ -345,7 +346,7 @@ int fs_set_particles(fs_ctx* c, const double* pos, const double* vel, int64_t n)
     FS_TRY(fs_alloc(c, c->slots, (size_t)n * sizeof(int)));
     FS_TRY(fs_alloc(c, c->perm, (size_t)n * sizeof(int)));
     for (int q = 0; q < 4; q++) FS_TRY(fs_alloc(c, c->sorted[q], (size_t)n * c->rsz));
-    FS_TRY(fs_alloc(c, c->p2g_w, (size_t)n * 12 * c->rsz));
+    FS_TRY(fs_alloc(c, c->p2g_w, (size_t)n * 13 * c->rsz));  // 12 factors + the z-bin (int)
     c->np = n;
   }
   return fs_particles_upload(c, pos, vel);

@@ -332,6 +332,7 @@ __global__ __launch_bounds__(TPB) void k_sort_bins(int nb, const int* __restrict
 // around TGrid::nearest. A particle in bin b (its nearest node) meets node n at offset
 // n - b + 1 on each axis, so the node weight CubicKernel::weight(h, p - pn) is
 // (W[ox] * W[4+oy]) * W[8+oz], the same product the reference forms for that pair.
+// W holds 12 * n factors followed by n ints: the particle's z-bin b2 (read by the gather).
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_p2g_records(int n, int axis, GridArgs<T> ga, const int* __restrict__ perm, const T* __restrict__ px,
                               const T* __restrict__ py, const T* __restrict__ pz, const T* __restrict__ vc,
@@ -345,6 +346,7 @@ __global__ __launch_bounds__(TPB) void k_p2g_records(int n, int axis, GridArgs<T
   for (int c = 0; c < 3; c++) {
     const T off = (c == axis) ? T(0) : T(0.5);
     const int nb = nearest1(pp[c], off, ga.h[c]);
+    if (c == 2) reinterpret_cast<int*>(W + (int64_t)12 * n)[t] = nb;
 #pragma unroll
     for (int o = 0; o < 4; o++) {
       const T pn = coord1(nb - 1 + o, off, ga.h[c]);
@@ -384,76 +386,179 @@ __device__ __forceinline__ int p2g_slot(int i) { return i + (i >> 3); }
 
 // One lane over its particles [t0, t1) of a staged row whose y-offset is C = y0 - b1 + 1:
 // y-node j (y = y0 + j) takes the y-factor W[4 + j + C] and is reached when 0 <= j + C <= 3.
-// The z-factor index is 3 - (bins of the lane's four passed), the x-factor that of the
-// wave's plane.
+// The z-factor of lane z for a particle of z-bin b2 is W[8 + z + 1 - b2]: its byte offset is
+// Lz[slot] (= (8 - b2) * CP * sizeof(T), staged) + lzb (= (z + 1) * CP * sizeof(T)) + slot bytes.
+// The x-factor is that of the wave's plane. FP64 forms (wx * wy) * wz, v * w and the sums in the
+// reference's order; FP32 (mixed precision) reassociates to (v wx wz) * wy and (wx wz) * wy,
+// two fused multiply-adds per node. Lz of the next particle is read one iteration ahead (Lz has
+// one entry beyond the last slot of a full chunk), so the z-factor load does not wait for it.
 template <typename T, bool NORM, int C>
-__device__ __forceinline__ void p2g_row(const T* __restrict__ L, int o, int t0, int t1, int c0, int B1, int B2,
-                                        int B3, T (&acc)[P2G_TY], T (&ws)[P2G_TY]) {
+__device__ __forceinline__ void p2g_row(const T* __restrict__ L, const int* __restrict__ Lz, int o, int t0, int t1,
+                                        int c0, int lzb, T (&acc)[P2G_TY], T (&ws)[P2G_TY]) {
   constexpr int CP = P2GCap<T>::padded;
+  const char* Lb = reinterpret_cast<const char*>(L);
+  if (t0 >= t1) return;
+  int sn = p2g_slot(t0 - c0), zn = Lz[sn];
   for (int t = t0; t < t1; t++) {
-    const int s = p2g_slot(t - c0);
-    const int iz = 3 - (int)(t >= B1) - (int)(t >= B2) - (int)(t >= B3);
+    const int s = sn, zo = zn;
+    sn = p2g_slot(t + 1 - c0);
+    zn = Lz[sn];
     const T wxo = L[o * CP + s];
-    const T wz = L[(8 + iz) * CP + s];
+    const T wz = *reinterpret_cast<const T*>(Lb + (zo + lzb + s * (int)sizeof(T)));
     const T v = L[12 * CP + s];
+    if constexpr (sizeof(T) == 4) {
+      const T xz = wxo * wz, vxz = v * xz;
 #pragma unroll
-    for (int j = 0; j < P2G_TY; j++) {
-      if (j + C < 0 || j + C > 3) continue;
-      const T w = wxo * L[(4 + j + C) * CP + s] * wz;
-      acc[j] = acc[j] + v * w;
-      if (NORM) ws[j] = ws[j] + w;
+      for (int j = 0; j < P2G_TY; j++) {
+        if (j + C < 0 || j + C > 3) continue;
+        const T wy = L[(4 + j + C) * CP + s];
+        acc[j] = fmaf(vxz, wy, acc[j]);
+        if (NORM) ws[j] = fmaf(xz, wy, ws[j]);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < P2G_TY; j++) {
+        if (j + C < 0 || j + C > 3) continue;
+        const T w = wxo * L[(4 + j + C) * CP + s] * wz;
+        acc[j] = acc[j] + v * w;
+        if (NORM) ws[j] = ws[j] + w;
+      }
     }
+    asm volatile("" ::"v"(zn));  // keeps the next Lz load in this iteration
   }
+}
+
+// Tile `tile` of a face grid (gx, gy, gz) with bins (bx, by, bz): y-tiles fastest, then
+// z-tiles, then x-segments. Nodes [xs, xe) x [y0, y1) x [z0, z1); bin rows b1 = b1s ..
+// b1s + nb1 - 1 of the slabs b0 = xs - 2 .. xe (nrow rows in all), bins zlo .. zhi of a row.
+struct P2GTile {
+  int y0, y1, z0, z1, xs, xe, zlo, zhi, b1s, nb1, nrow;
+  __device__ P2GTile(int tile, int gx, int gy, int gz, int nzt, int nyt, int nseg, int by, int bz) {
+    const int yt = tile % nyt, zt = (tile / nyt) % nzt, seg = tile / (nyt * nzt);
+    y0 = yt * gy / nyt;
+    y1 = (yt + 1) * gy / nyt;
+    z0 = zt * gz / nzt;
+    z1 = (zt + 1) * gz / nzt;
+    xs = seg * gx / nseg;
+    xe = (seg + 1) * gx / nseg;
+    zlo = max(z0 - 2, 0);
+    zhi = min(z1, bz - 1);
+    b1s = max(y0 - 2, 0);
+    nb1 = min(y1, by - 1) - b1s + 1;
+    nrow = (xe - xs + 3) * nb1;
+  }
+};
+
+// Tile order for the gather, heaviest first. Tiles differ in work by orders of magnitude (a
+// tile in air stages no particle), and the hardware hands workgroups to the XCDs round-robin in
+// index order, so the gather takes tile order[blockIdx.x]: a longest-first list schedule that
+// spreads the heavy tiles over all XCDs. The work of a tile is the number of particle records
+// it stages; key = (~work << 32) | tile sorts ascending into that order.
+constexpr int P2G_MAX_TILES = 8192;
+
+__global__ __launch_bounds__(TPB) void k_p2g_tile_work(int gx, int gy, int gz, int nzt, int nyt, int nseg, int bx,
+                                                       int by, int bz, const int* __restrict__ offs,
+                                                       unsigned long long* __restrict__ keys) {
+  __shared__ unsigned part[TPB / 64];
+  const int tile = blockIdx.x;
+  const P2GTile g(tile, gx, gy, gz, nzt, nyt, nseg, by, bz);
+  unsigned cnt = 0;
+  for (int ri = threadIdx.x; ri < g.nrow; ri += TPB) {
+    const int b0 = g.xs - 2 + ri / g.nb1, b1 = g.b1s + ri % g.nb1;
+    if (b0 < 0 || b0 >= bx) continue;
+    const int row = bz * (b1 + by * b0);
+    cnt += (unsigned)(offs[row + g.zhi + 1] - offs[row + g.zlo]);
+  }
+  for (int m = 32; m > 0; m >>= 1) cnt += __shfl_xor(cnt, m);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned w = 0;
+    for (int k = 0; k < TPB / 64; k++) w += part[k];
+    keys[tile] = ((unsigned long long)(0xFFFFFFFFu - w) << 32) | (unsigned)tile;
+  }
+}
+
+// Bitonic sort of n <= P2G_MAX_TILES keys in one workgroup; writes the tile of each rank.
+__global__ __launch_bounds__(1024) void k_p2g_tile_order(int n, const unsigned long long* __restrict__ keys,
+                                                         int* __restrict__ order) {
+  __shared__ unsigned long long k[P2G_MAX_TILES];
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int i = threadIdx.x; i < P; i += 1024) k[i] = i < n ? keys[i] : ~0ull;
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < P / 2; i += 1024) {
+        const int a = 2 * i - (i & (stride - 1)), b = a + stride;  // pair i of this stage
+        const bool up = (a & size) == 0;
+        const unsigned long long x = k[a], y = k[b];
+        if ((x > y) == up) {
+          k[a] = y;
+          k[b] = x;
+        }
+      }
+      __syncthreads();
+    }
+  for (int i = threadIdx.x; i < n; i += 1024) order[i] = (int)(unsigned)(k[i] & 0xFFFFFFFFull);
 }
 
 template <typename T, bool NORM = true>
 __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga, int gx, int gy, int gz, int n,
                                                                    int nzt, int nyt, int nseg,
+                                                                   const int* __restrict__ order,
                                                                    const int* __restrict__ offs,
                                                                    const T* __restrict__ W, const T* __restrict__ vc,
                                                                    T* __restrict__ G, T* __restrict__ Wt,
                                                                    uint8_t* __restrict__ valid) {
   constexpr int CAP = P2GCap<T>::value, CP = P2GCap<T>::padded;
   __shared__ T L[13 * CP];
-  // Workgroups go round-robin over the 8 XCDs (id = 8 * k + xcd); each XCD takes a contiguous
-  // run of tiles, y-tiles fastest, so the bin rows that y-neighbouring tiles share are read
-  // through one L2.
-  const int nwg = nzt * nyt * nseg;
-  const int xcd = blockIdx.x & 7, q = nwg >> 3, r = nwg & 7;
-  const int tile = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (int)(blockIdx.x >> 3);
-  const int yt = tile % nyt, zt = (tile / nyt) % nzt, seg = tile / (nyt * nzt);
-  const int y0 = yt * gy / nyt, y1 = (yt + 1) * gy / nyt;
-  const int z0 = zt * gz / nzt, z1 = (zt + 1) * gz / nzt;
-  const int xs = seg * gx / nseg, xe = (seg + 1) * gx / nseg;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ int Lz[CP + 1];
+  const int* __restrict__ zb = reinterpret_cast<const int*>(W + (int64_t)12 * n);
+  // Without an order, workgroups go round-robin over the 8 XCDs (id = 8 * k + xcd) and each
+  // XCD takes a contiguous run of tiles.
+  int tile;
+  if (order) {
+    tile = order[blockIdx.x];
+  } else {
+    const int nwg = nzt * nyt * nseg;
+    const int xcd = blockIdx.x & 7, q = nwg >> 3, r = nwg & 7;
+    tile = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (int)(blockIdx.x >> 3);
+  }
   const int bx = ga.nx + 1, by = ga.ny + 1, bz = ga.nz + 1;
+  const P2GTile g(tile, gx, gy, gz, nzt, nyt, nseg, by, bz);
+  const int y0 = g.y0, y1 = g.y1, z0 = g.z0, z1 = g.z1, xs = g.xs, xe = g.xe;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int z = z0 + lane;
   const bool zok = z < z1;
-  const int zlo = max(z0 - 2, 0), zhi = min(z1, bz - 1);
+  const int lzb = (z + 1) * CP * (int)sizeof(T);
+  const int zlo = g.zlo, zhi = g.zhi;
   T acc[P2G_TY], ws[P2G_TY];
 #pragma unroll
   for (int j = 0; j < P2G_TY; j++) acc[j] = ws[j] = T(0);
-  // Rows in walk order: ri = (b0 - xs + 2) * nb1 + (b1 - b1s). The first chunk of row ri + 1
-  // (records and the lane's bin bounds) is loaded into registers while row ri is summed.
-  const int b1s = max(y0 - 2, 0), nb1 = min(y1, by - 1) - b1s + 1, nrow = (xe - xs + 3) * nb1;
+  // Rows in walk order: ri = (b0 - xs + 2) * nb1 + (b1 - b1s). While row ri is summed, the
+  // first chunk of row ri + 1 is loaded into registers and the bin offsets of row ri + 2 (the
+  // row's record range and the lane's bin bounds) are read, so no load waits on another.
+  const int b1s = g.b1s, nb1 = g.nb1, nrow = g.nrow;
   constexpr int NT = P2G_TZ * P2G_WAVES, PF = (CAP + NT - 1) / NT;
   T pre[PF][13];
-  int pr0 = 0, pr1 = 0, pB[5];
-  auto fetch = [&](int ri) {
+  int prz[PF];
+  struct RowOff {
+    int r0, r1, B0, B1;  // the row's records [r0, r1); the lane's bins z-2 .. z+1 hold [B0, B1)
+  };
+  auto offsets = [&](int ri) {
+    RowOff q{0, 0, 0, 0};
     const int b0 = xs - 2 + ri / nb1, b1 = b1s + ri % nb1;
-    if (b0 < 0 || b0 >= bx) {
-      pr0 = pr1 = 0;
-#pragma unroll
-      for (int m = 0; m < 5; m++) pB[m] = 0;
-      return;
-    }
+    if (ri >= nrow || b0 < 0 || b0 >= bx) return q;
     const int row = bz * (b1 + by * b0);
-    pr0 = offs[row + zlo];
-    pr1 = offs[row + zhi + 1];
-    // the lane's bins z-2 .. z+1 start at pB[0] .. pB[3] and end at pB[4]
-#pragma unroll
-    for (int m = 0; m < 5; m++) pB[m] = zok ? offs[row + min(max(z - 2 + m, 0), bz)] : pr0;
-    const int cnt = min(CAP, pr1 - pr0);
+    q.r0 = offs[row + zlo];
+    q.r1 = offs[row + zhi + 1];
+    q.B0 = offs[row + (zok ? max(z - 2, 0) : zlo)];
+    q.B1 = zok ? offs[row + min(z + 2, bz)] : q.B0;
+    return q;
+  };
+  auto fetch = [&](const RowOff& q) {
+    const int pr0 = q.r0, cnt = min(CAP, q.r1 - q.r0);
 #pragma unroll
     for (int k = 0; k < PF; k++) {
       const int i = (int)threadIdx.x + k * NT;
@@ -461,15 +566,16 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
 #pragma unroll
         for (int f = 0; f < 12; f++) pre[k][f] = W[(int64_t)f * n + pr0 + i];
         pre[k][12] = vc[pr0 + i];
+        prz[k] = zb[pr0 + i];
       }
     }
   };
-  auto sum_chunk = [&](int o, int cy, int c0, int cnt, const int (&B)[5]) {
-    const int t0 = max(B[0], c0), t1 = min(B[4], c0 + cnt);
+  auto sum_chunk = [&](int o, int cy, int c0, int cnt, const int (&B)[2]) {
+    const int t0 = max(B[0], c0), t1 = min(B[1], c0 + cnt);
     switch (cy) {
 #define FS_P2G_ROW(CC) \
   case CC:             \
-    p2g_row<T, NORM, CC>(L, o, t0, t1, c0, B[1], B[2], B[3], acc, ws); \
+    p2g_row<T, NORM, CC>(L, Lz, o, t0, t1, c0, lzb, acc, ws); \
     break;
       FS_P2G_ROW(3)
       FS_P2G_ROW(2)
@@ -487,16 +593,15 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
         break;
     }
   };
-  fetch(0);
+  RowOff oc = offsets(0), on = offsets(1);
+  fetch(oc);
   for (int ri = 0; ri < nrow; ri++) {
     const int b0 = xs - 2 + ri / nb1, b1 = b1s + ri % nb1;
     const int o = (wave + 1 + xs - b0) & 3;  // this wave's plane x = b0 - 1 + o, x - xs = wave (mod 4)
     const int x = b0 - 1 + o;
     const bool live = x >= xs && x < xe;
-    const int r0 = pr0, r1 = pr1, cy = y0 - b1 + 1;
-    int B[5];
-#pragma unroll
-    for (int m = 0; m < 5; m++) B[m] = pB[m];
+    const int r0 = oc.r0, r1 = oc.r1, cy = y0 - b1 + 1;
+    const int B[2] = {oc.B0, oc.B1};
     if (r0 != r1) {
       const int cnt = min(CAP, r1 - r0);
       __syncthreads();  // the previous row has been read
@@ -507,11 +612,14 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
           const int s = p2g_slot(i);
 #pragma unroll
           for (int f = 0; f < 13; f++) L[f * CP + s] = pre[k][f];
+          Lz[s] = (8 - prz[k]) * CP * (int)sizeof(T);
         }
       }
       __syncthreads();
     }
-    if (ri + 1 < nrow) fetch(ri + 1);
+    oc = on;
+    if (ri + 1 < nrow) fetch(oc);
+    on = offsets(ri + 2);
     if (r0 != r1) {
       if (live) sum_chunk(o, cy, r0, min(CAP, r1 - r0), B);
       for (int c0 = r0 + CAP; c0 < r1; c0 += CAP) {  // a row beyond one chunk: stage the rest in place
@@ -522,6 +630,7 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
 #pragma unroll
           for (int f = 0; f < 12; f++) L[f * CP + s] = W[(int64_t)f * n + c0 + i];
           L[12 * CP + s] = vc[c0 + i];
+          Lz[s] = (8 - zb[c0 + i]) * CP * (int)sizeof(T);
         }
         __syncthreads();
         if (live) sum_chunk(o, cy, c0, cnt, B);
@@ -555,11 +664,20 @@ static_assert(P2G_TY == 8, "the row switch in k_p2g_gather lists the y-offsets 3
 // every segment walks three slabs more than it has planes.
 template <typename T, bool NORM>
 void launch_p2g_gather(hipStream_t st, GridArgs<T> ga, int gx, int gy, int gz, int n, const int* offs, const T* W,
-                       const T* vc, T* G, T* Wt, uint8_t* valid) {
+                       const T* vc, T* G, T* Wt, uint8_t* valid, DBuf& tiles) {
   const int nzt = (gz + P2G_TZ - 1) / P2G_TZ, nyt = (gy + P2G_TY - 1) / P2G_TY;
   const int nseg = std::max(1, std::min((2048 + nzt * nyt - 1) / (nzt * nyt), gx / 8));
-  k_p2g_gather<T, NORM><<<nzt * nyt * nseg, P2G_TZ * P2G_WAVES, 0, st>>>(ga, gx, gy, gz, n, nzt, nyt, nseg, offs, W,
-                                                                        vc, G, Wt, valid);
+  const int nwg = nzt * nyt * nseg;
+  const int* order = nullptr;
+  if (nwg <= P2G_MAX_TILES) {
+    unsigned long long* keys = tiles.as<unsigned long long>();
+    int* ord = reinterpret_cast<int*>(keys + P2G_MAX_TILES);
+    k_p2g_tile_work<<<nwg, TPB, 0, st>>>(gx, gy, gz, nzt, nyt, nseg, ga.nx + 1, ga.ny + 1, ga.nz + 1, offs, keys);
+    k_p2g_tile_order<<<1, 1024, 0, st>>>(nwg, keys, ord);
+    order = ord;
+  }
+  k_p2g_gather<T, NORM><<<nwg, P2G_TZ * P2G_WAVES, 0, st>>>(ga, gx, gy, gz, n, nzt, nyt, nseg, order, offs, W, vc, G,
+                                                            Wt, valid);
 }
 
 // NaiveReconstructor::reconstruct (FluidSim/include/FluidSim/cpu/sdf.h:210-225) as a particle
@@ -1208,8 +1326,10 @@ int p2g_t(fs_ctx* c) {
                                                                  c->vel[a].as<T>(), c->p2g_w.as<T>(),
                                                                  c->sorted[0].as<T>());
     int gx = c->fdim[a][0], gy = c->fdim[a][1], gz = c->fdim[a][2];
+    if (!c->p2g_tiles.p) FS_TRY(fs_alloc(c, c->p2g_tiles, (size_t)P2G_MAX_TILES * 12));
     launch_p2g_gather<T, true>(c->stream, ga, gx, gy, gz, n, c->offs.as<int>(), c->p2g_w.as<T>(),
-                               c->sorted[0].as<T>(), c->g[a].as<T>(), c->gw[a].as<T>(), c->gvalid[a].as<uint8_t>());
+                               c->sorted[0].as<T>(), c->g[a].as<T>(), c->gw[a].as<T>(), c->gvalid[a].as<uint8_t>(),
+                               c->p2g_tiles);
     fs_prof_end(c, pf);
     FS_HIP(c, hipGetLastError());
     if (c->cfg.advection == FS_ADVECT_FLIP)
@@ -1773,8 +1893,9 @@ int p2g_sum_t(fs_ctx* c, DBuf* pos, DBuf* coef, DBuf* out) {
                                                                  pos[1].as<T>(), pos[2].as<T>(), coef[a].as<T>(),
                                                                  c->p2g_w.as<T>(), c->sorted[0].as<T>());
     const int gx = c->fdim[a][0], gy = c->fdim[a][1], gz = c->fdim[a][2];
+    if (!c->p2g_tiles.p) FS_TRY(fs_alloc(c, c->p2g_tiles, (size_t)P2G_MAX_TILES * 12));
     launch_p2g_gather<T, false>(c->stream, ga, gx, gy, gz, n, c->offs.as<int>(), c->p2g_w.as<T>(),
-                                c->sorted[0].as<T>(), out[a].as<T>(), nullptr, nullptr);
+                                c->sorted[0].as<T>(), out[a].as<T>(), nullptr, nullptr, c->p2g_tiles);
     FS_HIP(c, hipGetLastError());
   }
   return FS_OK;
