@@ -1478,20 +1478,32 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
 }
 
 // pg += alpha s (saxpy, :332); s = z + beta s (scaleAndAdd, :341) unless the solve stopped.
-// The iterate x is FP64 in both precisions; s and z have the storage type.
+// The iterate x is FP64 in both precisions; s and z have the storage type. One thread per 4
+// consecutive elements (ne is a multiple of 64), with 4-wide loads and stores.
 template <typename S>
 __global__ __launch_bounds__(256) void k_update(int64_t ne, double* __restrict__ x, S* __restrict__ s,
                                                 const S* __restrict__ z, PcgScalars* sc, int iter) {
+  typedef S S4 __attribute__((ext_vector_type(4)));
+  typedef double D4 __attribute__((ext_vector_type(4)));
   const int st = sc->converged;
   if (st == STOP_ABORT) return;
   if (st == STOP_CONVERGED && sc->conv_iter != iter) return;
   const double alpha = sc->alpha;
   const S beta = (S)sc->beta;
-  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t e = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * 4;
   if (e >= ne) return;
-  const S sv = s[e];
-  x[e] = x[e] + alpha * (double)sv;
-  if (st == STOP_RUN) s[e] = z[e] + beta * sv;
+  const S4 sv = *reinterpret_cast<const S4*>(s + e);
+  D4 xv = *reinterpret_cast<const D4*>(x + e);
+#pragma unroll
+  for (int q = 0; q < 4; q++) xv[q] = xv[q] + alpha * (double)sv[q];
+  *reinterpret_cast<D4*>(x + e) = xv;
+  if (st == STOP_RUN) {
+    const S4 zv = *reinterpret_cast<const S4*>(z + e);
+    S4 nv;
+#pragma unroll
+    for (int q = 0; q < 4; q++) nv[q] = zv[q] + beta * sv[q];
+    *reinterpret_cast<S4*>(s + e) = nv;
+  }
 }
 
 template <typename S>
@@ -1816,7 +1828,7 @@ int solve_t(fs_ctx* c, fs_stats* st) {
           FS_TRY(launch_fwd<S>(c, true, it));
           FS_TRY(launch_bwd<S>(c, false));
           pf = fs_prof_begin(c, FS_KERNEL_PCG_UPDATE, (16.0 + 3.0 * sizeof(S)) * c->n_active_last);
-          k_update<S><<<fs_blocks(P.ne, 256), 256, 0, c->stream>>>(P.ne, P.x.as<double>(), y.s, y.z,
+          k_update<S><<<fs_blocks(P.ne / 4, 256), 256, 0, c->stream>>>(P.ne, P.x.as<double>(), y.s, y.z,
                                                                    P.scal.as<PcgScalars>(), it);
           fs_prof_end(c, pf);
         }
