@@ -43,13 +43,26 @@ __device__ inline T coord1(int i, T off, T h) {
   return ((T)i + off) * h + T(0);
 }
 
+// Quotient a / b for interpolation weights: IEEE division in FP64; in FP32 (the mixed-precision
+// mode) a times the hardware reciprocal of b (v_rcp_f32, 1 ulp), within about 2 ulp of the
+// quotient, in place of the ten-instruction correctly rounded division. Cell and bin indices
+// keep the exact division in both precisions.
+template <typename T>
+__device__ inline T wdiv(T a, T b) {
+  return a / b;
+}
+template <>
+__device__ inline float wdiv<float>(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
+}
+
 // Cubic B-spline kernel assumed for the absent Core/transfer-stencil.h CubicKernel.
 template <typename T>
 __device__ inline T bspline(T r) {
   if (r < T(1)) return (T(2) / T(3) - r * r) + T(0.5) * r * r * r;
   if (r < T(2)) {
     T t = T(2) - r;
-    return t * t * t / T(6);
+    return sizeof(T) == 4 ? t * t * t * T(1.0 / 6.0) : t * t * t / T(6);
   }
   return T(0);
 }
@@ -92,9 +105,9 @@ __device__ inline T sdf_eval(const T* __restrict__ g, int nx, int ny, int nz, co
   int x = fclamp(ix, 0, nx - 1), y = fclamp(iy, 0, ny - 1), z = fclamp(iz, 0, nz - 1);
   int xn = fclamp(ix + 1, 0, nx - 1), yn = fclamp(iy + 1, 0, ny - 1), zn = fclamp(iz + 1, 0, nz - 1);
   T cx = coord1(x, T(0.5), h[0]), cy = coord1(y, T(0.5), h[1]), cz = coord1(z, T(0.5), h[2]);
-  T tx = fclamp((px - cx) / h[0], T(0), T(1));
-  T ty = fclamp((py - cy) / h[1], T(0), T(1));
-  T tz = fclamp((pz - cz) / h[2], T(0), T(1));
+  T tx = fclamp(wdiv(px - cx, h[0]), T(0), T(1));
+  T ty = fclamp(wdiv(py - cy, h[1]), T(0), T(1));
+  T tz = fclamp(wdiv(pz - cz, h[2]), T(0), T(1));
   T r = tx * ty * tz * g[lin3(x, y, z, ny, nz)];
   r = r + tx * ty * (1 - tz) * g[lin3(x, y, zn, ny, nz)];
   r = r + tx * (1 - ty) * tz * g[lin3(x, yn, z, ny, nz)];
@@ -118,7 +131,7 @@ __device__ inline V3<T> sdf_gradient(const T* __restrict__ g, int nx, int ny, in
   T gx = g[lin3(xn, y, z, ny, nz)] - g[lin3(xp, y, z, ny, nz)];
   T gy = g[lin3(x, yn, z, ny, nz)] - g[lin3(x, yp, z, ny, nz)];
   T gz = g[lin3(x, y, zn, ny, nz)] - g[lin3(x, y, zp, ny, nz)];
-  return {gx / dx, gy / dy, gz / dz};
+  return {wdiv(gx, dx), wdiv(gy, dy), wdiv(gz, dz)};
 }
 
 // SDF::grad (sdf.h:86-118): mirrored trilinear blend of node gradients, t unclamped.
@@ -129,9 +142,9 @@ __device__ inline V3<T> sdf_grad(const T* __restrict__ g, int nx, int ny, int nz
   int iz = (int)((pz - T(0)) / h[2] - T(0.5));
   int x = fclamp(ix, 0, nx - 1), y = fclamp(iy, 0, ny - 1), z = fclamp(iz, 0, nz - 1);
   int xn = fclamp(ix + 1, 0, nx - 1), yn = fclamp(iy + 1, 0, ny - 1), zn = fclamp(iz + 1, 0, nz - 1);
-  T tx = (px - coord1(x, T(0.5), h[0])) / h[0];
-  T ty = (py - coord1(y, T(0.5), h[1])) / h[1];
-  T tz = (pz - coord1(z, T(0.5), h[2])) / h[2];
+  T tx = wdiv(px - coord1(x, T(0.5), h[0]), h[0]);
+  T ty = wdiv(py - coord1(y, T(0.5), h[1]), h[1]);
+  T tz = wdiv(pz - coord1(z, T(0.5), h[2]), h[2]);
   const int xs[8] = {x, x, x, x, xn, xn, xn, xn};
   const int ys[8] = {y, y, yn, yn, y, y, yn, yn};
   const int zs[8] = {z, zn, z, zn, z, zn, z, zn};

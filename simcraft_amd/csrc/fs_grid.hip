@@ -79,7 +79,7 @@ __device__ inline void make_stencil(const T p[3], int axis, const T h[3], const 
 #pragma unroll
     for (int o = 0; o < 4; o++) {
       T pn = coord1(nb - 1 + o, off, h[c]);
-      st.w[c][o] = bspline(fabs(p[c] - pn) / hs[c]);
+      st.w[c][o] = bspline(wdiv(fabs(p[c] - pn), hs[c]));
     }
   }
 }
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(TPB) void k_p2g_records(int n, int axis, GridArgs<T
 #pragma unroll
     for (int o = 0; o < 4; o++) {
       const T pn = coord1(nb - 1 + o, off, ga.h[c]);
-      W[(int64_t)(4 * c + o) * n + t] = bspline(fabs(pp[c] - pn) / ga.h[c]);
+      W[(int64_t)(4 * c + o) * n + t] = bspline(wdiv(fabs(pp[c] - pn), ga.h[c]));
     }
   }
 }
@@ -1686,9 +1686,9 @@ __device__ inline void make_stencil_d(const T p[3], int axis, const T h[3], cons
 #pragma unroll
     for (int o = 0; o < 4; o++) {
       const T dl = p[c] - coord1(nb - 1 + o, off, h[c]);
-      const T r = fabs(dl) / hs[c];
+      const T r = wdiv(fabs(dl), hs[c]);
       st.w[c][o] = bspline(r);
-      st.d[c][o] = bspline_d(r) * (dl < T(0) ? T(-1) : T(1)) / hs[c];
+      st.d[c][o] = wdiv(bspline_d(r) * (dl < T(0) ? T(-1) : T(1)), hs[c]);
     }
   }
 }
