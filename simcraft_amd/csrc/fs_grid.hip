@@ -776,19 +776,41 @@ __global__ __launch_bounds__(TPB) void k_recon_splat(int n, GridArgs<T> ga, cons
       const T dy = yc - p[1];
       const T dxy = dx2 + dy * dy;
       if (dxy > s_max || !in_range(1, yc)) continue;
-      for (int k = lo[2]; k <= hi[2]; k++) {
-        const T zc = coord1(k, T(0.5), ga.h[2]);
-        const T dz = zc - p[2];
-        const T d2 = dxy + dz * dz;
-        if (d2 > s_max || !in_range(2, zc)) continue;
-        UB* cell = dcopy + lin3(i, j, k, ga.ny, ga.nz);
-        const UB b = Bits<T>::of(d2);
+      // the cells of the z-run in batches of 8: candidates first, then their current minima
+      // (independent loads, one round trip), then the lowering atomics (no return value)
+      UB* const row = dcopy + lin3(i, j, 0, ga.ny, ga.nz);
+      for (int k0 = lo[2]; k0 <= hi[2]; k0 += 8) {
+        UB bq[8], cq[8];
+        bool ok[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const int k = k0 + q;
+          ok[q] = false;
+          bq[q] = 0;
+          if (k > hi[2]) continue;
+          const T zc = coord1(k, T(0.5), ga.h[2]);
+          const T dz = zc - p[2];
+          const T d2 = dxy + dz * dz;
+          if (d2 > s_max || !in_range(2, zc)) continue;
+          ok[q] = true;
+          bq[q] = Bits<T>::of(d2);
+        }
         if (local) {
-          if (b < __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-            __hip_atomic_fetch_min(cell, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+            cq[q] = ok[q] ? __hip_atomic_load(row + k0 + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0;
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+            if (ok[q] && bq[q] < cq[q])
+              __hip_atomic_fetch_min(row + k0 + q, bq[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
-          if (b < __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            __hip_atomic_fetch_min(cell, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+            cq[q] = ok[q] ? __hip_atomic_load(row + k0 + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+            if (ok[q] && bq[q] < cq[q])
+              __hip_atomic_fetch_min(row + k0 + q, bq[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
     }
