@@ -390,13 +390,17 @@ __global__ void k_pack(Lay L, double tau, const TI* __restrict__ Adiag, const TI
 }
 
 // E factor sweep (project-solver.cc:237-274), one wave per brick, tickets in dependency order.
-// Chunks of U slots: the chunk's inputs are loaded together, then the U steps run.
+// Chunks of UF slots: the inputs of chunk c + 1 (flags, diagonal, three off-diagonals and the
+// six tau terms) are loaded while chunk c runs, so the per-slot chain (two divisions per
+// neighbour term, a square root and a reciprocal) never waits on memory.
 __global__ __launch_bounds__(64) void k_factor_sweep(Lay L, double sigma, const double* __restrict__ adiag,
                                                      const double* __restrict__ al, const double* __restrict__ ad,
                                                      const double* __restrict__ ab, const uint16_t* __restrict__ tfl,
                                                      const double* __restrict__ tnum, double* __restrict__ E,
                                                      double* X, uint64_t* ticket, uint64_t tbase,
                                                      const int* __restrict__ order, PcgScalars* sc) {
+  constexpr int UF = 4;
+  static_assert(U % UF == 0, "the slot count is a multiple of U");
   const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
   const int B = take_ticket(ticket, tbase, order, L.nlive);
   const int2 ij = L.bij[B];
@@ -405,30 +409,43 @@ __global__ __launch_bounds__(64) void k_factor_sweep(Lay L, double sigma, const 
   const int S = L.S;
   const bool cL = a == 0 && BL >= 0, cD = b == 0 && BD >= 0;
   const int64_t base = (int64_t)B * S * 64 + lane;
-  double eprev = 0.0;
-  for (int s0 = 0; s0 < S; s0 += U) {
-    int tf[U];
-    double dg[U], vl[U], vd[U], vb[U], xl[U], xd[U];
+  struct In {
+    int tf[UF];
+    double dg[UF], vl[UF], vd[UF], vb[UF], tn[6][UF];
+  };
+  auto load = [&](In& in, int s0) {
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int u = 0; u < UF; u++) {
       const int64_t e = base + (int64_t)(s0 + u) * 64;
-      tf[u] = tfl[e];
-      dg[u] = adiag[e];
-      vl[u] = al[e];
-      vd[u] = ad[e];
-      vb[u] = ab[e];
+      in.tf[u] = tfl[e];
+      in.dg[u] = adiag[e];
+      in.vl[u] = al[e];
+      in.vd[u] = ad[e];
+      in.vb[u] = ab[e];
+#pragma unroll
+      for (int q = 0; q < 6; q++) in.tn[q][u] = tnum[q * L.ne + e];
+    }
+  };
+  In cur, nxt;
+  load(cur, 0);
+  double eprev = 0.0;
+  for (int s0 = 0; s0 < S; s0 += UF) {
+    if (s0 + UF < S) load(nxt, s0 + UF);
+    double xl[UF], xd[UF];
+#pragma unroll
+    for (int u = 0; u < UF; u++) {
       const int s = s0 + u;
       xl[u] = (cL && s + 7 < S) ? xload(xent(X, L, 0, BL, s + 7, b)) : 0.0;
       xd[u] = (cD && s + 7 < S) ? xload(xent(X, L, 1, BD, s + 7, a)) : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int u = 0; u < UF; u++) {
       const int s = s0 + u;
       if (cL && s + 7 < S) xl[u] = xpoll(xent(X, L, 0, BL, s + 7, b), xl[u], sc);
       if (cD && s + 7 < S) xd[u] = xpoll(xent(X, L, 1, BD, s + 7, a), xd[u], sc);
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int u = 0; u < UF; u++) {
       const int s = s0 + u;
       const int64_t e = base + (int64_t)s * 64;
       double EL = __shfl_up(eprev, 8, 64);
@@ -436,28 +453,28 @@ __global__ __launch_bounds__(64) void k_factor_sweep(Lay L, double sigma, const 
       if (a == 0) EL = xl[u];
       if (b == 0) ED = xd[u];
       double ev = 0.0;
-      const int f = tf[u];
+      const int f = cur.tf[u];
       if (f & TF_DO) {
-        double x = dg[u];
+        double x = cur.dg[u];
         if (f & TF_GL) {
-          double t = vl[u] * EL;
+          double t = cur.vl[u] * EL;
           x -= t * t;
-          if (f & TF_L1) x -= tnum[e] / (EL * EL);
-          if (f & TF_L2) x -= tnum[L.ne + e] / (EL * EL);
+          if (f & TF_L1) x -= cur.tn[0][u] / (EL * EL);
+          if (f & TF_L2) x -= cur.tn[1][u] / (EL * EL);
         }
         if (f & TF_GD) {
-          double t = vd[u] * ED;
+          double t = cur.vd[u] * ED;
           x -= t * t;
-          if (f & TF_D1) x -= tnum[2 * L.ne + e] / (ED * ED);
-          if (f & TF_D2) x -= tnum[3 * L.ne + e] / (ED * ED);
+          if (f & TF_D1) x -= cur.tn[2][u] / (ED * ED);
+          if (f & TF_D2) x -= cur.tn[3][u] / (ED * ED);
         }
         if (f & TF_GB) {
-          double t = vb[u] * eprev;
+          double t = cur.vb[u] * eprev;
           x -= t * t;
-          if (f & TF_B1) x -= tnum[4 * L.ne + e] / (eprev * eprev);
-          if (f & TF_B2) x -= tnum[5 * L.ne + e] / (eprev * eprev);
+          if (f & TF_B1) x -= cur.tn[4][u] / (eprev * eprev);
+          if (f & TF_B2) x -= cur.tn[5][u] / (eprev * eprev);
         }
-        if (x < sigma * dg[u]) x = dg[u];
+        if (x < sigma * cur.dg[u]) x = cur.dg[u];
         ev = 1.0 / sqrt(x);
       }
       E[e] = ev;
@@ -466,11 +483,12 @@ __global__ __launch_bounds__(64) void k_factor_sweep(Lay L, double sigma, const 
       eprev = ev;
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int u = 0; u < UF; u++) {
       const int s = s0 + u;
       if (cL && s + 7 < S) xstore(xent(X, L, 0, BL, s + 7, b), xsent());
       if (cD && s + 7 < S) xstore(xent(X, L, 1, BD, s + 7, a), xsent());
     }
+    cur = nxt;
   }
 }
 
