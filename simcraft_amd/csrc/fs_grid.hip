@@ -290,19 +290,25 @@ __global__ __launch_bounds__(TPB) void k_count(int n, const int* __restrict__ ke
   const int i = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
   const bool ok = i < n;
   const int key = ok ? keys[i] : -1;
+  // lanes with equal keys form a group; the group's first lane adds the group size, all leaders
+  // in one atomic instruction, and every lane takes its leader's old count plus its rank
   uint64_t todo = __builtin_amdgcn_ballot_w64(ok);
-  int slot = 0;
+  int myleader = lane, myoff = 0, cnt = 0;
   while (todo != 0) {
     const int leader = __builtin_ctzll(todo);
-    const int lk = __shfl(key, leader, 64);
+    const int lk = __builtin_amdgcn_readlane(key, leader);
     const uint64_t grp = __builtin_amdgcn_ballot_w64(key == lk) & todo;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&counts[lk], __builtin_popcountll(grp));
-    base = __shfl(base, leader, 64);
-    if ((grp >> lane) & 1) slot = base + __builtin_popcountll(grp & ((1ull << lane) - 1));
+    if (lane == leader) cnt = __builtin_popcountll(grp);
+    if ((grp >> lane) & 1) {
+      myleader = leader;
+      myoff = __builtin_popcountll(grp & ((1ull << lane) - 1));
+    }
     todo &= ~grp;
   }
-  if (ok) slots[i] = slot;
+  int base = 0;
+  if (cnt > 0) base = atomicAdd(&counts[key], cnt);
+  base = __shfl(base, myleader, 64);
+  if (ok) slots[i] = base + myoff;
 }
 
 __global__ __launch_bounds__(TPB) void k_scatter(int n, const int* __restrict__ keys, const int* __restrict__ slots,
@@ -316,6 +322,30 @@ __global__ __launch_bounds__(TPB) void k_sort_bins(int nb, const int* __restrict
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   int s = offs[b], e = offs[b + 1];
+  if (e - s <= 1) return;
+  if (e - s <= 16) {  // in registers: load, sorting network (INT_MAX padding), store
+    int v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = s + q < e ? perm[s + q] : INT_MAX;
+#pragma unroll
+    for (int size = 2; size <= 16; size <<= 1)
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+          const int r = q ^ stride;
+          if (r > q) {
+            const bool up = (q & size) == 0;
+            const int lo = min(v[q], v[r]), hi = max(v[q], v[r]);
+            v[q] = up ? lo : hi;
+            v[r] = up ? hi : lo;
+          }
+        }
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+      if (s + q < e) perm[s + q] = v[q];
+    return;
+  }
   for (int i = s + 1; i < e; i++) {
     int v = perm[i];
     int j = i - 1;
