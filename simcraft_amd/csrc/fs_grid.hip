@@ -743,12 +743,14 @@ __global__ __launch_bounds__(TPB) void k_bin_last(int n, const int* __restrict__
   const bool ok = i < n;
   const int key = ok ? keys[i] : -1;
   uint64_t todo = __builtin_amdgcn_ballot_w64(ok);
+  bool last = false;  // the highest lane of its key group; all of them update in one instruction
   while (todo != 0) {
-    const int lk = __shfl(key, __builtin_ctzll(todo), 64);
+    const int lk = __builtin_amdgcn_readlane(key, __builtin_ctzll(todo));
     const uint64_t grp = __builtin_amdgcn_ballot_w64(key == lk) & todo;
-    if (lane == 63 - __builtin_clzll(grp)) atomicMax(&binmax[lk], i);
+    if (lane == 63 - __builtin_clzll(grp)) last = true;
     todo &= ~grp;
   }
+  if (last) atomicMax(&binmax[key], i);
 }
 
 template <typename T>
