@@ -1315,10 +1315,19 @@ __global__ __launch_bounds__(256) void k_fwd(Lay L, T* __restrict__ r, const T* 
         T qD = dpp_row<DPP_ROW_SHR1>(qprev);
         qL = a == 0 ? x0 : qL;
         qD = b == 0 ? x1 : qD;
-        T tt = rv[k] - lv[k] * qL;
-        tt = tt - dv[k] * qD;
-        tt = tt - bv[k] * qprev;
-        const T qv = tt * ev[k];
+        T qv;
+        if constexpr (sizeof(T) == 4) {
+          // FP32 (mixed precision): q = rE - (bE) q_prev - (dE) qD - (lE) qL with the products
+          // by E formed off the chain, three fused multiply-adds, the lane shuffle's operand last
+          T tt = fmaf(-bv[k] * ev[k], qprev, rv[k] * ev[k]);
+          tt = fmaf(-dv[k] * ev[k], qD, tt);
+          qv = fmaf(-lv[k] * ev[k], qL, tt);
+        } else {
+          T tt = rv[k] - lv[k] * qL;
+          tt = tt - dv[k] * qD;
+          tt = tt - bv[k] * qprev;
+          qv = tt * ev[k];
+        }
         sh.ob[slot][u * 64 + lane] = qv;
         qprev = qv;
         xw.post(c * U + u + 1);
@@ -1467,10 +1476,17 @@ __global__ __launch_bounds__(256) void k_bwd(Lay L, const T* __restrict__ qv, co
         T zU = dpp_row<DPP_ROW_SHL1>(zprev);
         zR = a == 7 ? x0 : zR;
         zU = b == 7 ? x1 : zU;
-        T tt = qv[k] - v1[k] * zR;
-        tt = tt - v2[k] * zU;
-        tt = tt - v3[k] * zprev;
-        const T zv = tt * ev[k];
+        T zv;
+        if constexpr (sizeof(T) == 4) {  // FP32: as the forward step, the shuffle's operand last
+          T tt = fmaf(-v3[k] * ev[k], zprev, qv[k] * ev[k]);
+          tt = fmaf(-v2[k] * ev[k], zU, tt);
+          zv = fmaf(-v1[k] * ev[k], zR, tt);
+        } else {
+          T tt = qv[k] - v1[k] * zR;
+          tt = tt - v2[k] * zU;
+          tt = tt - v3[k] * zprev;
+          zv = tt * ev[k];
+        }
         sh.ob[slot][(U - 1 - u) * 64 + lane] = zv;
         zprev = zv;
         xw.post(c * U + u + 1);
