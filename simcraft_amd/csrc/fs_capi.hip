@@ -346,7 +346,7 @@ int fs_set_particles(fs_ctx* c, const double* pos, const double* vel, int64_t n)
     FS_TRY(fs_alloc(c, c->slots, (size_t)n * sizeof(int)));
     FS_TRY(fs_alloc(c, c->perm, (size_t)n * sizeof(int)));
     for (int q = 0; q < 4; q++) FS_TRY(fs_alloc(c, c->sorted[q], (size_t)n * c->rsz));
-    FS_TRY(fs_alloc(c, c->p2g_w, (size_t)n * 13 * c->rsz));  // 12 factors + the z-bin (int)
+    FS_TRY(fs_alloc(c, c->p2g_w, (size_t)n * 12 * c->rsz));
     c->np = n;
   }
   return fs_particles_upload(c, pos, vel);

@@ -135,7 +135,7 @@ struct fs_ctx {
   // binning
   DBuf keys, slots, perm, counts, offs, scan_tmp;
   DBuf sorted[4];   // bin-ordered velocity component (P2G); spare particle-sized scratch
-  DBuf p2g_w;       // 12 kernel factors per particle in bin order, then its z-bin (P2G)
+  DBuf p2g_w;       // 12 kernel factors per particle in bin order (P2G)
   DBuf recon_tmp;   // per-cell squared-distance minima, one array per XCD and a spare (reconstruction)
   DBuf p2g_tiles;   // P2G gather tile keys and heaviest-first tile order
   size_t scan_tmp_bytes = 0;

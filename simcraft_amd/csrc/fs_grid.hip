@@ -362,7 +362,6 @@ __global__ __launch_bounds__(TPB) void k_sort_bins(int nb, const int* __restrict
 // around TGrid::nearest. A particle in bin b (its nearest node) meets node n at offset
 // n - b + 1 on each axis, so the node weight CubicKernel::weight(h, p - pn) is
 // (W[ox] * W[4+oy]) * W[8+oz], the same product the reference forms for that pair.
-// W holds 12 * n factors followed by n ints: the particle's z-bin b2 (read by the gather).
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_p2g_records(int n, int axis, GridArgs<T> ga, const int* __restrict__ perm, const T* __restrict__ px,
                               const T* __restrict__ py, const T* __restrict__ pz, const T* __restrict__ vc,
@@ -376,7 +375,6 @@ __global__ __launch_bounds__(TPB) void k_p2g_records(int n, int axis, GridArgs<T
   for (int c = 0; c < 3; c++) {
     const T off = (c == axis) ? T(0) : T(0.5);
     const int nb = nearest1(pp[c], off, ga.h[c]);
-    if (c == 2) reinterpret_cast<int*>(W + (int64_t)12 * n)[t] = nb;
 #pragma unroll
     for (int o = 0; o < 4; o++) {
       const T pn = coord1(nb - 1 + o, off, ga.h[c]);
@@ -416,25 +414,20 @@ __device__ __forceinline__ int p2g_slot(int i) { return i + (i >> 3); }
 
 // One lane over its particles [t0, t1) of a staged row whose y-offset is C = y0 - b1 + 1:
 // y-node j (y = y0 + j) takes the y-factor W[4 + j + C] and is reached when 0 <= j + C <= 3.
-// The z-factor of lane z for a particle of z-bin b2 is W[8 + z + 1 - b2]: its byte offset is
-// Lz[slot] (= (8 - b2) * CP * sizeof(T), staged) + lzb (= (z + 1) * CP * sizeof(T)) + slot bytes.
-// The x-factor is that of the wave's plane. FP64 forms (wx * wy) * wz, v * w and the sums in the
-// reference's order; FP32 (mixed precision) reassociates to (v wx wz) * wy and (wx wz) * wy,
-// two fused multiply-adds per node. Lz of the next particle is read one iteration ahead (Lz has
-// one entry beyond the last slot of a full chunk), so the z-factor load does not wait for it.
+// The z-factor index is 3 - (bins of the lane's four passed, from the bin bounds B1 .. B3 in
+// registers), the x-factor that of the wave's plane. FP64 forms (wx * wy) * wz, v * w and the
+// sums in the reference's order; FP32 (mixed precision) reassociates to (v wx wz) * wy and
+// (wx wz) * wy, two fused multiply-adds per node. The kernel is bound by LDS reads (seven per
+// particle and lane, at irregular slot spacing), so the bin compares stay in registers.
 template <typename T, bool NORM, int C>
-__device__ __forceinline__ void p2g_row(const T* __restrict__ L, const int* __restrict__ Lz, int o, int t0, int t1,
-                                        int c0, int lzb, T (&acc)[P2G_TY], T (&ws)[P2G_TY]) {
+__device__ __forceinline__ void p2g_row(const T* __restrict__ L, int o, int t0, int t1, int c0, int B1, int B2,
+                                        int B3, T (&acc)[P2G_TY], T (&ws)[P2G_TY]) {
   constexpr int CP = P2GCap<T>::padded;
-  const char* Lb = reinterpret_cast<const char*>(L);
-  if (t0 >= t1) return;
-  int sn = p2g_slot(t0 - c0), zn = Lz[sn];
   for (int t = t0; t < t1; t++) {
-    const int s = sn, zo = zn;
-    sn = p2g_slot(t + 1 - c0);
-    zn = Lz[sn];
+    const int s = p2g_slot(t - c0);
+    const int iz = 3 - (int)(t >= B1) - (int)(t >= B2) - (int)(t >= B3);
     const T wxo = L[o * CP + s];
-    const T wz = *reinterpret_cast<const T*>(Lb + (zo + lzb + s * (int)sizeof(T)));
+    const T wz = L[(8 + iz) * CP + s];
     const T v = L[12 * CP + s];
     if constexpr (sizeof(T) == 4) {
       const T xz = wxo * wz, vxz = v * xz;
@@ -454,7 +447,6 @@ __device__ __forceinline__ void p2g_row(const T* __restrict__ L, const int* __re
         if (NORM) ws[j] = ws[j] + w;
       }
     }
-    asm volatile("" ::"v"(zn));  // keeps the next Lz load in this iteration
   }
 }
 
@@ -543,8 +535,7 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
                                                                    uint8_t* __restrict__ valid) {
   constexpr int CAP = P2GCap<T>::value, CP = P2GCap<T>::padded;
   __shared__ T L[13 * CP];
-  __shared__ int Lz[CP + 1];
-  const int* __restrict__ zb = reinterpret_cast<const int*>(W + (int64_t)12 * n);
+
   // Without an order, workgroups go round-robin over the 8 XCDs (id = 8 * k + xcd) and each
   // XCD takes a contiguous run of tiles.
   int tile;
@@ -561,7 +552,6 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int z = z0 + lane;
   const bool zok = z < z1;
-  const int lzb = (z + 1) * CP * (int)sizeof(T);
   const int zlo = g.zlo, zhi = g.zhi;
   T acc[P2G_TY], ws[P2G_TY];
 #pragma unroll
@@ -572,19 +562,18 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
   const int b1s = g.b1s, nb1 = g.nb1, nrow = g.nrow;
   constexpr int NT = P2G_TZ * P2G_WAVES, PF = (CAP + NT - 1) / NT;
   T pre[PF][13];
-  int prz[PF];
   struct RowOff {
-    int r0, r1, B0, B1;  // the row's records [r0, r1); the lane's bins z-2 .. z+1 hold [B0, B1)
+    int r0, r1, B[5];  // the row's records [r0, r1); the lane's bins z-2 .. z+1 start at B[0 .. 3], end at B[4]
   };
   auto offsets = [&](int ri) {
-    RowOff q{0, 0, 0, 0};
+    RowOff q{0, 0, {0, 0, 0, 0, 0}};
     const int b0 = xs - 2 + ri / nb1, b1 = b1s + ri % nb1;
     if (ri >= nrow || b0 < 0 || b0 >= bx) return q;
     const int row = bz * (b1 + by * b0);
     q.r0 = offs[row + zlo];
     q.r1 = offs[row + zhi + 1];
-    q.B0 = offs[row + (zok ? max(z - 2, 0) : zlo)];
-    q.B1 = zok ? offs[row + min(z + 2, bz)] : q.B0;
+#pragma unroll
+    for (int m = 0; m < 5; m++) q.B[m] = zok ? offs[row + min(max(z - 2 + m, 0), bz)] : q.r0;
     return q;
   };
   auto fetch = [&](const RowOff& q) {
@@ -596,16 +585,15 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
 #pragma unroll
         for (int f = 0; f < 12; f++) pre[k][f] = W[(int64_t)f * n + pr0 + i];
         pre[k][12] = vc[pr0 + i];
-        prz[k] = zb[pr0 + i];
       }
     }
   };
-  auto sum_chunk = [&](int o, int cy, int c0, int cnt, const int (&B)[2]) {
-    const int t0 = max(B[0], c0), t1 = min(B[1], c0 + cnt);
+  auto sum_chunk = [&](int o, int cy, int c0, int cnt, const int (&B)[5]) {
+    const int t0 = max(B[0], c0), t1 = min(B[4], c0 + cnt);
     switch (cy) {
 #define FS_P2G_ROW(CC) \
   case CC:             \
-    p2g_row<T, NORM, CC>(L, Lz, o, t0, t1, c0, lzb, acc, ws); \
+    p2g_row<T, NORM, CC>(L, o, t0, t1, c0, B[1], B[2], B[3], acc, ws); \
     break;
       FS_P2G_ROW(3)
       FS_P2G_ROW(2)
@@ -631,7 +619,7 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
     const int x = b0 - 1 + o;
     const bool live = x >= xs && x < xe;
     const int r0 = oc.r0, r1 = oc.r1, cy = y0 - b1 + 1;
-    const int B[2] = {oc.B0, oc.B1};
+    const int B[5] = {oc.B[0], oc.B[1], oc.B[2], oc.B[3], oc.B[4]};
     if (r0 != r1) {
       const int cnt = min(CAP, r1 - r0);
       __syncthreads();  // the previous row has been read
@@ -642,7 +630,6 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
           const int s = p2g_slot(i);
 #pragma unroll
           for (int f = 0; f < 13; f++) L[f * CP + s] = pre[k][f];
-          Lz[s] = (8 - prz[k]) * CP * (int)sizeof(T);
         }
       }
       __syncthreads();
@@ -660,7 +647,6 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
 #pragma unroll
           for (int f = 0; f < 12; f++) L[f * CP + s] = W[(int64_t)f * n + c0 + i];
           L[12 * CP + s] = vc[c0 + i];
-          Lz[s] = (8 - zb[c0 + i]) * CP * (int)sizeof(T);
         }
         __syncthreads();
         if (live) sum_chunk(o, cy, c0, cnt, B);
