@@ -408,6 +408,14 @@ struct P2GCap {
   static constexpr int padded = value + value / 8;
 };
 
+// Staged chunk layout: fields 0-3 (x-factors), 8-11 (z-factors) and 12 (value) as arrays of CP
+// slots; the four y-factors of a slot together at [4 CP + 4 slot] (one 16-byte read per particle).
+template <typename T>
+__device__ __forceinline__ T& p2g_field(T* L, int f, int s) {
+  constexpr int CP = P2GCap<T>::padded;
+  return (f >= 4 && f < 8) ? L[4 * CP + 4 * s + (f - 4)] : L[f * CP + s];
+}
+
 // LDS slot of particle i of a chunk: one pad word after every 8, so that the lanes of a wave,
 // whose particles lie about one bin (about 8 particles) apart, fall into different banks.
 __device__ __forceinline__ int p2g_slot(int i) { return i + (i >> 3); }
@@ -426,15 +434,17 @@ __device__ __forceinline__ void p2g_row(const T* __restrict__ L, int o, int t0, 
   for (int t = t0; t < t1; t++) {
     const int s = p2g_slot(t - c0);
     const int iz = 3 - (int)(t >= B1) - (int)(t >= B2) - (int)(t >= B3);
+    typedef T T4 __attribute__((ext_vector_type(4)));
     const T wxo = L[o * CP + s];
     const T wz = L[(8 + iz) * CP + s];
     const T v = L[12 * CP + s];
+    const T4 wy4 = *reinterpret_cast<const T4*>(L + 4 * CP + 4 * s);
     if constexpr (sizeof(T) == 4) {
       const T xz = wxo * wz, vxz = v * xz;
 #pragma unroll
       for (int j = 0; j < P2G_TY; j++) {
         if (j + C < 0 || j + C > 3) continue;
-        const T wy = L[(4 + j + C) * CP + s];
+        const T wy = wy4[j + C];
         acc[j] = fmaf(vxz, wy, acc[j]);
         if (NORM) ws[j] = fmaf(xz, wy, ws[j]);
       }
@@ -442,7 +452,7 @@ __device__ __forceinline__ void p2g_row(const T* __restrict__ L, int o, int t0, 
 #pragma unroll
       for (int j = 0; j < P2G_TY; j++) {
         if (j + C < 0 || j + C > 3) continue;
-        const T w = wxo * L[(4 + j + C) * CP + s] * wz;
+        const T w = wxo * wy4[j + C] * wz;
         acc[j] = acc[j] + v * w;
         if (NORM) ws[j] = ws[j] + w;
       }
@@ -534,7 +544,7 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
                                                                    T* __restrict__ G, T* __restrict__ Wt,
                                                                    uint8_t* __restrict__ valid) {
   constexpr int CAP = P2GCap<T>::value, CP = P2GCap<T>::padded;
-  __shared__ T L[13 * CP];
+  __shared__ __align__(16) T L[13 * CP];
 
   // Without an order, workgroups go round-robin over the 8 XCDs (id = 8 * k + xcd) and each
   // XCD takes a contiguous run of tiles.
@@ -629,7 +639,7 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
         if (i < cnt) {
           const int s = p2g_slot(i);
 #pragma unroll
-          for (int f = 0; f < 13; f++) L[f * CP + s] = pre[k][f];
+          for (int f = 0; f < 13; f++) p2g_field(L, f, s) = pre[k][f];
         }
       }
       __syncthreads();
@@ -645,7 +655,7 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
         for (int i = threadIdx.x; i < cnt; i += NT) {
           const int s = p2g_slot(i);
 #pragma unroll
-          for (int f = 0; f < 12; f++) L[f * CP + s] = W[(int64_t)f * n + c0 + i];
+          for (int f = 0; f < 12; f++) p2g_field(L, f, s) = W[(int64_t)f * n + c0 + i];
           L[12 * CP + s] = vc[c0 + i];
         }
         __syncthreads();
