@@ -389,17 +389,18 @@ __global__ __launch_bounds__(TPB) void k_p2g_records(int n, int axis, GridArgs<T
 // then ug = sum(v w) / sum(w) where the weight sum is positive, else 0 and not valid.
 //
 // A workgroup owns a tile of up to P2G_TZ consecutive z-nodes (one per lane) by up to P2G_TY
-// y-nodes (in each lane's registers) over a segment [xs, xe) of x-planes. It walks the bin
-// slabs b0 = xs-2 .. xe and, inside a slab, the bin rows b1 = y0-2 .. y1 in ascending order.
-// The bins z0-2 .. z1 of a row hold one contiguous run of bin-ordered particle records, which
-// the workgroup copies to LDS with coalesced loads. Wave k owns the x-planes x with
-// (x - xs) % 4 == k: at slab b0 the four planes b0-1 .. b0+2 belong to four different waves,
-// so every wave uses every staged row, and plane b0-1 is complete (its wave writes and clears
-// it) once slab b0 is done. A lane walks the particles of bins z-2 .. z+1 of the row in
-// order, so every node still sees its contributions in (b0, b1, b2, index) order.
+// y-nodes over a segment [xs, xe) of x-planes. It walks the bin slabs b0 = xs-2 .. xe and,
+// inside a slab, the bin rows b1 = y0-2 .. y1 in ascending order. The bins z0-2 .. z1 of a row
+// hold one contiguous run of bin-ordered particle records, which the workgroup copies to LDS
+// with coalesced loads. Wave k owns the x-planes x with (x - xs) % 2 == k % 2 and half of the
+// y-nodes (k / 2): at slab b0 two of the four planes b0-1 .. b0+2 belong to each wave, a wave
+// reads the shared factors of a particle once for both, and skips a row that reaches none of
+// its y-nodes; plane b0-1 is complete (its wave writes it) once slab b0 is done. A lane walks
+// the particles of bins z-2 .. z+1 of the row in order, so every node still sees its
+// contributions in (b0, b1, b2, index) order.
 constexpr int P2G_TZ = 64;    // z-nodes per workgroup, one per lane of a wave
-constexpr int P2G_TY = 8;     // y-nodes per lane
-constexpr int P2G_WAVES = 4;  // one per x-plane of the four a particle reaches
+constexpr int P2G_TY = 8;     // y-nodes per tile (4 per wave)
+constexpr int P2G_WAVES = 4;  // two x-parities by two y-halves
 
 // Particles per staged chunk (13 fields each); a row with more is staged in several chunks.
 template <typename T>
@@ -420,44 +421,73 @@ __device__ __forceinline__ T& p2g_field(T* L, int f, int s) {
 // whose particles lie about one bin (about 8 particles) apart, fall into different banks.
 __device__ __forceinline__ int p2g_slot(int i) { return i + (i >> 3); }
 
-// One lane over its particles [t0, t1) of a staged row whose y-offset is C = y0 - b1 + 1:
-// y-node j (y = y0 + j) takes the y-factor W[4 + j + C] and is reached when 0 <= j + C <= 3.
-// The z-factor index is 3 - (bins of the lane's four passed, from the bin bounds B1 .. B3 in
-// registers), the x-factor that of the wave's plane. FP64 forms (wx * wy) * wz, v * w and the
-// sums in the reference's order; FP32 (mixed precision) reassociates to (v wx wz) * wy and
-// (wx wz) * wy, two fused multiply-adds per node. The kernel is bound by LDS reads (seven per
-// particle and lane, at irregular slot spacing), so the bin compares stay in registers.
+// One lane over its particles [t0, t1) of a staged row, for the wave's two x-planes (offsets oN
+// and oN + 2 from b0 - 1) and its P2G_YH y-nodes, whose y-offset is C = y0 + YH h - b1 + 1: y-node
+// j takes the y-factor W[4 + j + C] and is reached when 0 <= j + C <= 3. The z-factor index is
+// 3 - (bins of the lane's four passed, from the bin bounds B1 .. B3 in registers). FP64 forms
+// (wx * wy) * wz, v * w and the sums in the reference's order; FP32 (mixed precision)
+// reassociates to (v wx wz) * wy and (wx wz) * wy, two fused multiply-adds per node. The loop is
+// bound by LDS reads (at the irregular slot spacing of the bins): five per particle and lane for
+// two planes, the four y-factors in one 16-byte read.
+constexpr int P2G_YH = P2G_TY / 2;  // y-nodes per wave: the waves split the tile's y-nodes in two
 template <typename T, bool NORM, int C>
-__device__ __forceinline__ void p2g_row(const T* __restrict__ L, int o, int t0, int t1, int c0, int B1, int B2,
-                                        int B3, T (&acc)[P2G_TY], T (&ws)[P2G_TY]) {
+__device__ __forceinline__ void p2g_row(const T* __restrict__ L, int oN, int t0, int t1, int c0, int B1, int B2,
+                                        int B3, T (&acc)[2][P2G_YH], T (&ws)[2][P2G_YH]) {
   constexpr int CP = P2GCap<T>::padded;
-  for (int t = t0; t < t1; t++) {
+  typedef T T4 __attribute__((ext_vector_type(4)));
+  if (t0 >= t1) return;
+  // The operands of the next particle are read while the current one is summed (up to two slots
+  // past the chunk are read after the last one: the LDS array has room for them), so a particle
+  // does not wait for its own LDS reads.
+  struct Op {
+    T wx0, wx1, wz, v;
+    T4 wy4;
+  };
+  auto read = [&](int t) __attribute__((always_inline)) {
     const int s = p2g_slot(t - c0);
     const int iz = 3 - (int)(t >= B1) - (int)(t >= B2) - (int)(t >= B3);
-    typedef T T4 __attribute__((ext_vector_type(4)));
-    const T wxo = L[o * CP + s];
-    const T wz = L[(8 + iz) * CP + s];
-    const T v = L[12 * CP + s];
-    const T4 wy4 = *reinterpret_cast<const T4*>(L + 4 * CP + 4 * s);
-    if constexpr (sizeof(T) == 4) {
-      const T xz = wxo * wz, vxz = v * xz;
+    return Op{L[oN * CP + s], L[(oN + 2) * CP + s], L[(8 + iz) * CP + s], L[12 * CP + s],
+              *reinterpret_cast<const T4*>(L + 4 * CP + 4 * s)};
+  };
+  auto sum = [&](const Op& op) __attribute__((always_inline)) {
+    const T wx[2] = {op.wx0, op.wx1};
+    const T wz = op.wz, v = op.v;
+    const T4 wy4 = op.wy4;
 #pragma unroll
-      for (int j = 0; j < P2G_TY; j++) {
-        if (j + C < 0 || j + C > 3) continue;
-        const T wy = wy4[j + C];
-        acc[j] = fmaf(vxz, wy, acc[j]);
-        if (NORM) ws[j] = fmaf(xz, wy, ws[j]);
-      }
-    } else {
+    for (int q = 0; q < 2; q++) {
+      if constexpr (sizeof(T) == 4) {
+        const T xz = wx[q] * wz, vxz = v * xz;
 #pragma unroll
-      for (int j = 0; j < P2G_TY; j++) {
-        if (j + C < 0 || j + C > 3) continue;
-        const T w = wxo * wy4[j + C] * wz;
-        acc[j] = acc[j] + v * w;
-        if (NORM) ws[j] = ws[j] + w;
+        for (int j = 0; j < P2G_YH; j++) {
+          if (j + C < 0 || j + C > 3) continue;
+          acc[q][j] = fmaf(vxz, wy4[j + C], acc[q][j]);
+          if (NORM) ws[q][j] = fmaf(xz, wy4[j + C], ws[q][j]);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < P2G_YH; j++) {
+          if (j + C < 0 || j + C > 3) continue;
+          const T w = wx[q] * wy4[j + C] * wz;
+          acc[q][j] = acc[q][j] + v * w;
+          if (NORM) ws[q][j] = ws[q][j] + w;
+        }
       }
     }
+  };
+  // two particles per trip, operands in two register sets, so the rotation needs no copies
+  Op a = read(t0);
+  int t = t0;
+  for (; t + 1 < t1; t += 2) {
+    Op b = read(t + 1);
+    __builtin_amdgcn_sched_barrier(0);  // the reads are issued before the arithmetic
+    sum(a);
+    asm volatile("" ::"v"(b.wx0), "v"(b.wx1), "v"(b.wz), "v"(b.v), "v"(b.wy4));
+    a = read(t + 2);
+    __builtin_amdgcn_sched_barrier(0);
+    sum(b);
+    asm volatile("" ::"v"(a.wx0), "v"(a.wx1), "v"(a.wz), "v"(a.v), "v"(a.wy4));
   }
+  if (t < t1) sum(a);
 }
 
 // Tile `tile` of a face grid (gx, gy, gz) with bins (bx, by, bz): y-tiles fastest, then
@@ -544,7 +574,7 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
                                                                    T* __restrict__ G, T* __restrict__ Wt,
                                                                    uint8_t* __restrict__ valid) {
   constexpr int CAP = P2GCap<T>::value, CP = P2GCap<T>::padded;
-  __shared__ __align__(16) T L[13 * CP];
+  __shared__ __align__(16) T L[13 * CP + 8];  // + the slots read past a full chunk
 
   // Without an order, workgroups go round-robin over the 8 XCDs (id = 8 * k + xcd) and each
   // XCD takes a contiguous run of tiles.
@@ -563,9 +593,15 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
   const int z = z0 + lane;
   const bool zok = z < z1;
   const int zlo = g.zlo, zhi = g.zhi;
-  T acc[P2G_TY], ws[P2G_TY];
+  // Wave w owns the x-planes x with (x - xs) % 2 == w % 2 and the y-nodes y0 + YH (w / 2) + j.
+  // At slab b0 it holds two planes, b0 - 1 + oN (near) and b0 + 1 + oN (far), oN = 0 or 1; when
+  // oN = 0 the near plane has seen its last slab after this one and the far plane moves up.
+  const int par = wave & 1, yh = wave >> 1, ya = y0 + P2G_YH * yh;
+  T acc[2][P2G_YH], ws[2][P2G_YH];
 #pragma unroll
-  for (int j = 0; j < P2G_TY; j++) acc[j] = ws[j] = T(0);
+  for (int q = 0; q < 2; q++)
+#pragma unroll
+    for (int j = 0; j < P2G_YH; j++) acc[q][j] = ws[q][j] = T(0);
   // Rows in walk order: ri = (b0 - xs + 2) * nb1 + (b1 - b1s). While row ri is summed, the
   // first chunk of row ri + 1 is loaded into registers and the bin offsets of row ri + 2 (the
   // row's record range and the lane's bin bounds) are read, so no load waits on another.
@@ -598,12 +634,12 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
       }
     }
   };
-  auto sum_chunk = [&](int o, int cy, int c0, int cnt, const int (&B)[5]) {
+  auto sum_chunk = [&](int oN, int cy, int c0, int cnt, const int (&B)[5]) {
     const int t0 = max(B[0], c0), t1 = min(B[4], c0 + cnt);
-    switch (cy) {
+    switch (cy) {  // a row with |cy| > 3 reaches none of the wave's y-nodes
 #define FS_P2G_ROW(CC) \
   case CC:             \
-    p2g_row<T, NORM, CC>(L, o, t0, t1, c0, B[1], B[2], B[3], acc, ws); \
+    p2g_row<T, NORM, CC>(L, oN, t0, t1, c0, B[1], B[2], B[3], acc, ws); \
     break;
       FS_P2G_ROW(3)
       FS_P2G_ROW(2)
@@ -612,10 +648,6 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
       FS_P2G_ROW(-1)
       FS_P2G_ROW(-2)
       FS_P2G_ROW(-3)
-      FS_P2G_ROW(-4)
-      FS_P2G_ROW(-5)
-      FS_P2G_ROW(-6)
-      FS_P2G_ROW(-7)
 #undef FS_P2G_ROW
       default:
         break;
@@ -625,10 +657,10 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
   fetch(oc);
   for (int ri = 0; ri < nrow; ri++) {
     const int b0 = xs - 2 + ri / nb1, b1 = b1s + ri % nb1;
-    const int o = (wave + 1 + xs - b0) & 3;  // this wave's plane x = b0 - 1 + o, x - xs = wave (mod 4)
-    const int x = b0 - 1 + o;
-    const bool live = x >= xs && x < xe;
-    const int r0 = oc.r0, r1 = oc.r1, cy = y0 - b1 + 1;
+    const int oN = (par + 1 + xs - b0) & 1;  // near plane x = b0 - 1 + oN, x - xs = par (mod 2)
+    const int x = b0 - 1 + oN;
+    const bool live = x + 2 >= xs && x < xe;  // either plane in the segment
+    const int r0 = oc.r0, r1 = oc.r1, cy = ya - b1 + 1;
     const int B[5] = {oc.B[0], oc.B[1], oc.B[2], oc.B[3], oc.B[4]};
     if (r0 != r1) {
       const int cnt = min(CAP, r1 - r0);
@@ -648,7 +680,7 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
     if (ri + 1 < nrow) fetch(oc);
     on = offsets(ri + 2);
     if (r0 != r1) {
-      if (live) sum_chunk(o, cy, r0, min(CAP, r1 - r0), B);
+      if (live) sum_chunk(oN, cy, r0, min(CAP, r1 - r0), B);
       for (int c0 = r0 + CAP; c0 < r1; c0 += CAP) {  // a row beyond one chunk: stage the rest in place
         const int cnt = min(CAP, r1 - c0);
         __syncthreads();
@@ -659,16 +691,16 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
           L[12 * CP + s] = vc[c0 + i];
         }
         __syncthreads();
-        if (live) sum_chunk(o, cy, c0, cnt, B);
+        if (live) sum_chunk(oN, cy, c0, cnt, B);
       }
     }
-    if (ri % nb1 == nb1 - 1 && o == 0) {  // plane x = b0 - 1 has seen every slab x-2 .. x+1
-      if (live && zok) {
+    if (ri % nb1 == nb1 - 1 && oN == 0) {  // near plane x = b0 - 1 has seen every slab x-2 .. x+1
+      if (x >= xs && x < xe && zok) {
 #pragma unroll
-        for (int j = 0; j < P2G_TY; j++) {
-          if (y0 + j >= y1) break;
-          const int64_t nid = ((int64_t)x * gy + (y0 + j)) * gz + z;
-          const T a = acc[j], s = ws[j];
+        for (int j = 0; j < P2G_YH; j++) {
+          if (ya + j >= y1) break;
+          const int64_t nid = ((int64_t)x * gy + (ya + j)) * gz + z;
+          const T a = acc[0][j], s = ws[0][j];
           if (NORM) {
             G[nid] = s > T(0) ? a / s : T(0);
             valid[nid] = s > T(0) ? 1 : 0;
@@ -679,11 +711,15 @@ __global__ __launch_bounds__(P2G_TZ* P2G_WAVES) void k_p2g_gather(GridArgs<T> ga
         }
       }
 #pragma unroll
-      for (int j = 0; j < P2G_TY; j++) acc[j] = ws[j] = T(0);
+      for (int j = 0; j < P2G_YH; j++) {  // the far plane (x + 2) becomes the near one
+        acc[0][j] = acc[1][j];
+        ws[0][j] = ws[1][j];
+        acc[1][j] = ws[1][j] = T(0);
+      }
     }
   }
 }
-static_assert(P2G_TY == 8, "the row switch in k_p2g_gather lists the y-offsets 3 .. 1 - P2G_TY");
+static_assert(P2G_TY == 8 && P2G_WAVES == 4, "four waves: two x-parities by two halves of the tile's 8 y-nodes");
 
 // Launch of k_p2g_gather over face grid dims (gx, gy, gz). x is cut into segments so that
 // there are about 2048 workgroups (several per CU), but not below 8 planes per segment:
