@@ -85,7 +85,7 @@ void release_all(fs_ctx* c) {
   for (DBuf* b : {&c->fsdf, &c->fsdfbuf, &c->sdfvalid, &c->sdfvalidbuf, &c->collider, &c->pg, &c->adiag, &c->rhs,
                   &c->active, &c->keys, &c->slots, &c->perm, &c->counts, &c->offs, &c->scan_tmp, &c->red,
                   &c->recon_tmp, &c->sorted[0], &c->sorted[1], &c->sorted[2], &c->sorted[3], &c->p2g_w,
-                  &c->p2g_tiles})
+                  &c->p2g_tiles, &c->offs2, &c->p2g_w2, &c->p2g_tiles2})
     fs_free(*b);
   for (int q = 0; q < 6; q++) fs_free(c->an[q]);
   fs_pcg_release(c);
@@ -293,7 +293,11 @@ int fs_create(const fs_config* cfg, fs_ctx** out) {
 void fs_destroy(fs_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->side) (void)hipStreamSynchronize(c->side);
   release_all(c);
+  if (c->p2g_ev_ok)
+    for (int q = 0; q < 6; q++) (void)hipEventDestroy(c->p2g_ev[q]);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->ev_ok)
     for (int q = 0; q < 12; q++) (void)hipEventDestroy(c->ev[q]);
   if (c->pev_ok)
@@ -336,6 +340,7 @@ int fs_set_particles(fs_ctx* c, const double* pos, const double* vel, int64_t n)
     fs_free(c->perm);
     for (int q = 0; q < 4; q++) fs_free(c->sorted[q]);
     fs_free(c->p2g_w);
+    fs_free(c->p2g_w2);
     c->np = 0;
     for (int d = 0; d < 3; d++) {
       FS_TRY(fs_alloc(c, c->pos[d], (size_t)n * c->rsz));
@@ -347,6 +352,7 @@ int fs_set_particles(fs_ctx* c, const double* pos, const double* vel, int64_t n)
     FS_TRY(fs_alloc(c, c->perm, (size_t)n * sizeof(int)));
     for (int q = 0; q < 4; q++) FS_TRY(fs_alloc(c, c->sorted[q], (size_t)n * c->rsz));
     FS_TRY(fs_alloc(c, c->p2g_w, (size_t)n * 12 * c->rsz));
+    FS_TRY(fs_alloc(c, c->p2g_w2, (size_t)n * 12 * c->rsz));
     c->np = n;
   }
   return fs_particles_upload(c, pos, vel);

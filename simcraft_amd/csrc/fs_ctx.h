@@ -138,6 +138,12 @@ struct fs_ctx {
   DBuf p2g_w;       // 12 kernel factors per particle in bin order (P2G)
   DBuf recon_tmp;   // per-cell squared-distance minima, one array per XCD and a spare (reconstruction)
   DBuf p2g_tiles;   // P2G gather tile keys and heaviest-first tile order
+  // second P2G buffer set (bin offsets, records, tile order; the velocity records go to
+  // sorted[1]): a face grid's gather runs on `side` while the next face grid is binned
+  DBuf offs2, p2g_w2, p2g_tiles2;
+  hipStream_t side = nullptr;
+  hipEvent_t p2g_ev[6];  // gather done (face grid a), records ready (3 + a)
+  bool p2g_ev_ok = false;
   size_t scan_tmp_bytes = 0;
   size_t bins_cap = 0;
   DBuf red;  // small reduction scratch

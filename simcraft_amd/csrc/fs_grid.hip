@@ -1282,9 +1282,11 @@ int ensure_bins(fs_ctx* c, size_t nbins) {
   if (nbins <= c->bins_cap && c->counts.p) return FS_OK;
   fs_free(c->counts);
   fs_free(c->offs);
+  fs_free(c->offs2);
   fs_free(c->scan_tmp);
   FS_TRY(fs_alloc(c, c->counts, (nbins + 1) * sizeof(int)));
   FS_TRY(fs_alloc(c, c->offs, (nbins + 1) * sizeof(int)));
+  FS_TRY(fs_alloc(c, c->offs2, (nbins + 1) * sizeof(int)));
   size_t tmp = 0;
   FS_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (int*)nullptr, (int*)nullptr, (int)(nbins + 1), c->stream));
   FS_TRY(fs_alloc(c, c->scan_tmp, tmp + 256));
@@ -1294,18 +1296,19 @@ int ensure_bins(fs_ctx* c, size_t nbins) {
 }
 
 // keys must already be in c->keys; leaves offs[nbins+1] and perm.
-int counting_sort(fs_ctx* c, int nbins) {
+int counting_sort(fs_ctx* c, int nbins, DBuf* offs_out = nullptr) {
   FS_TRY(ensure_bins(c, (size_t)nbins));
+  DBuf& offs = offs_out ? *offs_out : c->offs;
   int n = (int)c->np;
   FS_HIP(c, hipMemsetAsync(c->counts.p, 0, (size_t)(nbins + 1) * sizeof(int), c->stream));
   if (n > 0) k_count<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->counts.as<int>(), c->slots.as<int>());
   size_t tmp = c->scan_tmp_bytes;
-  FS_HIP(c, hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.p, tmp, c->counts.as<int>(), c->offs.as<int>(), nbins + 1,
+  FS_HIP(c, hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.p, tmp, c->counts.as<int>(), offs.as<int>(), nbins + 1,
                                              c->stream));
   if (n > 0) {
-    k_scatter<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->slots.as<int>(), c->offs.as<int>(),
+    k_scatter<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->slots.as<int>(), offs.as<int>(),
                                                        c->perm.as<int>());
-    k_sort_bins<<<fs_blocks(nbins, TPB), TPB, 0, c->stream>>>(nbins, c->offs.as<int>(), c->perm.as<int>());
+    k_sort_bins<<<fs_blocks(nbins, TPB), TPB, 0, c->stream>>>(nbins, offs.as<int>(), c->perm.as<int>());
   }
   FS_HIP(c, hipGetLastError());
   return FS_OK;
@@ -1395,32 +1398,64 @@ int smooth_t(fs_ctx* c, int iters) {
   return FS_OK;
 }
 
+// The side stream and its events, created on first use.
+int ensure_side(fs_ctx* c) {
+  if (!c->side) FS_HIP(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  if (!c->p2g_ev_ok) {
+    for (int q = 0; q < 6; q++) FS_HIP(c, hipEventCreateWithFlags(&c->p2g_ev[q], hipEventDisableTiming));
+    c->p2g_ev_ok = true;
+  }
+  return FS_OK;
+}
+
+// P2G of the three face grids. The gather of face grid a runs on the side stream while face grid
+// a + 1 is binned on the main stream (two buffer sets, alternating); the main stream waits for the
+// gathers before the stage ends. With per-launch event timing on, everything stays on the main
+// stream, so that the timed intervals hold the whole stage.
 template <typename T>
 int p2g_t(fs_ctx* c) {
   GridArgs<T> ga = grid_args<T>(c);
   int n = (int)c->np;
   int nb = (c->nx + 1) * (c->ny + 1) * (c->nz + 1);
+  const bool overlap = !(c->cfg.profile_stages >= 2 && c->ktime_on);
+  if (overlap) FS_TRY(ensure_side(c));
+  if (!c->p2g_tiles.p) FS_TRY(fs_alloc(c, c->p2g_tiles, (size_t)P2G_MAX_TILES * 12));
+  if (overlap && !c->p2g_tiles2.p) FS_TRY(fs_alloc(c, c->p2g_tiles2, (size_t)P2G_MAX_TILES * 12));
   for (int a = 0; a < 3; a++) {
+    const int set = overlap ? (a & 1) : 0;
+    DBuf& offs = set ? c->offs2 : c->offs;
+    DBuf& W = set ? c->p2g_w2 : c->p2g_w;
+    DBuf& sv = c->sorted[set];
+    DBuf& tiles = set ? c->p2g_tiles2 : c->p2g_tiles;
+    if (overlap && a >= 2) FS_HIP(c, hipStreamWaitEvent(c->stream, c->p2g_ev[a - 2], 0));  // set free again
     if (n > 0)
       k_keys_nearest<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, a, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
                                                                   c->pos[2].as<T>(), c->keys.as<int>());
     const int pf = fs_prof_begin(c, FS_KERNEL_P2G, (double)n * 4 * sizeof(T) + (double)c->n_face[a] * 2 * sizeof(T));
-    FS_TRY(counting_sort(c, nb));
+    FS_TRY(counting_sort(c, nb, &offs));
     if (n > 0)
       k_p2g_records<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, a, ga, c->perm.as<int>(), c->pos[0].as<T>(),
                                                                  c->pos[1].as<T>(), c->pos[2].as<T>(),
-                                                                 c->vel[a].as<T>(), c->p2g_w.as<T>(),
-                                                                 c->sorted[0].as<T>());
+                                                                 c->vel[a].as<T>(), W.as<T>(), sv.as<T>());
+    hipStream_t gs = c->stream;
+    if (overlap) {
+      FS_HIP(c, hipEventRecord(c->p2g_ev[3 + a], c->stream));
+      FS_HIP(c, hipStreamWaitEvent(c->side, c->p2g_ev[3 + a], 0));
+      gs = c->side;
+    }
     int gx = c->fdim[a][0], gy = c->fdim[a][1], gz = c->fdim[a][2];
-    if (!c->p2g_tiles.p) FS_TRY(fs_alloc(c, c->p2g_tiles, (size_t)P2G_MAX_TILES * 12));
-    launch_p2g_gather<T, true>(c->stream, ga, gx, gy, gz, n, c->offs.as<int>(), c->p2g_w.as<T>(),
-                               c->sorted[0].as<T>(), c->g[a].as<T>(), c->gw[a].as<T>(), c->gvalid[a].as<uint8_t>(),
-                               c->p2g_tiles);
+    launch_p2g_gather<T, true>(gs, ga, gx, gy, gz, n, offs.as<int>(), W.as<T>(), sv.as<T>(), c->g[a].as<T>(),
+                               c->gw[a].as<T>(), c->gvalid[a].as<uint8_t>(), tiles);
     fs_prof_end(c, pf);
     FS_HIP(c, hipGetLastError());
     if (c->cfg.advection == FS_ADVECT_FLIP)
       FS_HIP(c, hipMemcpyAsync(c->glast[a].p, c->g[a].p, (size_t)c->n_face[a] * sizeof(T), hipMemcpyDeviceToDevice,
-                               c->stream));
+                               gs));
+    if (overlap) FS_HIP(c, hipEventRecord(c->p2g_ev[a], c->side));
+  }
+  if (overlap) {
+    FS_HIP(c, hipStreamWaitEvent(c->stream, c->p2g_ev[1], 0));
+    FS_HIP(c, hipStreamWaitEvent(c->stream, c->p2g_ev[2], 0));
   }
   return FS_OK;
 }
