@@ -85,7 +85,7 @@ void release_all(fs_ctx* c) {
   for (DBuf* b : {&c->fsdf, &c->fsdfbuf, &c->sdfvalid, &c->sdfvalidbuf, &c->collider, &c->pg, &c->adiag, &c->rhs,
                   &c->active, &c->keys, &c->slots, &c->perm, &c->counts, &c->offs, &c->scan_tmp, &c->red,
                   &c->recon_tmp, &c->sorted[0], &c->sorted[1], &c->sorted[2], &c->sorted[3], &c->p2g_w,
-                  &c->p2g_tiles, &c->offs2, &c->p2g_w2, &c->p2g_tiles2})
+                  &c->p2g_tiles, &c->offs2, &c->p2g_w2, &c->p2g_tiles2, &c->rkeys, &c->rbinmax})
     fs_free(*b);
   for (int q = 0; q < 6; q++) fs_free(c->an[q]);
   fs_pcg_release(c);
@@ -294,10 +294,14 @@ void fs_destroy(fs_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->side) (void)hipStreamSynchronize(c->side);
+  if (c->side2) (void)hipStreamSynchronize(c->side2);
   release_all(c);
   if (c->p2g_ev_ok)
     for (int q = 0; q < 6; q++) (void)hipEventDestroy(c->p2g_ev[q]);
+  if (c->sdf_ev_ok)
+    for (int q = 0; q < 2; q++) (void)hipEventDestroy(c->sdf_ev[q]);
   if (c->side) (void)hipStreamDestroy(c->side);
+  if (c->side2) (void)hipStreamDestroy(c->side2);
   if (c->ev_ok)
     for (int q = 0; q < 12; q++) (void)hipEventDestroy(c->ev[q]);
   if (c->pev_ok)
@@ -341,6 +345,7 @@ int fs_set_particles(fs_ctx* c, const double* pos, const double* vel, int64_t n)
     for (int q = 0; q < 4; q++) fs_free(c->sorted[q]);
     fs_free(c->p2g_w);
     fs_free(c->p2g_w2);
+    fs_free(c->rkeys);
     c->np = 0;
     for (int d = 0; d < 3; d++) {
       FS_TRY(fs_alloc(c, c->pos[d], (size_t)n * c->rsz));
@@ -353,6 +358,7 @@ int fs_set_particles(fs_ctx* c, const double* pos, const double* vel, int64_t n)
     for (int q = 0; q < 4; q++) FS_TRY(fs_alloc(c, c->sorted[q], (size_t)n * c->rsz));
     FS_TRY(fs_alloc(c, c->p2g_w, (size_t)n * 12 * c->rsz));
     FS_TRY(fs_alloc(c, c->p2g_w2, (size_t)n * 12 * c->rsz));
+    FS_TRY(fs_alloc(c, c->rkeys, (size_t)n * sizeof(int)));
     c->np = n;
   }
   return fs_particles_upload(c, pos, vel);
@@ -512,9 +518,33 @@ int fs_substep_core(fs_ctx* c, double dt, fs_stats* S, AdjStep* rec) {
   }
   if (rec) FS_TRY(fs_adj_record(c, rec, ADJ_ADVECTED));
   mark(1);
-  FS_TRY(fs_stage_reconstruct(c, c->cfg.recon_radius_factor * c->h[0]));
-  FS_TRY(fs_stage_extrapolate_sdf(c, c->cfg.sdf_extrapolate_iters));
-  FS_TRY(fs_stage_smooth(c, c->cfg.sdf_smooth_iters));
+  // The fluid-SDF stages and the velocity stages (P2G .. force) are independent until the system
+  // is built: without a recording and without per-launch event timing, the SDF stages run on
+  // side2, forked from and joined back into the context's stream.
+  const bool sdf_side = !rec && !(c->cfg.profile_stages >= 2 && c->ktime_on);
+  auto sdf_stages = [&]() -> int {
+    FS_TRY(fs_stage_reconstruct(c, c->cfg.recon_radius_factor * c->h[0]));
+    FS_TRY(fs_stage_extrapolate_sdf(c, c->cfg.sdf_extrapolate_iters));
+    FS_TRY(fs_stage_smooth(c, c->cfg.sdf_smooth_iters));
+    return FS_OK;
+  };
+  if (sdf_side) {
+    if (!c->side2) FS_HIP(c, hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking));
+    if (!c->sdf_ev_ok) {
+      for (int q = 0; q < 2; q++) FS_HIP(c, hipEventCreateWithFlags(&c->sdf_ev[q], hipEventDisableTiming));
+      c->sdf_ev_ok = true;
+    }
+    FS_HIP(c, hipEventRecord(c->sdf_ev[0], c->stream));
+    FS_HIP(c, hipStreamWaitEvent(c->side2, c->sdf_ev[0], 0));
+    hipStream_t main_stream = c->stream;
+    c->stream = c->side2;  // the stage functions enqueue on c->stream
+    const int st = sdf_stages();
+    c->stream = main_stream;
+    FS_TRY(st);
+    FS_HIP(c, hipEventRecord(c->sdf_ev[1], c->side2));
+  } else {
+    FS_TRY(sdf_stages());
+  }
   mark(2);
   FS_TRY(fs_stage_p2g(c));
   if (rec) FS_TRY(fs_adj_record(c, rec, ADJ_P2G));
@@ -524,6 +554,7 @@ int fs_substep_core(fs_ctx* c, double dt, fs_stats* S, AdjStep* rec) {
   FS_TRY(fs_stage_extrapolate_velocity(c, c->cfg.vel_extrapolate_iters));
   FS_TRY(fs_stage_force(c, dt));
   mark(4);
+  if (sdf_side) FS_HIP(c, hipStreamWaitEvent(c->stream, c->sdf_ev[1], 0));  // join the SDF stages
   FS_TRY(fs_stage_build(c, dt));
   if (rec) FS_TRY(fs_adj_record(c, rec, ADJ_BUILT));
   mark(5);

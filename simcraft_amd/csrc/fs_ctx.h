@@ -144,6 +144,12 @@ struct fs_ctx {
   hipStream_t side = nullptr;
   hipEvent_t p2g_ev[6];  // gather done (face grid a), records ready (3 + a)
   bool p2g_ev_ok = false;
+  // reconstruction's own keys and per-bin last particle, so that the fluid-SDF stages (on side2)
+  // can run beside P2G and the velocity stages (on the main stream) within a substep
+  DBuf rkeys, rbinmax;
+  hipStream_t side2 = nullptr;
+  hipEvent_t sdf_ev[2];  // fork (main), join (side2)
+  bool sdf_ev_ok = false;
   size_t scan_tmp_bytes = 0;
   size_t bins_cap = 0;
   DBuf red;  // small reduction scratch

@@ -1348,7 +1348,8 @@ int reconstruct_t(fs_ctx* c, double radius) {
   GridArgs<T> ga = grid_args<T>(c);
   int n = (int)c->np;
   int nb = morton3D(c->nx - 1, c->ny - 1, c->nz - 1) + 1;
-  FS_TRY(ensure_bins(c, (size_t)nb));
+  // own keys and per-bin last particle (rkeys, rbinmax), not the P2G binning buffers
+  if (!c->rbinmax.p) FS_TRY(fs_alloc(c, c->rbinmax, (size_t)nb * sizeof(int)));
   if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)c->n_cell * RC_COPIES * sizeof(double)));
   UB* dmin = c->recon_tmp.as<UB>();
   k_fill_bits<T><<<fs_blocks(c->n_cell * RC_COPIES, TPB), TPB, 0, c->stream>>>(c->n_cell * RC_COPIES, dmin);
@@ -1359,13 +1360,13 @@ int reconstruct_t(fs_ctx* c, double radius) {
   else smax = (T)smax_for_f((float)r2);
   const int pf = fs_prof_begin(c, FS_KERNEL_RECONSTRUCT, (double)n * 3 * sizeof(T) + (double)c->n_cell * (sizeof(T) + 1));
   if (n > 0) {
-    FS_HIP(c, hipMemsetAsync(c->counts.p, 0xFF, (size_t)nb * sizeof(int), c->stream));
+    FS_HIP(c, hipMemsetAsync(c->rbinmax.p, 0xFF, (size_t)nb * sizeof(int), c->stream));
     k_keys_morton<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
-                                                               c->pos[2].as<T>(), c->keys.as<int>());
-    k_bin_last<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->counts.as<int>());
+                                                               c->pos[2].as<T>(), c->rkeys.as<int>());
+    k_bin_last<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->rkeys.as<int>(), c->rbinmax.as<int>());
     k_recon_splat<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
-                                                               c->pos[2].as<T>(), c->keys.as<int>(),
-                                                               c->counts.as<int>(), r2, smax, dmin, c->n_cell);
+                                                               c->pos[2].as<T>(), c->rkeys.as<int>(),
+                                                               c->rbinmax.as<int>(), r2, smax, dmin, c->n_cell);
   }
   k_recon_final<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->n_cell, dmin, rad, c->fsdf.as<T>(),
                                                                      c->sdfvalid.as<uint8_t>());
