@@ -13,8 +13,11 @@
  * Particles are AoS xyz doubles (std::vector<Vec3d> in the reference,
  * FluidSim/include/FluidSim/cpu/fluid-simulator.h:200-202).
  *
- * A context owns every device buffer and one HIP stream. Host arrays are copied at the
- * call and never retained. A context is not re-entrant; contexts are independent.
+ * A context owns every device buffer and its HIP streams: the stream every entry point orders
+ * its work on, plus two internal side streams that a substep forks from and joins back into that
+ * stream before it returns (P2G gathers beside the next face grid's binning; the fluid-SDF stages
+ * beside P2G and the velocity stages). Host arrays are copied at the call and never retained.
+ * A context is not re-entrant; contexts are independent.
  * No entry point aborts the process: failures return a status and fs_last_error()
  * holds the message.
  */
@@ -86,10 +89,18 @@ typedef struct fs_config {
   double recon_radius_factor;     /* fluid-simulator.cc:147 (0.8 * h) */
   int32_t advect_on_cleared_grid; /* 1 = reference order: clear() before advect (fluid-simulator.cc:141-143) */
   double cfl_factor;              /* fluid-simulator.cc:202 (5.0) */
-  double inner_tolerance;         /* unused (kept for layout) */
-  int32_t max_refinements;        /* unused (kept for layout) */
+  double inner_tolerance;         /* unused, kept for layout: the mixed-precision "outer FP64
+                                     refinement" is residual replacement inside one PCG run (the
+                                     FP64 residual b - Ax replaces the FP32 recurrence whenever it
+                                     has dropped tenfold and whenever it claims convergence), not
+                                     nested solves with an inner tolerance */
+  int32_t max_refinements;        /* unused, kept for layout (see inner_tolerance) */
   int32_t check_interval;         /* PCG iterations between host convergence reads */
-  int32_t profile_stages;         /* 1 = record per-stage device times into fs_stats */
+  int32_t profile_stages;         /* 1 = record per-stage device times into fs_stats (the fluid-SDF
+                                     stages then stay on the main stream, so ms_reconstruct is
+                                     their time); 2 = also per-launch kernel timing (see
+                                     fs_kernel_profile); while that timing is switched off, the SDF
+                                     stages overlap P2G and ms_reconstruct is not separable */
   int32_t reserved[7];
 } fs_config;
 
@@ -209,12 +220,14 @@ int fs_set_kernel_timing(fs_ctx* ctx, int32_t on);
  * symmetric matrix), velocity extrapolation, gravity and advection are differentiated exactly.
  * Requires advect_on_cleared_grid = 1 (the reference order). Arrays are AoS xyz doubles. */
 int fs_rollout_begin(fs_ctx* ctx, int32_t checkpoint_every); /* record the following substeps */
+int fs_rollout_stop(fs_ctx* ctx);  /* stop appending substeps; the tape stays for fs_rollout_vjp */
 int fs_rollout_end(fs_ctx* ctx);                             /* stop recording, free the tape */
 int fs_rollout_steps(const fs_ctx* ctx);                     /* substeps recorded so far */
-/* Given dL/dpos and dL/dvel of the current (final) particles, returns dL/dpos and dL/dvel of the
- * particles at fs_rollout_begin and dL/dG for the gravity vector G = (0, -gravity, 0) (the
- * generalised force u_a += G_a dt on valid interior faces). Recomputes the rollout segment by
- * segment from checkpoints taken every `checkpoint_every` substeps; restores the final state.
+/* Given dL/dpos and dL/dvel of the particles at the end of the recorded substeps, returns dL/dpos
+ * and dL/dvel of the particles at fs_rollout_begin and dL/dG for the gravity vector
+ * G = (0, -gravity, 0) (the generalised force u_a += G_a dt on valid interior faces). Recomputes
+ * the rollout segment by segment from checkpoints taken every `checkpoint_every` substeps; then
+ * restores the state found at the call (particles, fluid SDF buffers, face grids, pressure).
  * Any output may be NULL. */
 int fs_rollout_vjp(fs_ctx* ctx, const double* dpos, const double* dvel, double* dpos0, double* dvel0,
                    double* dgravity);

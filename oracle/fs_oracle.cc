@@ -34,6 +34,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <omp.h>
 #include <string>
 #include <vector>
 
@@ -663,20 +664,33 @@ struct Sim {
     if (p2g_order == 0)
       std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nb[a] < nb[b]; });
     double o0 = face_off(axis, 0), o1 = face_off(axis, 1), o2 = face_off(axis, 2);
-    for (int idx : order) {
-      V3 p = pos[idx];
-      double vc = axis == 0 ? vel[idx].x : (axis == 1 ? vel[idx].y : vel[idx].z);
-      for (int i = -1; i <= 2; i++)
-        for (int j = -1; j <= 2; j++)
-          for (int k = -1; k <= 2; k++) {
-            int x = nb[idx][0] + i, y = nb[idx][1] + j, zz = nb[idx][2] + k;
-            if (x >= 0 && x < G.nx && y >= 0 && y < G.ny && zz >= 0 && zz < G.nz) {
-              V3 pn{coord1(x, o0, h[0]), coord1(y, o1, h[1]), coord1(zz, o2, h[2])};
-              double w = kweight(h[0], h[1], h[2], p - pn);
-              G(x, y, zz) += vc * w;
-              Wt(x, y, zz) += w;
+    // Every node sums its particles in `order`. The node planes x are split into slabs, one
+    // per thread; a thread walks the whole order and adds only into its own planes, so each
+    // node still receives its terms in exactly that order (the serial loop's rounding).
+    const int nslab = std::max(1, std::min(G.nx, 4 * omp_get_max_threads()));
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int sl = 0; sl < nslab; sl++) {
+      const int xlo = (int)((int64_t)G.nx * sl / nslab), xhi = (int)((int64_t)G.nx * (sl + 1) / nslab);
+      for (int idx : order) {
+        const int b0 = nb[idx][0];
+        if (b0 + 2 < xlo || b0 - 1 >= xhi) continue;
+        V3 p = pos[idx];
+        double vc = axis == 0 ? vel[idx].x : (axis == 1 ? vel[idx].y : vel[idx].z);
+        for (int i = -1; i <= 2; i++) {
+          int x = b0 + i;
+          if (x < xlo || x >= xhi) continue;
+          for (int j = -1; j <= 2; j++)
+            for (int k = -1; k <= 2; k++) {
+              int y = nb[idx][1] + j, zz = nb[idx][2] + k;
+              if (x >= 0 && x < G.nx && y >= 0 && y < G.ny && zz >= 0 && zz < G.nz) {
+                V3 pn{coord1(x, o0, h[0]), coord1(y, o1, h[1]), coord1(zz, o2, h[2])};
+                double w = kweight(h[0], h[1], h[2], p - pn);
+                G(x, y, zz) += vc * w;
+                Wt(x, y, zz) += w;
+              }
             }
-          }
+        }
+      }
     }
   }
   // p2g_order 2: the reference's parallel structure (TBB parallel_for over particles,

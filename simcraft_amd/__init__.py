@@ -244,6 +244,10 @@ class FluidSim:
         """Record the following substeps for rollout_vjp (checkpoints every `checkpoint_every`)."""
         self._check(self._lib.fs_rollout_begin(self._ctx, checkpoint_every))
 
+    def rollout_stop(self) -> None:
+        """Stop appending substeps to the recorded rollout; the tape stays for rollout_vjp."""
+        self._check(self._lib.fs_rollout_stop(self._ctx))
+
     def rollout_end(self) -> None:
         self._check(self._lib.fs_rollout_end(self._ctx))
 

@@ -102,6 +102,7 @@ SIGNATURES = {
     "fs_kernel_profile_reset": (C.c_int, [_P]),
     "fs_set_kernel_timing": (C.c_int, [_P, C.c_int32]),
     "fs_rollout_begin": (C.c_int, [_P, C.c_int32]),
+    "fs_rollout_stop": (C.c_int, [_P]),
     "fs_rollout_end": (C.c_int, [_P]),
     "fs_rollout_steps": (C.c_int, [_P]),
     "fs_rollout_vjp": (C.c_int, [_P, _D, _D, _D, _D, _D]),

@@ -6,10 +6,14 @@ differentiable interface, not as the fast path of a forward-only run.
 """
 from __future__ import annotations
 
+import itertools
+
 import numpy as np
 import torch
 
 from . import FluidSim
+
+_ROLLOUT_IDS = itertools.count(1)
 
 
 def _np(t: torch.Tensor) -> np.ndarray:
@@ -22,6 +26,11 @@ class FlipRollout(torch.autograd.Function):
     gravity is the vector G = (0, -g, 0) the simulator was configured with (the reference applies
     gravity to v only); it is an input so that dL/dG is returned. Checkpoints are taken every
     `checkpoint_every` substeps and the backward pass recomputes one segment at a time.
+
+    Recording stops when forward returns, so substeps run on `sim` between forward and backward
+    are not part of the differentiated rollout. One sim holds one tape: a later flip_rollout on
+    the same sim replaces it, and backward of the earlier one then raises instead of returning
+    the gradient of a different rollout.
     """
 
     @staticmethod
@@ -33,13 +42,21 @@ class FlipRollout(torch.autograd.Function):
         sim.rollout_begin(checkpoint_every)
         for dt in dts:
             sim.substep(float(dt))
+        sim.rollout_stop()
         pos, vel = sim.particles()
         ctx.sim = sim
+        ctx.nsteps = len(dts)
+        ctx.rollout_id = next(_ROLLOUT_IDS)
+        sim._rollout_id = ctx.rollout_id
         return torch.from_numpy(pos), torch.from_numpy(vel)
 
     @staticmethod
     def backward(ctx, gpos, gvel):
-        dp0, dv0, dg = ctx.sim.rollout_vjp(None if gpos is None else _np(gpos), None if gvel is None else _np(gvel))
+        sim = ctx.sim
+        if getattr(sim, "_rollout_id", None) != ctx.rollout_id or sim.rollout_steps() != ctx.nsteps:
+            raise RuntimeError("the simulator's recorded rollout is no longer the one this output came from "
+                               "(another rollout was recorded on it since)")
+        dp0, dv0, dg = sim.rollout_vjp(None if gpos is None else _np(gpos), None if gvel is None else _np(gvel))
         return torch.from_numpy(dp0), torch.from_numpy(dv0), torch.from_numpy(dg), None, None, None
 
 

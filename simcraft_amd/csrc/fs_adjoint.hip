@@ -537,10 +537,16 @@ int alloc_work(fs_ctx* c, AdjWork& w) {
   return FS_OK;
 }
 
-int save_checkpoint(fs_ctx* c, AdjCheckpoint& k) {
+// with_grids: also the face grids and the pressure, so that the state a caller can read back
+// after fs_rollout_vjp is the one it left (the recomputation overwrites them).
+int save_checkpoint(fs_ctx* c, AdjCheckpoint& k, bool with_grids = false) {
   for (int d = 0; d < 3; d++) {
     FS_TRY(copy_buf(c, k.x[d], c->pos[d]));
     FS_TRY(copy_buf(c, k.v[d], c->vel[d]));
+  }
+  if (with_grids) {
+    for (int a = 0; a < 3; a++) FS_TRY(copy_buf(c, k.g[a], c->g[a]));
+    FS_TRY(copy_buf(c, k.pg, c->pg));
   }
   FS_TRY(copy_buf(c, k.fsdf, c->fsdf));
   FS_TRY(copy_buf(c, k.fsdfbuf, c->fsdfbuf));
@@ -558,6 +564,10 @@ int load_checkpoint(fs_ctx* c, const AdjCheckpoint& k) {
   FS_TRY(copy_buf(c, c->fsdfbuf, k.fsdfbuf));
   FS_TRY(copy_buf(c, c->sdfvalid, k.sdfvalid));
   FS_TRY(copy_buf(c, c->sdfvalidbuf, k.sdfvalidbuf));
+  if (k.pg.p) {
+    for (int a = 0; a < 3; a++) FS_TRY(copy_buf(c, c->g[a], k.g[a]));
+    FS_TRY(copy_buf(c, c->pg, k.pg));
+  }
   return FS_OK;
 }
 
@@ -565,8 +575,9 @@ void free_checkpoint(AdjCheckpoint& k) {
   for (int d = 0; d < 3; d++) {
     fs_free(k.x[d]);
     fs_free(k.v[d]);
+    fs_free(k.g[d]);
   }
-  for (DBuf* b : {&k.fsdf, &k.fsdfbuf, &k.sdfvalid, &k.sdfvalidbuf}) fs_free(*b);
+  for (DBuf* b : {&k.fsdf, &k.fsdfbuf, &k.sdfvalid, &k.sdfvalidbuf, &k.pg}) fs_free(*b);
 }
 
 void free_step(AdjStep& s) {
@@ -612,7 +623,7 @@ int rollout_vjp_t(fs_ctx* c, const double* dpos, const double* dvel, double* dpo
   AdjWork w;
   int st = alloc_work(c, w);
   double grav[3] = {0, 0, 0};
-  if (st == FS_OK) st = save_checkpoint(c, tp.end);
+  if (st == FS_OK) st = save_checkpoint(c, tp.end, true);
   if (st == FS_OK) st = upload_particles<T>(c, dpos, w.xb);
   if (st == FS_OK) st = upload_particles<T>(c, dvel, w.vb);
   const int every = std::max(1, tp.every);
@@ -740,6 +751,7 @@ void fs_adj_release(fs_ctx* c) {
   tp.seg.clear();
   tp.dts.clear();
   tp.on = false;
+  tp.recording = false;
 }
 
 extern "C" {
@@ -756,7 +768,14 @@ int fs_rollout_begin(fs_ctx* c, int32_t checkpoint_every) {
   }
   fs_adj_release(c);
   c->tape.on = true;
+  c->tape.recording = true;
   c->tape.every = checkpoint_every;
+  return FS_OK;
+}
+
+int fs_rollout_stop(fs_ctx* c) {
+  if (!c) return FS_BAD_ARG;
+  c->tape.recording = false;
   return FS_OK;
 }
 
@@ -772,7 +791,7 @@ int fs_rollout_vjp(fs_ctx* c, const double* dpos, const double* dvel, double* dp
                    double* dgravity) {
   if (!c) return FS_BAD_ARG;
   if (!c->tape.on) {
-    c->set_error("no rollout is being recorded (fs_rollout_begin)");
+    c->set_error("no rollout has been recorded (fs_rollout_begin)");
     return FS_BAD_ARG;
   }
   return c->mixed ? rollout_vjp_t<float>(c, dpos, dvel, dpos0, dvel0, dgravity)

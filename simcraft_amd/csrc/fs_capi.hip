@@ -259,7 +259,7 @@ int fs_create(const fs_config* cfg, fs_ctx** out) {
   size_t nc = (size_t)c->n_cell;
   for (DBuf* b : {&c->fsdf, &c->fsdfbuf, &c->collider, &c->pg})
     if (st == FS_OK) st = fs_alloc(c, *b, nc * R);
-  // the assembled system is kept in FP64 for the pressure solve
+  // the assembled system, at storage width (the pressure solve widens it to FP64 where needed)
   for (DBuf* b : {&c->adiag, &c->rhs, &c->an[0], &c->an[1], &c->an[2], &c->an[3], &c->an[4], &c->an[5]})
     if (st == FS_OK) st = fs_alloc(c, *b, nc * R);
   for (DBuf* b : {&c->sdfvalid, &c->sdfvalidbuf, &c->active})
@@ -491,7 +491,7 @@ int fs_substep(fs_ctx* c, double dt, fs_stats* st) {
   fs_stats local;
   fs_stats* S = st ? st : &local;
   std::memset(S, 0, sizeof(*S));
-  if (c->tape.on) FS_TRY(fs_adj_on_substep(c, dt));
+  if (c->tape.on && c->tape.recording) FS_TRY(fs_adj_on_substep(c, dt));
   return fs_substep_core(c, dt, S, nullptr);
 }
 
@@ -519,9 +519,20 @@ int fs_substep_core(fs_ctx* c, double dt, fs_stats* S, AdjStep* rec) {
   if (rec) FS_TRY(fs_adj_record(c, rec, ADJ_ADVECTED));
   mark(1);
   // The fluid-SDF stages and the velocity stages (P2G .. force) are independent until the system
-  // is built: without a recording and without per-launch event timing, the SDF stages run on
-  // side2, forked from and joined back into the context's stream.
-  const bool sdf_side = !rec && !(c->cfg.profile_stages >= 2 && c->ktime_on);
+  // is built: without a recording, without per-stage timing (profile_stages 1) and without
+  // per-launch event timing, the SDF stages run on side2, forked from and joined back into the
+  // context's stream. Every return after the fork joins side2 first (SideJoin), so no SDF work
+  // is left running behind a later call that only synchronises the context's stream.
+  const bool sdf_side =
+      !rec && c->cfg.profile_stages != 1 && !(c->cfg.profile_stages >= 2 && c->ktime_on);
+  struct SideJoin {
+    fs_ctx* c = nullptr;
+    ~SideJoin() {
+      if (!c) return;
+      (void)hipEventRecord(c->sdf_ev[1], c->side2);
+      (void)hipStreamWaitEvent(c->stream, c->sdf_ev[1], 0);
+    }
+  } side_join;
   auto sdf_stages = [&]() -> int {
     FS_TRY(fs_stage_reconstruct(c, c->cfg.recon_radius_factor * c->h[0]));
     FS_TRY(fs_stage_extrapolate_sdf(c, c->cfg.sdf_extrapolate_iters));
@@ -536,12 +547,12 @@ int fs_substep_core(fs_ctx* c, double dt, fs_stats* S, AdjStep* rec) {
     }
     FS_HIP(c, hipEventRecord(c->sdf_ev[0], c->stream));
     FS_HIP(c, hipStreamWaitEvent(c->side2, c->sdf_ev[0], 0));
+    side_join.c = c;
     hipStream_t main_stream = c->stream;
     c->stream = c->side2;  // the stage functions enqueue on c->stream
     const int st = sdf_stages();
     c->stream = main_stream;
     FS_TRY(st);
-    FS_HIP(c, hipEventRecord(c->sdf_ev[1], c->side2));
   } else {
     FS_TRY(sdf_stages());
   }
@@ -554,7 +565,11 @@ int fs_substep_core(fs_ctx* c, double dt, fs_stats* S, AdjStep* rec) {
   FS_TRY(fs_stage_extrapolate_velocity(c, c->cfg.vel_extrapolate_iters));
   FS_TRY(fs_stage_force(c, dt));
   mark(4);
-  if (sdf_side) FS_HIP(c, hipStreamWaitEvent(c->stream, c->sdf_ev[1], 0));  // join the SDF stages
+  if (sdf_side) {  // join the SDF stages
+    side_join.c = nullptr;
+    FS_HIP(c, hipEventRecord(c->sdf_ev[1], c->side2));
+    FS_HIP(c, hipStreamWaitEvent(c->stream, c->sdf_ev[1], 0));
+  }
   FS_TRY(fs_stage_build(c, dt));
   if (rec) FS_TRY(fs_adj_record(c, rec, ADJ_BUILT));
   mark(5);

@@ -96,9 +96,11 @@ struct AdjStep {
 };
 struct AdjCheckpoint {
   DBuf x[3], v[3], fsdf, fsdfbuf, sdfvalid, sdfvalidbuf;
+  DBuf g[3], pg;  // face grids and pressure: only in the end-of-rollout checkpoint
 };
 struct AdjTape {
-  bool on = false;
+  bool on = false;         // a tape exists (fs_rollout_begin .. fs_rollout_end)
+  bool recording = false;  // substeps are appended to it (until fs_rollout_stop)
   int every = 8;
   std::vector<double> dts;
   std::vector<AdjCheckpoint> ck;
