@@ -15,6 +15,12 @@ The JSON line carries:
   cpu_baseline  the FP64 CPU oracle (a restatement of the reference, which does not build) on
                 a bounded sample: one substep of the same scene at 160^3, scaled by the
                 cell/particle count and the device's PCG iteration count.
+  config.pcg_*  how many timed solves stopped without meeting the stopping rule (FS_NOT_CONVERGED
+                at the iteration cap) and the largest final FP64 residuals.
+  other_stopping_rule  the same workload timed under the other PCG stopping rule: the
+                reference's ||r||_inf < 1e-6 with its cap of 300 (--tol-mode abs, the default
+                headline), or ||r||_2 <= 1e-6 ||rhs||_2 with a cap of 1000 (--tol-mode rel).
+  fp64_step     the same workload with FP64 arithmetic throughout, and the mixed/FP64 ratio.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
 """
@@ -45,6 +51,55 @@ def workload_config(simcraft_amd, wl: dict, tol_mode: int, profile: int):
     return simcraft_amd.default_config(nx=n, ny=n, nz=n, flip_alpha=wl["flip_alpha"], precision=prec,
                                        tol_mode=tol_mode, cg_max_iterations=wl.get("max_iters", 300),
                                        profile_stages=profile)
+
+
+REL_MAX_ITERS = 1000
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def solve_summary(stats: list) -> dict:
+    """Convergence record of the timed substeps: how many solves stopped without meeting the
+    stopping rule, and the largest final residuals (FP64, both norms)."""
+    return {
+        "pcg_not_converged_steps": int(sum(1 for s in stats if s["status"] != 0)),
+        "pcg_max_residual_inf": float(max(s["residual_inf"] for s in stats)),
+        "pcg_max_residual_rel_l2": float(max(s["residual_rel"] for s in stats)),
+    }
+
+
+def timed_run(simcraft_amd, scenes, wl: dict, tol_mode: str, steps: int, warmup: int) -> dict:
+    """Steps/s of `steps` substeps of the workload (after `warmup`) under one stopping rule:
+    "abs" = the reference's ||r||_inf < 1e-6 with its cap of 300 iterations, "rel" =
+    ||r||_2 <= 1e-6 ||rhs||_2 with a cap of REL_MAX_ITERS."""
+    wl = dict(wl, max_iters=300 if tol_mode == "abs" else REL_MAX_ITERS)
+    tm = simcraft_amd.TOL_REL_L2 if tol_mode == "rel" else simcraft_amd.TOL_ABS_LINF
+    sim = simcraft_amd.FluidSim(workload_config(simcraft_amd, wl, tm, 0))
+    pos, vel = scenes.dam_break(wl["n"])
+    sim.set_collider_sdf(scenes.collider_for(wl["collider"], wl["n"]))
+    sim.set_particles(pos, vel)
+    del pos, vel
+    for _ in range(warmup):
+        sim.substep(wl["dt"])
+    sim.synchronize()
+    t0 = time.perf_counter()
+    stats = [sim.substep(wl["dt"]) for _ in range(steps)]
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    sim.close()
+    return {"value": steps / elapsed, "unit": "steps/s", "steps": steps, "warmup": warmup,
+            "ms_per_step": 1000.0 * elapsed / steps, "precision": wl["precision"],
+            "tol_mode": "abs_linf" if tol_mode == "abs" else "rel_l2", "pcg_max_iterations": wl["max_iters"],
+            "pcg_iterations_mean": float(np.mean([s["cg_iterations"] for s in stats])), **solve_summary(stats)}
 
 
 def cpu_baseline(simcraft_amd, scenes, wl: dict, tol_mode: int, gpu_iters: float, cap: int, sample_n: int) -> dict:
@@ -91,6 +146,7 @@ def cpu_baseline(simcraft_amd, scenes, wl: dict, tol_mode: int, gpu_iters: float
         "value": 1.0 / step_s,
         "unit": "steps/s",
         "cores": cores,
+        "cpu_model": cpu_model(),
         "kind": "port",
         "sample": (f"one substep of the same scene at {n}^3 (cells and particles 1/{scale:g} of {n_full}^3), "
                    f"timed per stage and scaled x{scale:g}; its PCG ran {pcg_iters} iterations (cap {cap}, "
@@ -175,6 +231,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="dam_break_sphere_256_mixed")
     ap.add_argument("--tol-mode", choices=["abs", "rel"], default="abs")
+    ap.add_argument("--max-iters", type=int, default=None,
+                    help="PCG iteration cap (default: the reference's 300 for abs, 1000 for rel)")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the secondary lines (the other stopping rule, the FP64 step)")
+    ap.add_argument("--secondary-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=300)
     ap.add_argument("--cpu-sample-n", type=int, default=160)
@@ -203,6 +264,10 @@ def main() -> int:
     tol_mode = simcraft_amd.TOL_REL_L2 if args.tol_mode == "rel" else simcraft_amd.TOL_ABS_LINF
     log(f"workload {args.workload}: {wl}")
     t0 = time.perf_counter()
+    if args.max_iters is not None:
+        wl["max_iters"] = args.max_iters
+    elif args.tol_mode == "rel":
+        wl["max_iters"] = REL_MAX_ITERS
     cfg = workload_config(simcraft_amd, wl, tol_mode, 0 if args.no_kernel_timing else 2)
     sim = simcraft_amd.FluidSim(cfg)
     pos, vel = scenes.dam_break(wl["n"])
@@ -218,14 +283,14 @@ def main() -> int:
     sim.synchronize()
     if not args.no_kernel_timing:
         sim.set_kernel_timing(False)
-    iters, active = [], []
+    stats = []
     t0 = time.perf_counter()
     for s in range(args.steps):
-        st = sim.substep(dt)
-        iters.append(st["cg_iterations"])
-        active.append(st["n_active"])
+        stats.append(sim.substep(dt))
     sim.synchronize()
     elapsed = time.perf_counter() - t0
+    iters = [st["cg_iterations"] for st in stats]
+    active = [st["n_active"] for st in stats]
     log(f"timed {args.steps} steps in {elapsed:.3f}s; PCG iterations {iters}")
     sps = args.steps / elapsed
 
@@ -277,14 +342,31 @@ def main() -> int:
             "precision": wl["precision"],
             "pcg_tolerance": 1e-6,
             "tol_mode": "abs_linf" if args.tol_mode == "abs" else "rel_l2",
+            "pcg_max_iterations": wl.get("max_iters", 300),
             "pcg_iterations_mean": float(np.mean(iters)),
+            **solve_summary(stats),
         },
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": None,
         "adjoint": None,
+        "other_stopping_rule": None,
+        "fp64_step": None,
     }
     sim.close()
+    del sim
+    if not args.no_secondary:
+        other = "rel" if args.tol_mode == "abs" else "abs"
+        log(f"secondary: the same workload with the {other} stopping rule ...")
+        out["other_stopping_rule"] = timed_run(simcraft_amd, scenes, wl, other, args.secondary_steps, args.warmup)
+        log(f"secondary: {out['other_stopping_rule']}")
+        if wl["precision"] == "mixed":
+            wl64 = dict(wl, precision="fp64")
+            log("secondary: the FP64 step of the same workload ...")
+            r64 = timed_run(simcraft_amd, scenes, wl64, args.tol_mode, args.secondary_steps, args.warmup)
+            r64["mixed_over_fp64"] = round(sps / r64["value"], 3)
+            out["fp64_step"] = r64
+            log(f"secondary: {r64}")
     if not args.no_adjoint:
         log("adjoint rollout ...")
         out["adjoint"] = adjoint_ratio(simcraft_amd, scenes, args.adjoint_n, args.adjoint_steps, args.adjoint_every,

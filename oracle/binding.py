@@ -63,6 +63,10 @@ _SIGS = {
     "fso_sample_velocity": (None, [_P, _D, _D]),
     "fso_nearest": (None, [_P, C.c_int, _D, C.POINTER(C.c_int)]),
     "fso_last_iterations": (C.c_int, [_P]),
+    "fso_rollout_begin": (C.c_int, [_P]),
+    "fso_rollout_stop": (C.c_int, [_P]),
+    "fso_rollout_steps": (C.c_int, [_P]),
+    "fso_rollout_vjp": (C.c_int, [_P, _D, _D, _D, _D, _D]),
 }
 _lib = None
 
@@ -271,6 +275,30 @@ class OracleSim:
         out = np.empty(3)
         self._lib.fso_sample_velocity(self._ctx, _dp(pa), _dp(out))
         return out
+
+    def rollout_begin(self):
+        """Record the following substeps for rollout_vjp (all of them are kept in memory)."""
+        if self._lib.fso_rollout_begin(self._ctx) != 0:
+            raise ValueError("the reverse pass needs advect_on_cleared_grid = 1 and smoothing")
+
+    def rollout_stop(self):
+        self._lib.fso_rollout_stop(self._ctx)
+
+    def rollout_steps(self) -> int:
+        return self._lib.fso_rollout_steps(self._ctx)
+
+    def rollout_vjp(self, dpos, dvel):
+        """dL/dpos0, dL/dvel0 and dL/dG of the recorded substeps from dL/dpos, dL/dvel at their end
+        (the FP64 CPU reverse pass)."""
+        n = self.num_particles()
+        dp0, dv0, dg = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(3)
+        a = None if dpos is None else np.ascontiguousarray(dpos, np.float64)
+        b = None if dvel is None else np.ascontiguousarray(dvel, np.float64)
+        st = self._lib.fso_rollout_vjp(self._ctx, None if a is None else _dp(a), None if b is None else _dp(b),
+                                       _dp(dp0), _dp(dv0), _dp(dg))
+        if st != 0:
+            raise RuntimeError(f"oracle reverse pass failed (status {st})")
+        return dp0, dv0, dg
 
     def nearest(self, axis, p):
         pa = np.ascontiguousarray(p, dtype=np.float64)

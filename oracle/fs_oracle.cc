@@ -332,6 +332,19 @@ static double tree_dot_sweep(const PcgLayout& L, const RArr& a, const RArr& b) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// What the reverse pass needs of one substep, recorded while the substep runs forward.
+struct AdjRecord {
+  double dt = 0;
+  std::vector<V3> x0, v0, x1, v1;  // particles at the start and after advection
+  RArr sdfbuf0;                    // fluidSurfaceBuf the substep inherits (its stale buffer, Q3)
+  RArr g0[3], w0[3];               // P2G output (FLIP's u_last) and weight sums
+  BArr valid[3];                   // valid flags after applyDirichletBoundary
+  RArr weights[3], fsdf;           // face weights and the smoothed fluid SDF of buildSystem
+  BArr active;
+  RArr adiag, an[6];               // the assembled system
+  RArr g5[3], pg;                  // the grids G2P reads, and the pressure
+};
+
 struct Sim {
   fs_config cfg{};
   int nx, ny, nz;
@@ -360,6 +373,9 @@ struct Sim {
   PcgLayout layout;
 
   enum Dir { Left, Right, Up, Down, Front, Back };  // project-solver.cc:7-14
+  // reverse mode: substeps recorded since fso_rollout_begin
+  bool recording = false;
+  std::vector<AdjRecord> tape;
 
   explicit Sim(const fs_config& c) : cfg(c) {
     nx = c.nx;
@@ -529,68 +545,88 @@ struct Sim {
     }
   }
 
+  // ParticleNeighbourSearcher::update (Spatify/include/Spatify/ns-util.h:46-79): Morton key of
+  // each particle's clamped cell, indices sorted by key (stable: Q4), bin begin and inclusive end.
+  struct Bins {
+    std::vector<int> perm, cbeg, cend;
+  };
+  Bins binParticles(const std::vector<V3>& P) const {
+    const double* sp = h;  // resetGrid(..., sdf.grid.gridSpacing())
+    int n = (int)P.size();
+    Bins b;
+    std::vector<int> cellmap(n);
+    b.perm.resize(n);
+    for (int i = 0; i < n; i++) {
+      b.perm[i] = i;
+      int ix = (int)std::floor(P[i].x / sp[0]);
+      if (ix < 0) ix = 0;
+      if (ix >= nx) ix = nx - 1;
+      int iy = (int)std::floor(P[i].y / sp[1]);
+      if (iy < 0) iy = 0;
+      if (iy >= ny) iy = ny - 1;
+      int iz = (int)std::floor(P[i].z / sp[2]);
+      if (iz < 0) iz = 0;
+      if (iz >= nz) iz = nz - 1;
+      cellmap[i] = morton3D(ix, iy, iz);
+    }
+    std::stable_sort(b.perm.begin(), b.perm.end(), [&](int a, int c) { return cellmap[a] < cellmap[c]; });
+    std::vector<int> sorted(n);
+    for (int i = 0; i < n; i++) sorted[i] = cellmap[b.perm[i]];
+    // The reference sizes the bin tables W*H*D, which holds every Morton code only for
+    // power-of-two cubes; they are sized to the largest code so other sizes stay in range.
+    size_t nb = (size_t)morton3D(nx - 1, ny - 1, nz - 1) + 1;
+    b.cbeg.assign(nb, -1);
+    b.cend.assign(nb, -1);
+    for (int i = 0; i < n; i++) {
+      int g = sorted[i];
+      if (i == 0 || sorted[i - 1] != g) b.cbeg[g] = i;
+      if (i == n - 1 || sorted[i + 1] != g) b.cend[g] = i;  // inclusive end (Q4)
+    }
+    return b;
+  }
+  // forNeighbours (ns-util.h:80-109) around the centre of cell (i, j, k): fn(index, position) for
+  // every particle of the bins within 2r whose distance is at most 2r, skipping each bin's last
+  // particle (the loop runs to the inclusive end with '<', Q4).
+  template <class F>
+  void forNeighbours(const Bins& b, const std::vector<V3>& P, int i, int j, int k, double r2, F&& fn) const {
+    const double* sp = h;
+    V3 p{coord1(i, 0.5, h[0]), coord1(j, 0.5, h[1]), coord1(k, 0.5, h[2])};
+    int x_min = std::max(0, (int)std::floor((p.x - r2) / sp[0]));
+    int x_max = std::min(nx - 1, (int)std::ceil((p.x + r2) / sp[0]));
+    int y_min = std::max(0, (int)std::floor((p.y - r2) / sp[1]));
+    int y_max = std::min(ny - 1, (int)std::ceil((p.y + r2) / sp[1]));
+    int z_min = std::max(0, (int)std::floor((p.z - r2) / sp[2]));
+    int z_max = std::min(nz - 1, (int)std::ceil((p.z + r2) / sp[2]));
+    for (int a = x_min; a <= x_max; a++)
+      for (int bb = y_min; bb <= y_max; bb++)
+        for (int c = z_min; c <= z_max; c++) {
+          int cell = morton3D(a, bb, c);
+          for (int l = b.cbeg[cell]; l < b.cend[cell]; l++) {
+            const int idx = b.perm[l];
+            V3 q = P[idx];
+            if (glen(p - q) <= r2) fn(idx, p, q);
+          }
+        }
+  }
+
   // NaiveReconstructor<T,3>::reconstruct (FluidSim/include/FluidSim/cpu/sdf.h:210-225) with
   // ParticleNeighbourSearcher::update / forNeighbours (Spatify/include/Spatify/ns-util.h:46-109).
   void reconstruct(double radius) {
     SDF& sdf = *fluidSurface;
     BArr& valid = *sdfValid;
-    int W = nx, H = ny, D = nz;
-    const double* sp = h;  // resetGrid(..., sdf.grid.gridSpacing())
-    int n = (int)pos.size();
-    std::vector<int> cellmap(n), perm(n);
-    for (int i = 0; i < n; i++) {
-      perm[i] = i;
-      int ix = (int)std::floor(pos[i].x / sp[0]);
-      if (ix < 0) ix = 0;
-      if (ix >= W) ix = W - 1;
-      int iy = (int)std::floor(pos[i].y / sp[1]);
-      if (iy < 0) iy = 0;
-      if (iy >= H) iy = H - 1;
-      int iz = (int)std::floor(pos[i].z / sp[2]);
-      if (iz < 0) iz = 0;
-      if (iz >= D) iz = D - 1;
-      cellmap[i] = morton3D(ix, iy, iz);
-    }
-    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return cellmap[a] < cellmap[b]; });
-    std::vector<int> sorted(n);
-    for (int i = 0; i < n; i++) sorted[i] = cellmap[perm[i]];
-    // The reference sizes the bin tables W*H*D, which holds every Morton code only for
-    // power-of-two cubes; they are sized to the largest code so other sizes stay in range.
-    size_t nb = (size_t)morton3D(W - 1, H - 1, D - 1) + 1;
-    std::vector<int> cbeg(nb, -1), cend(nb, -1);
-    for (int i = 0; i < n; i++) {
-      int g = sorted[i];
-      if (i == 0 || sorted[i - 1] != g) cbeg[g] = i;
-      if (i == n - 1 || sorted[i + 1] != g) cend[g] = i;  // inclusive end (Q4)
-    }
+    const Bins b = binParticles(pos);
     sdf.g.fill(1e9);
     valid.fill(0);
     double r2 = 2 * radius;
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int i = 0; i < W; i++)
-      for (int j = 0; j < H; j++)
-        for (int k = 0; k < D; k++) {
-          V3 p{coord1(i, 0.5, h[0]), coord1(j, 0.5, h[1]), coord1(k, 0.5, h[2])};
-          int x_min = std::max(0, (int)std::floor((p.x - r2) / sp[0]));
-          int x_max = std::min(W - 1, (int)std::ceil((p.x + r2) / sp[0]));
-          int y_min = std::max(0, (int)std::floor((p.y - r2) / sp[1]));
-          int y_max = std::min(H - 1, (int)std::ceil((p.y + r2) / sp[1]));
-          int z_min = std::max(0, (int)std::floor((p.z - r2) / sp[2]));
-          int z_max = std::min(D - 1, (int)std::ceil((p.z + r2) / sp[2]));
-          for (int a = x_min; a <= x_max; a++)
-            for (int b = y_min; b <= y_max; b++)
-              for (int c = z_min; c <= z_max; c++) {
-                int cell = morton3D(a, b, c);
-                for (int l = cbeg[cell]; l < cend[cell]; l++) {  // skips the bin's last particle (Q4)
-                  V3 q = pos[perm[l]];
-                  if (glen(p - q) <= r2) {
-                    double dis = glen(q - p);
-                    sdf.g(i, j, k) = std::min(sdf.g(i, j, k), dis - radius);
-                    valid(i, j, k) = 1;
-                  }
-                }
-              }
-        }
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++)
+          forNeighbours(b, pos, i, j, k, r2, [&](int, V3 p, V3 q) {
+            double dis = glen(q - p);
+            sdf.g(i, j, k) = std::min(sdf.g(i, j, k), dis - radius);
+            valid(i, j, k) = 1;
+          });
   }
 
   // extrapolateFluidSdf (FluidSim/include/FluidSim/cpu/fluid-simulator.h:123-166): valid cells
@@ -1289,6 +1325,15 @@ struct Sim {
 
   // substep (fluid-simulator.cc:140-182)
   int substep(double dt, fs_stats* st) {
+    AdjRecord* rec = nullptr;
+    if (recording) {
+      tape.emplace_back();
+      rec = &tape.back();
+      rec->dt = dt;
+      rec->x0 = pos;
+      rec->v0 = vel;
+      rec->sdfbuf0 = fluidSurfaceBuf->g;
+    }
     if (cfg.advect_on_cleared_grid) {
       clear();
       advect(dt);
@@ -1296,17 +1341,49 @@ struct Sim {
       advect(dt);
       clear();
     }
+    if (rec) {
+      rec->x1 = pos;
+      rec->v1 = vel;
+    }
     reconstruct(cfg.recon_radius_factor * h[0]);
     extrapolateFluidSdf(cfg.sdf_extrapolate_iters);
     smoothFluidSurface(cfg.sdf_smooth_iters);
     p2g();
+    if (rec) {
+      const RArr* G[3] = {ug.get(), vg.get(), wg.get()};
+      const RArr* Wt[3] = {&uw, &vw, &ww};
+      for (int a = 0; a < 3; a++) {
+        rec->g0[a] = *G[a];
+        rec->w0[a] = *Wt[a];
+      }
+    }
     applyDirichletBoundary();
+    if (rec) {
+      rec->valid[0] = *uValid;
+      rec->valid[1] = *vValid;
+      rec->valid[2] = *wValid;
+    }
     extrapolateVelocity(cfg.vel_extrapolate_iters);
     applyForce(dt);
     buildSystem(dt);
+    if (rec) {
+      rec->weights[0] = uWeights;
+      rec->weights[1] = vWeights;
+      rec->weights[2] = wWeights;
+      rec->fsdf = fluidSurface->g;
+      rec->active = active;
+      rec->adiag = Adiag;
+      for (int q = 0; q < 6; q++) rec->an[q] = An[q];
+    }
     solvePressure();
     project(dt);
     applyCollider();
+    if (rec) {
+      rec->g5[0] = *ug;
+      rec->g5[1] = *vg;
+      rec->g5[2] = *wg;
+      rec->pg = pg;
+    }
     g2p();
     if (st) fill_stats(st);
     return last_status;
@@ -1387,6 +1464,591 @@ struct Sim {
   }
 };
 
+
+// ---------------------------------------------------------------------------------------------
+// Reverse mode, FP64 and serial: the vector-Jacobian product of a recorded rollout of substeps.
+// The reference has no derivative code; this is the CPU statement of the derivative that the
+// device library's reverse pass computes (simcraft_amd/csrc/fs_adjoint.hip and the particle-side
+// adjoint kernels of fs_grid.hip), written from the forward functions above: each block names the
+// forward it reverses. The discrete structure of a substep is held fixed (marking, active set,
+// face weights, valid masks, min/clamp branches, matrix sparsity); everything else is exact:
+// G2P and P2G including the kernel weights' dependence on positions, the collider and the
+// projection, the pressure solve (lambda = A^-1 pbar, A symmetric), the ghost-fluid theta through
+// the fluid SDF (reconstruction, extrapolation and smoothing, including the SDF buffer one substep
+// hands to the next), the velocity extrapolation with its stale buffers, gravity and advection on
+// the cleared grid (k2 = k3 = 0) with the collision push-out (normal held fixed) and clamps.
+// Where the device gathers, these loops scatter, so the two are independent evaluations.
+
+// dN/dr of the cubic B-spline (r >= 0); N is C2 so the branches meet at r = 1 and r = 2.
+static inline double bspline_d(double r) {
+  if (r < 1.0) return 1.5 * r * r - 2.0 * r;
+  if (r < 2.0) {
+    double t = 2.0 - r;
+    return -0.5 * t * t;
+  }
+  return 0.0;
+}
+
+struct Reverse {
+  Sim& s;
+  double grav[3] = {0, 0, 0};
+  mutable int mismatches = 0;
+  explicit Reverse(Sim& sim) : s(sim) {}
+
+  // The 4x4x4 transfer stencil of face grid a around TGrid::nearest, per axis: base node, kernel
+  // factors N(|p - x_n| / hs) and their derivatives along the axis.
+  struct Stencil {
+    int b[3];
+    double w[3][4], d[3][4];
+  };
+  Stencil stencil(V3 p, int a, const double hs[3]) const {
+    Stencil st;
+    const double pc[3] = {p.x, p.y, p.z};
+    for (int c = 0; c < 3; c++) {
+      const double off = face_off(a, c);
+      st.b[c] = nearest1(pc[c], off, s.h[c]) - 1;
+      for (int o = 0; o < 4; o++) {
+        const double dl = pc[c] - coord1(st.b[c] + o, off, s.h[c]);
+        const double r = std::fabs(dl) / hs[c];
+        st.w[c][o] = bspline(r);
+        st.d[c][o] = bspline_d(r) * (dl < 0.0 ? -1.0 : 1.0) / hs[c];
+      }
+    }
+    return st;
+  }
+  // Visits the in-grid nodes of a stencil: fn(node id, w, dw/dp).
+  template <class F>
+  static void nodes(const Stencil& st, const RArr& G, F&& fn) {
+    for (int i = 0; i < 4; i++) {
+      const int x = st.b[0] + i;
+      if (x < 0 || x >= G.nx) continue;
+      for (int j = 0; j < 4; j++) {
+        const int y = st.b[1] + j;
+        if (y < 0 || y >= G.ny) continue;
+        for (int k = 0; k < 4; k++) {
+          const int z = st.b[2] + k;
+          if (z < 0 || z >= G.nz) continue;
+          const double wx = st.w[0][i], wy = st.w[1][j], wz = st.w[2][k];
+          const double dw[3] = {st.d[0][i] * wy * wz, wx * st.d[1][j] * wz, wx * wy * st.d[2][k]};
+          fn(G.id(x, y, z), wx * wy * wz, dw);
+        }
+      }
+    }
+  }
+  static double& comp(V3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+  static double compc(const V3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+  static void zero_like(RArr& dst, const RArr& src) { dst = RArr(src.nx, src.ny, src.nz); }
+
+  // solveG2P (advect-solver.cc:7-17, 157-169) reversed at positions X: pic_a = sum w g5 / S and
+  // old_a = sum w g0 / S (S = sum w, sampleVelocity's scalar spacing, util.cc:40-85); the new
+  // velocity is alpha v + pic - alpha old (FLIP) or pic (PIC). Adds the position terms to xb,
+  // returns the coefficients vbar / S of the transposed transfer in coef and alpha vbar in vb1.
+  void g2p(const AdjRecord& t, const std::vector<V3>& vb, std::vector<V3>& xb, std::vector<V3>& vb1,
+           std::vector<V3>& coef) const {
+    const bool flip = s.cfg.advection == FS_ADVECT_FLIP;
+    const double alpha = s.cfg.flip_alpha, hs[3] = {s.h[0], s.h[0], s.h[0]};
+    for (size_t p = 0; p < t.x1.size(); p++) {
+      for (int a = 0; a < 3; a++) {
+        const Stencil st = stencil(t.x1[p], a, hs);
+        double S = 0, A = 0, B = 0, dS[3] = {0, 0, 0}, dA[3] = {0, 0, 0}, dB[3] = {0, 0, 0};
+        nodes(st, t.g5[a], [&](size_t id, double w, const double dw[3]) {
+          S += w;
+          A += w * t.g5[a].d[id];
+          B += w * t.g0[a].d[id];
+          for (int d = 0; d < 3; d++) {
+            dS[d] += dw[d];
+            dA[d] += dw[d] * t.g5[a].d[id];
+            dB[d] += dw[d] * t.g0[a].d[id];
+          }
+        });
+        const double v = compc(vb[p], a);
+        double cf = 0.0;
+        if (S > 0.0) {
+          const double pic = A / S, old = B / S;
+          for (int d = 0; d < 3; d++) {
+            double g = dA[d] - pic * dS[d];
+            if (flip) g -= alpha * (dB[d] - old * dS[d]);
+            comp(xb[p], d) += v * g / S;
+          }
+          cf = v / S;
+        }
+        comp(coef[p], a) = cf;
+        comp(vb1[p], a) = flip ? alpha * v : 0.0;
+      }
+    }
+  }
+  // The transposed transfer: out_a(node) = sum over particles of w_pa(node) coef_a(p).
+  void transfer_t(const AdjRecord& t, const std::vector<V3>& coef, RArr out[3]) const {
+    const double hs[3] = {s.h[0], s.h[0], s.h[0]};
+    for (int a = 0; a < 3; a++) zero_like(out[a], t.g5[a]);
+    for (size_t p = 0; p < t.x1.size(); p++)
+      for (int a = 0; a < 3; a++) {
+        const double c = compc(coef[p], a);
+        nodes(stencil(t.x1[p], a, hs), out[a], [&](size_t id, double w, const double*) { out[a].d[id] += w * c; });
+      }
+  }
+  // applyCollider (fluid-simulator.cc:76-111): walls u = 0, inside the collider u -= u n_a.
+  void collider(RArr G[3]) const {
+    for (int a = 0; a < 3; a++) {
+      RArr& g = G[a];
+      for (int i = 0; i < g.nx; i++)
+        for (int j = 0; j < g.ny; j++)
+          for (int k = 0; k < g.nz; k++) {
+            const int ix[3] = {i, j, k}, dims[3] = {g.nx, g.ny, g.nz};
+            double& gb = g(i, j, k);
+            if (ix[a] == 0 || ix[a] == dims[a] - 1) {
+              gb = 0.0;
+              continue;
+            }
+            V3 p{coord1(i, face_off(a, 0), s.h[0]), coord1(j, face_off(a, 1), s.h[1]),
+                 coord1(k, face_off(a, 2), s.h[2])};
+            if (s.collider.eval(p) < 0.0) gb = gb - gb * compc(gnormalize(s.collider.grad(p)), a);
+          }
+    }
+  }
+  // Face a between cells lo and hi (lower and upper along a), of face grid index (i, j, k).
+  static void face_cells(int a, int i, int j, int k, int lo[3], int hi[3]) {
+    lo[0] = hi[0] = i;
+    lo[1] = hi[1] = j;
+    lo[2] = hi[2] = k;
+    lo[a] -= 1;
+  }
+  // FvmSolver3D::project (project-solver.cc:348-421) reversed. Returns pbar (only active cells
+  // carry a pressure) and zeroes the adjoint of the wall faces the projection sets to 0.
+  void project(const AdjRecord& t, RArr gb[3], RArr& pbar, double dt) const {
+    const double hh = s.h[0];
+    pbar = RArr(s.nx, s.ny, s.nz);
+    for (int a = 0; a < 3; a++)
+      for (int i = 0; i < gb[a].nx; i++)
+        for (int j = 0; j < gb[a].ny; j++)
+          for (int k = 0; k < gb[a].nz; k++) {
+            const int ix[3] = {i, j, k}, dims[3] = {gb[a].nx, gb[a].ny, gb[a].nz};
+            if (t.weights[a](i, j, k) <= 0.0) continue;
+            double& g = gb[a](i, j, k);
+            if (ix[a] == 0 || ix[a] == dims[a] - 1) {
+              g = 0.0;
+              continue;
+            }
+            int lo[3], hi[3];
+            face_cells(a, i, j, k, lo, hi);
+            const double sl = t.fsdf(lo[0], lo[1], lo[2]), sh = t.fsdf(hi[0], hi[1], hi[2]);
+            double* pl = &pbar(lo[0], lo[1], lo[2]);
+            double* ph = &pbar(hi[0], hi[1], hi[2]);
+            if (sl >= 0.0 && sh >= 0.0) continue;
+            if (sl < 0.0 && sh < 0.0) {  // u -= (p_hi - p_lo) dt/h
+              *ph -= g * dt / hh;
+              *pl += g * dt / hh;
+            } else if (sl < 0.0) {  // u += p_lo dt/h/theta
+              *pl += g * dt / hh / std::max(sl / (sl - sh), 0.01);
+            } else {  // u -= p_hi dt/h/(1 - theta)
+              *ph -= g * dt / hh / (1.0 - std::min(sl / (sl - sh), 0.99));
+            }
+          }
+    for (size_t e = 0; e < pbar.size(); e++)
+      if (!t.active.d[e]) pbar.d[e] = 0.0;
+  }
+  // lambda = A^-1 pbar with the substep's own system and the same MIC-PCG (A is symmetric),
+  // relative residual 1e-10. The simulation's system arrays are restored afterwards.
+  void solve(const AdjRecord& t, const RArr& pbar, RArr& lam) {
+    const fs_config cfg0 = s.cfg;
+    RArr ad0 = s.Adiag, rhs0 = s.rhs, pg0 = s.pg, an0[6];
+    for (int q = 0; q < 6; q++) an0[q] = s.An[q];
+    BArr act0 = s.active;
+    const int it0 = s.last_iters, st0 = s.last_status;
+    const double res0 = s.last_residual;
+    s.Adiag = t.adiag;
+    for (int q = 0; q < 6; q++) s.An[q] = t.an[q];
+    s.active = t.active;
+    s.rhs = pbar;
+    s.cfg.tol_mode = FS_TOL_REL_L2;
+    s.cfg.cg_tolerance = 1e-10;
+    s.cfg.cg_max_iterations = 4 * std::max(cfg0.cg_max_iterations, 100);
+    s.solvePressure();
+    lam = s.pg;
+    s.cfg = cfg0;
+    s.Adiag = ad0;
+    for (int q = 0; q < 6; q++) s.An[q] = an0[q];
+    s.active = act0;
+    s.rhs = rhs0;
+    s.pg = pg0;
+    s.last_iters = it0;
+    s.last_status = st0;
+    s.last_residual = res0;
+  }
+  // theta = a / (a - b): d theta / da = -b / (a-b)^2, d theta / db = a / (a-b)^2.
+  static void dtheta(double a, double b, double& da, double& db) {
+    const double den = (a - b) * (a - b);
+    da = -b / den;
+    db = a / den;
+  }
+  // The fluid SDF enters through the ghost-fluid theta: Adiag of buildSystem
+  // (project-solver.cc:143-223), where dL/dAdiag(c) = -lambda(c) p(c), and the one-sided faces of
+  // project (:348-421), weighted by the face adjoint gb (of the projected velocities). A clamped
+  // theta carries no derivative. Scatters into phibar (adjoint of the smoothed SDF).
+  void theta(const AdjRecord& t, const RArr& lam, const RArr gb[3], double dt, RArr& phibar) const {
+    const double f = dt / s.h[0];
+    phibar = RArr(s.nx, s.ny, s.nz);
+    const RArr& fs = t.fsdf;
+    const int cd[3] = {s.nx, s.ny, s.nz};
+    // (a) buildSystem: every active cell c and each air neighbour across a face of weight w
+    for (int i = 0; i < s.nx; i++)
+      for (int j = 0; j < s.ny; j++)
+        for (int k = 0; k < s.nz; k++) {
+          if (!t.active(i, j, k)) continue;
+          const double sd = fs(i, j, k), adc = -lam(i, j, k) * t.pg(i, j, k);
+          const int ix[3] = {i, j, k};
+          for (int a = 0; a < 3; a++) {
+            const int upper = a == 1 ? s.nx : cd[a];  // the "up" bound uses the width (Q8)
+            for (int side = 0; side < 2; side++) {
+              if (side == 0 ? ix[a] <= 0 : ix[a] >= upper - 1) continue;
+              int o[3] = {i, j, k};
+              o[a] += side == 0 ? -1 : 1;
+              const double sn = fs(o[0], o[1], o[2]);
+              if (!(sn > 0.0)) continue;
+              int fc[3] = {i, j, k};
+              fc[a] += side;
+              const double w = t.weights[a](fc[0], fc[1], fc[2]);
+              double dsn, dsd;
+              if (side == 0) {  // theta = min(sn/(sn - sd), 0.99); Adiag += w f / (1 - theta)
+                const double th = sn / (sn - sd);
+                if (!(th < 0.99)) continue;
+                dtheta(sn, sd, dsn, dsd);
+                const double dA = w * f / ((1.0 - th) * (1.0 - th));
+                phibar(i, j, k) += adc * dA * dsd;
+                phibar(o[0], o[1], o[2]) += adc * dA * dsn;
+              } else {  // theta = max(sd/(sd - sn), 0.01); Adiag += w f / theta
+                const double th = sd / (sd - sn);
+                if (!(th > 0.01)) continue;
+                dtheta(sd, sn, dsd, dsn);
+                const double dA = -w * f / (th * th);
+                phibar(i, j, k) += adc * dA * dsd;
+                phibar(o[0], o[1], o[2]) += adc * dA * dsn;
+              }
+            }
+          }
+        }
+    // (b) project: faces of positive weight, off the walls, with exactly one fluid side
+    for (int a = 0; a < 3; a++)
+      for (int i = 0; i < gb[a].nx; i++)
+        for (int j = 0; j < gb[a].ny; j++)
+          for (int k = 0; k < gb[a].nz; k++) {
+            const int ix[3] = {i, j, k}, dims[3] = {gb[a].nx, gb[a].ny, gb[a].nz};
+            if (ix[a] == 0 || ix[a] == dims[a] - 1 || !(t.weights[a](i, j, k) > 0.0)) continue;
+            int lo[3], hi[3];
+            face_cells(a, i, j, k, lo, hi);
+            const double sl = fs(lo[0], lo[1], lo[2]), sh = fs(hi[0], hi[1], hi[2]);
+            const double g = gb[a](i, j, k);
+            double dl, dh;
+            if (sl < 0.0 && sh >= 0.0) {  // u += p_lo f / theta, theta = max(sl/(sl - sh), 0.01)
+              const double th = sl / (sl - sh);
+              if (!(th > 0.01)) continue;
+              dtheta(sl, sh, dl, dh);
+              const double du = -t.pg(lo[0], lo[1], lo[2]) * f / (th * th);
+              phibar(lo[0], lo[1], lo[2]) += g * du * dl;
+              phibar(hi[0], hi[1], hi[2]) += g * du * dh;
+            } else if (sl >= 0.0 && sh < 0.0) {  // u -= p_hi f / (1 - theta), theta = min(., 0.99)
+              const double th = sl / (sl - sh);
+              if (!(th < 0.99)) continue;
+              dtheta(sl, sh, dl, dh);
+              const double du = -t.pg(hi[0], hi[1], hi[2]) * f / ((1.0 - th) * (1.0 - th));
+              phibar(lo[0], lo[1], lo[2]) += g * du * dl;
+              phibar(hi[0], hi[1], hi[2]) += g * du * dh;
+            }
+          }
+  }
+  // buildSystem's right-hand side (project-solver.cc:143-223): rhs(c) += w u on c's lower
+  // faces and -= w u on its upper ones (interior faces, active c). gb += w (lam_hi - lam_lo).
+  void rhs(const AdjRecord& t, const RArr& lam, RArr gb[3]) const {
+    for (int a = 0; a < 3; a++)
+      for (int i = 0; i < gb[a].nx; i++)
+        for (int j = 0; j < gb[a].ny; j++)
+          for (int k = 0; k < gb[a].nz; k++) {
+            const int ix[3] = {i, j, k}, dims[3] = {gb[a].nx, gb[a].ny, gb[a].nz};
+            if (ix[a] == 0 || ix[a] == dims[a] - 1) continue;
+            int lo[3], hi[3];
+            face_cells(a, i, j, k, lo, hi);
+            const double w = t.weights[a](i, j, k);
+            if (t.active(hi[0], hi[1], hi[2])) gb[a](i, j, k) += w * lam(hi[0], hi[1], hi[2]);
+            if (t.active(lo[0], lo[1], lo[2])) gb[a](i, j, k) -= w * lam(lo[0], lo[1], lo[2]);
+          }
+  }
+  // extrapolate (cpu/fluid-simulator.h:204-253): valid mask after one iteration.
+  static BArr grow(const BArr& V) {
+    BArr N = V;
+    for (int i = 0; i < V.nx; i++)
+      for (int j = 0; j < V.ny; j++)
+        for (int k = 0; k < V.nz; k++)
+          if (!V(i, j, k))
+            N(i, j, k) = (i > 0 && V(i - 1, j, k)) || (i < V.nx - 1 && V(i + 1, j, k)) ||
+                         (j > 0 && V(i, j - 1, k)) || (j < V.ny - 1 && V(i, j + 1, k)) ||
+                         (k > 0 && V(i, j, k - 1)) || (k < V.nz - 1 && V(i, j, k + 1));
+    return N;
+  }
+  static int valid_nbrs(const BArr& V, int i, int j, int k) {
+    return (i > 0 && V(i - 1, j, k)) + (i < V.nx - 1 && V(i + 1, j, k)) + (j > 0 && V(i, j - 1, k)) +
+           (j < V.ny - 1 && V(i, j + 1, k)) + (k > 0 && V(i, j, k - 1)) + (k < V.nz - 1 && V(i, j, k + 1));
+  }
+  // One iteration of extrapolate / extrapolateFluidSdf reversed (mask V, then the arrays swap):
+  // forward, an invalid cell c with valid neighbours takes their mean into the buffer, and the
+  // buffer keeps its old value elsewhere (valid cells; and cells without valid neighbours in the
+  // SDF version, which are set to 0 in the velocity version). From (A, B), the adjoints of the
+  // (array, buffer) pair after the swap, returns those before it in (A, B).
+  static void extrap_iter(const BArr& V, RArr& A, RArr& B, bool keep_far) {
+    RArr cur = B, buf(A.nx, A.ny, A.nz);
+    for (int i = 0; i < V.nx; i++)
+      for (int j = 0; j < V.ny; j++)
+        for (int k = 0; k < V.nz; k++) {
+          const double a = A(i, j, k);
+          if (V(i, j, k)) {
+            buf(i, j, k) = a;
+            continue;
+          }
+          const int cnt = valid_nbrs(V, i, j, k);
+          if (cnt == 0) {
+            if (keep_far) buf(i, j, k) = a;
+            continue;
+          }
+          const int nb[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+          for (auto& o : nb) {
+            const int x = i + o[0], y = j + o[1], z = k + o[2];
+            if (x < 0 || x >= V.nx || y < 0 || y >= V.ny || z < 0 || z >= V.nz || !V(x, y, z)) continue;
+            cur(x, y, z) += a / cnt;
+          }
+        }
+    A = cur;
+    B = buf;
+  }
+  // applyForce (fluid-simulator.cc:13-18) adds G_a dt to the valid faces of grid a off its walls;
+  // applyDirichletBoundary (:113-138) overwrites the walls. extrapolate reversed in between.
+  void velocity_chain(const AdjRecord& t, RArr gb[3], RArr gb0[3]) {
+    const int iters = s.cfg.vel_extrapolate_iters;
+    for (int a = 0; a < 3; a++) {
+      std::vector<BArr> masks{t.valid[a]};
+      for (int it = 0; it < iters; it++) masks.push_back(grow(masks.back()));
+      const BArr& Vf = masks[iters];
+      double sum = 0.0;
+      for (int i = 0; i < gb[a].nx; i++)
+        for (int j = 0; j < gb[a].ny; j++)
+          for (int k = 0; k < gb[a].nz; k++) {
+            const int ix[3] = {i, j, k}, dims[3] = {gb[a].nx, gb[a].ny, gb[a].nz};
+            if (Vf(i, j, k) && ix[a] > 0 && ix[a] < dims[a] - 1) sum += gb[a](i, j, k);
+          }
+      grav[a] += t.dt * sum;
+      RArr A = gb[a], B(A.nx, A.ny, A.nz);
+      for (int it = iters - 1; it >= 0; it--) extrap_iter(masks[it], A, B, false);
+      for (int i = 0; i < A.nx; i++)
+        for (int j = 0; j < A.ny; j++)
+          for (int k = 0; k < A.nz; k++) {
+            const int ix[3] = {i, j, k}, dims[3] = {A.nx, A.ny, A.nz};
+            if (ix[a] == 0 || ix[a] == dims[a] - 1) continue;
+            gb0[a](i, j, k) += A(i, j, k);
+          }
+    }
+  }
+  // FlipAdvectionSolver3D::solveP2G (advect-solver.cc:188-271) reversed: g0 = sum w v / W where
+  // W = sum w > 0. vb1_a += sum w gb0/W; xb += sum dw (v_a - g0) gb0/W.
+  void p2g(const AdjRecord& t, const RArr gb0[3], std::vector<V3>& xb, std::vector<V3>& vb1) const {
+    for (size_t p = 0; p < t.x1.size(); p++)
+      for (int a = 0; a < 3; a++) {
+        const double va = compc(t.v1[p], a);
+        double sv = 0.0, sx[3] = {0, 0, 0};
+        nodes(stencil(t.x1[p], a, s.h), t.g0[a], [&](size_t id, double w, const double dw[3]) {
+          if (!(t.w0[a].d[id] > 0.0)) return;
+          const double H = gb0[a].d[id] / t.w0[a].d[id];
+          sv += w * H;
+          for (int d = 0; d < 3; d++) sx[d] += dw[d] * H * (va - t.g0[a].d[id]);
+        });
+        comp(vb1[p], a) += sv;
+        for (int d = 0; d < 3; d++) comp(xb[p], d) += sx[d];
+      }
+  }
+  // The fluid-SDF chain reversed: reconstruction (sdf.h:210-225) -> extrapolateFluidSdf
+  // (cpu/fluid-simulator.h:123-166) -> smoothFluidSurface (fluid-simulator.cc:39-74), recomputed
+  // forward from the substep's advected particles and inherited buffer to get the masks and
+  // branches. phibar: adjoint of the smoothed SDF; psibar: in, the adjoint of the last smoothing
+  // input (the buffer the next substep inherits); out, that of the buffer this substep inherited.
+  void sdf_chain(const AdjRecord& t, const RArr& phibar, RArr& psibar, std::vector<V3>& xb) const {
+    const int ie = s.cfg.sdf_extrapolate_iters, is = s.cfg.sdf_smooth_iters;
+    const double radius = s.cfg.recon_radius_factor * s.h[0], r2 = 2 * radius;
+    const Sim::Bins bins = s.binParticles(t.x1);
+    RArr cur(s.nx, s.ny, s.nz), dmin(s.nx, s.ny, s.nz);
+    BArr V(s.nx, s.ny, s.nz);
+    cur.fill(1e9);
+    dmin.fill(HUGE_VAL);
+    for (int i = 0; i < s.nx; i++)
+      for (int j = 0; j < s.ny; j++)
+        for (int k = 0; k < s.nz; k++)
+          s.forNeighbours(bins, t.x1, i, j, k, r2, [&](int, V3 p, V3 q) {
+            const double dis = glen(q - p);
+            cur(i, j, k) = std::min(cur(i, j, k), dis - radius);
+            dmin(i, j, k) = std::min(dmin(i, j, k), dis);
+            V(i, j, k) = 1;
+          });
+    RArr buf = t.sdfbuf0;
+    std::vector<BArr> masks;
+    for (int it = 0; it < ie; it++) {
+      masks.push_back(V);
+      BArr Vn(s.nx, s.ny, s.nz);
+      for (int i = 0; i < s.nx; i++)
+        for (int j = 0; j < s.ny; j++)
+          for (int k = 0; k < s.nz; k++) {
+            if (V(i, j, k)) {
+              Vn(i, j, k) = 1;
+              continue;
+            }
+            double sum = 0.0;
+            int cnt = 0;
+            const int nb[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+            for (auto& o : nb) {
+              const int x = i + o[0], y = j + o[1], z = k + o[2];
+              if (x < 0 || x >= s.nx || y < 0 || y >= s.ny || z < 0 || z >= s.nz || !V(x, y, z)) continue;
+              sum += cur(x, y, z);
+              cnt++;
+            }
+            if (cnt > 0) {
+              buf(i, j, k) = sum / cnt;
+              Vn(i, j, k) = 1;
+            }
+          }
+      std::swap(cur, buf);
+      V = Vn;
+    }
+    std::vector<BArr> took_mean;  // smoothing: 1 where the cell took its neighbours' mean
+    for (int it = 0; it < is; it++) {
+      BArr F(s.nx, s.ny, s.nz);
+      for (int i = 0; i < s.nx; i++)
+        for (int j = 0; j < s.ny; j++)
+          for (int k = 0; k < s.nz; k++) {
+            double sum = 0.0;
+            int cnt = 0;
+            const int nb[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+            for (auto& o : nb) {
+              const int x = i + o[0], y = j + o[1], z = k + o[2];
+              if (x < 0 || x >= s.nx || y < 0 || y >= s.ny || z < 0 || z >= s.nz) continue;
+              sum += cur(x, y, z);
+              cnt++;
+            }
+            const double avg = sum / cnt;
+            F(i, j, k) = !(cur(i, j, k) < avg);
+            buf(i, j, k) = F(i, j, k) ? avg : cur(i, j, k);
+          }
+      took_mean.push_back(F);
+      std::swap(cur, buf);
+    }
+    if (cur.d != t.fsdf.d) mismatches++;  // the recomputation must give the recorded SDF
+    // smoothing reversed: out(c) = cur(c), or the mean of its in-grid neighbours
+    RArr ob = phibar;
+    for (int it = is - 1; it >= 0; it--) {
+      const BArr& F = took_mean[it];
+      RArr cb(s.nx, s.ny, s.nz);
+      for (int i = 0; i < s.nx; i++)
+        for (int j = 0; j < s.ny; j++)
+          for (int k = 0; k < s.nz; k++) {
+            const double o = ob(i, j, k);
+            if (!F(i, j, k)) {
+              cb(i, j, k) += o;
+              continue;
+            }
+            const int nb[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+            int cnt = 0;
+            for (auto& d : nb) {
+              const int x = i + d[0], y = j + d[1], z = k + d[2];
+              cnt += !(x < 0 || x >= s.nx || y < 0 || y >= s.ny || z < 0 || z >= s.nz);
+            }
+            for (auto& d : nb) {
+              const int x = i + d[0], y = j + d[1], z = k + d[2];
+              if (x < 0 || x >= s.nx || y < 0 || y >= s.ny || z < 0 || z >= s.nz) continue;
+              cb(x, y, z) += o / cnt;
+            }
+          }
+      if (it == is - 1)
+        for (size_t e = 0; e < cb.size(); e++) cb.d[e] += psibar.d[e];
+      ob = cb;
+    }
+    // extrapolation reversed; the first smoothing overwrites the whole buffer, so its adjoint
+    // starts at 0
+    RArr A = ob, B(s.nx, s.ny, s.nz);
+    for (int it = ie - 1; it >= 0; it--) extrap_iter(masks[it], A, B, true);
+    psibar = B;
+    // reconstruction reversed: a valid cell's value is |x_c - x_p| - r of its nearest kept
+    // particle(s); each of them receives abar (x_p - x_c) / |x_c - x_p|
+    for (int i = 0; i < s.nx; i++)
+      for (int j = 0; j < s.ny; j++)
+        for (int k = 0; k < s.nz; k++) {
+          const double ab = A(i, j, k);
+          if (ab == 0.0 || !(dmin(i, j, k) < HUGE_VAL)) continue;
+          s.forNeighbours(bins, t.x1, i, j, k, r2, [&](int idx, V3 p, V3 q) {
+            const double dis = glen(q - p);
+            if (dis != dmin(i, j, k) || !(dis > 0.0)) return;
+            const double c = ab / dis;
+            xb[idx].x += c * (q.x - p.x);
+            xb[idx].y += c * (q.y - p.y);
+            xb[idx].z += c * (q.z - p.z);
+          });
+        }
+  }
+  // advect on the cleared grid (advect-solver.cc:170-187: p = x + (2/9) dt v) and
+  // handleCollision (:100-137) reversed, the collider normal held fixed and d(sdf)/dp taken as
+  // SDF::grad; a clamped coordinate and a velocity component the wall zeroes pass nothing back.
+  void advect(const AdjRecord& t, std::vector<V3>& xb, std::vector<V3>& vb) const {
+    const double dt = t.dt;
+    for (size_t i = 0; i < t.x0.size(); i++) {
+      V3 v = t.v0[i];
+      V3 p = t.x0[i] + 2.0 / 9.0 * dt * v;
+      const double d = s.collider.eval(p);
+      const bool inside = d < 0.0;
+      V3 g{0, 0, 0}, n{0, 0, 0};
+      bool vproj = false;
+      if (inside) {
+        g = s.collider.grad(p);
+        n = gnormalize(g);
+        p = p - d * n;
+        const double vn = gdot(v, n);
+        vproj = vn < 0.0;
+        v = v - std::min(vn, 0.0) * n;
+      }
+      V3 pb = xb[i], vbb = vb[i];
+      const double ext[3] = {s.width, s.height, s.depth};
+      for (int c = 0; c < 3; c++) {
+        const double pc = compc(p, c), vc = compc(v, c);
+        const bool lo = pc < 0.0, hi = pc > ext[c];
+        if (lo || hi) comp(pb, c) = 0.0;
+        if ((lo && vc < 0.0) || (hi && vc > 0.0)) comp(vbb, c) = 0.0;
+      }
+      if (inside) {
+        if (vproj) vbb = vbb - gdot(n, vbb) * n;
+        pb = pb - gdot(n, pb) * g;
+      }
+      xb[i] = pb;
+      vb[i] = vbb + 2.0 / 9.0 * dt * pb;
+    }
+  }
+
+  // One substep reversed: (xb, vb) of its outputs -> of its inputs; psibar carried.
+  void step(const AdjRecord& t, std::vector<V3>& xb, std::vector<V3>& vb, RArr& psibar) {
+    const size_t np = t.x1.size();
+    std::vector<V3> vb1(np, V3{0, 0, 0}), coef(np, V3{0, 0, 0});
+    g2p(t, vb, xb, vb1, coef);
+    RArr gb[3], gb0[3];
+    transfer_t(t, coef, gb);
+    const bool flip = s.cfg.advection == FS_ADVECT_FLIP;
+    for (int a = 0; a < 3; a++) {
+      gb0[a] = gb[a];
+      for (double& x : gb0[a].d) x = flip ? -s.cfg.flip_alpha * x : 0.0;
+    }
+    collider(gb);
+    RArr pbar, lam, phibar;
+    project(t, gb, pbar, t.dt);
+    solve(t, pbar, lam);
+    theta(t, lam, gb, t.dt, phibar);
+    rhs(t, lam, gb);
+    velocity_chain(t, gb, gb0);
+    p2g(t, gb0, xb, vb1);
+    sdf_chain(t, phibar, psibar, xb);
+    advect(t, xb, vb1);
+    vb = vb1;
+  }
+};
 }  // namespace fso
 
 // ---------------------------------------------------------------------------------------------
@@ -1613,4 +2275,47 @@ void fso_nearest(void* sp, int axis, const double* p, int* out) {
   for (int c = 0; c < 3; c++) out[c] = fso::nearest1(p[c], fso::face_off(axis, c), s->h[c]);
 }
 int fso_last_iterations(void* s) { return static_cast<Sim*>(s)->last_iters; }
+
+// Reverse mode (same meaning as fs_rollout_* of the device library). The oracle keeps every
+// recorded substep in memory (no checkpoints), so it is meant for small grids.
+int fso_rollout_begin(void* sp) {
+  Sim* s = static_cast<Sim*>(sp);
+  if (!s->cfg.advect_on_cleared_grid || s->cfg.sdf_smooth_iters < 1) return FS_UNSUPPORTED;
+  s->tape.clear();
+  s->recording = true;
+  return FS_OK;
+}
+int fso_rollout_stop(void* sp) {
+  static_cast<Sim*>(sp)->recording = false;
+  return FS_OK;
+}
+int fso_rollout_steps(void* sp) { return (int)static_cast<Sim*>(sp)->tape.size(); }
+int fso_rollout_vjp(void* sp, const double* dpos, const double* dvel, double* dpos0, double* dvel0, double* dgrav) {
+  Sim* s = static_cast<Sim*>(sp);
+  if (s->tape.empty()) return FS_BAD_ARG;
+  const size_t np = s->tape.back().x1.size();
+  std::vector<fso::V3> xb(np, fso::V3{0, 0, 0}), vb(np, fso::V3{0, 0, 0});
+  for (size_t i = 0; i < np; i++) {
+    if (dpos) xb[i] = {dpos[3 * i], dpos[3 * i + 1], dpos[3 * i + 2]};
+    if (dvel) vb[i] = {dvel[3 * i], dvel[3 * i + 1], dvel[3 * i + 2]};
+  }
+  fso::Reverse rev(*s);
+  fso::RArr psibar(s->nx, s->ny, s->nz);
+  for (size_t j = s->tape.size(); j-- > 0;) rev.step(s->tape[j], xb, vb, psibar);
+  for (size_t i = 0; i < np; i++) {
+    if (dpos0) {
+      dpos0[3 * i] = xb[i].x;
+      dpos0[3 * i + 1] = xb[i].y;
+      dpos0[3 * i + 2] = xb[i].z;
+    }
+    if (dvel0) {
+      dvel0[3 * i] = vb[i].x;
+      dvel0[3 * i + 1] = vb[i].y;
+      dvel0[3 * i + 2] = vb[i].z;
+    }
+  }
+  if (dgrav)
+    for (int a = 0; a < 3; a++) dgrav[a] = rev.grav[a];
+  return rev.mismatches ? FS_DEVICE_ERROR : FS_OK;
+}
 }

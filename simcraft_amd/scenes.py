@@ -92,3 +92,47 @@ WORKLOADS = {
 
 def collider_for(kind: str, n: int) -> np.ndarray:
     return sphere_sdf(n) if kind == "sphere" else no_collider(n)
+
+
+def gaussian_potential_params(seed: int, count: int = 8, lo: float = 0.35, hi: float = 0.65,
+                              amplitude: float = 0.05):
+    """Centres (count, 3) uniform in [lo, hi]^3 and vector amplitudes (count, 3) uniform in
+    [-amplitude, amplitude], drawn with the splitmix64 generator from `seed`."""
+    u = uniform01(seed, np.arange(count * 6, dtype=np.uint64)).reshape(count, 6)
+    return lo + (hi - lo) * u[:, :3], amplitude * (2.0 * u[:, 3:] - 1.0)
+
+
+def curl_gaussian_velocity(n: int, seed: int = 4, sigma: float = 0.1, count: int = 8, lo: float = 0.35,
+                           hi: float = 0.65, amplitude: float = 0.05):
+    """Divergence-free face velocities (u, v, w) in the reference's face-grid layout: the discrete
+    curl of a vector potential psi(x) = sum_m a_m exp(-|x - c_m|^2 / (2 sigma^2)) sampled on the
+    cell edges (psi_x at (i+1/2, j, k) h, psi_y at (i, j+1/2, k) h, psi_z at (i, j, k+1/2) h). On
+    the MAC grid the discrete divergence of a discrete curl telescopes to zero, so every cell's
+    net face flux vanishes up to rounding.
+
+    Shapes: u (n+1, n, n), v (n, n+1, n), w (n, n, n+1)."""
+    h = 1.0 / n
+    centres, amps = gaussian_potential_params(seed, count, lo, hi, amplitude)
+    node = np.arange(n + 1) * h
+    mid = (np.arange(n) + 0.5) * h
+
+    def potential(xs, ys, zs, comp):
+        X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+        out = np.zeros(X.shape)
+        for c, a in zip(centres, amps):
+            r2 = (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2
+            out += a[comp] * np.exp(-r2 / (2.0 * sigma * sigma))
+        return out
+
+    px = potential(mid, node, node, 0)  # (n, n+1, n+1)
+    py = potential(node, mid, node, 1)  # (n+1, n, n+1)
+    pz = potential(node, node, mid, 2)  # (n+1, n+1, n)
+    u = (pz[:, 1:, :] - pz[:, :-1, :]) / h - (py[:, :, 1:] - py[:, :, :-1]) / h
+    v = (px[:, :, 1:] - px[:, :, :-1]) / h - (pz[1:, :, :] - pz[:-1, :, :]) / h
+    w = (py[1:, :, :] - py[:-1, :, :]) / h - (px[:, 1:, :] - px[:, :-1, :]) / h
+    return u, v, w
+
+
+def centred_block(n: int, seed: int = 2024, lo: float = 0.35, hi: float = 0.65) -> np.ndarray:
+    """Particles of the differentiable-rollout workload: 8 per cell of the centred block."""
+    return particles_in_cells(centred_block_mask(n, lo, hi), seed, 1.0 / n)

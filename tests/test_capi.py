@@ -90,3 +90,71 @@ def test_create_rejects_bad_config():
     st = lib.fs_create(C.byref(cfg), C.byref(ctx))
     assert st in (_capi.FS_UNSUPPORTED, _capi.FS_NO_DEVICE)
     assert not ctx.value
+
+
+CALLER = ROOT / "tests" / "capi_caller.c"
+
+
+def build_c_caller(out_dir: Path) -> Path:
+    """Compile tests/capi_caller.c with gcc against the header and link the library."""
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    exe = out_dir / "capi_caller"
+    lib_dir = _capi.LIB_PATH.parent
+    subprocess.run([gcc, "-std=c99", "-O2", "-Wall", "-Wextra", "-Werror", "-I", str(HEADER.parent), str(CALLER),
+                    "-L", str(lib_dir), "-lsimcraft_fluid", f"-Wl,-rpath,{lib_dir}", "-o", str(exe)], check=True)
+    return exe
+
+
+def test_c_caller_compiles_and_links(tmp_path):
+    exe = build_c_caller(tmp_path)
+    out = subprocess.run(["ldd", str(exe)], capture_output=True, text=True).stdout
+    assert "libsimcraft_fluid.so" in out and "not found" not in out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("advection", [1, 0], ids=["flip", "pic"])
+def test_c_caller_two_substeps_match_oracle(tmp_path, oracle, advection):
+    """The C program's two 16^3 substeps give the oracle's particles and pressure bit for bit."""
+    import numpy as np
+
+    from simcraft_amd import scenes
+    from tests.conftest import announce
+
+    n, dt, steps, alpha = 16, 0.004, 2, 0.95
+    announce(f"C caller n={n} advection={advection}")
+    pos, _ = scenes.dam_break(n)
+    vel = np.random.default_rng(12).uniform(-0.3, 0.3, pos.shape)
+    col = scenes.sphere_sdf(n)
+    src = tmp_path / "in.bin"
+    with open(src, "wb") as f:
+        np.array([n, advection, steps, 0], np.int32).tofile(f)
+        np.array([dt, alpha], np.float64).tofile(f)
+        np.array([len(pos)], np.int64).tofile(f)
+        for a in (col, pos, vel):
+            np.ascontiguousarray(a, np.float64).tofile(f)
+    exe = build_c_caller(tmp_path)
+    dst = tmp_path / "out.bin"
+    r = subprocess.run([str(exe), str(src), str(dst)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    raw = dst.read_bytes()
+    assert int(np.frombuffer(raw, np.int32, 1)[0]) == steps
+    rec = np.frombuffer(raw, np.dtype([("status", np.int32), ("iters", np.int32), ("res", np.float64)]), steps, 4)
+    off = 4 + rec.nbytes
+    body = np.frombuffer(raw, np.float64, offset=off)
+    npart = len(pos)
+    gp, gv = body[:3 * npart].reshape(-1, 3), body[3 * npart:6 * npart].reshape(-1, 3)
+    gpg = body[6 * npart:].reshape(n, n, n)
+
+    cfg = oracle.default_config(nx=n, ny=n, nz=n, flip_alpha=alpha, advection=advection)
+    o = oracle.OracleSim(cfg, dot_mode=oracle.DOT_TREE)
+    o.set_collider_sdf(col)
+    o.set_particles(pos, vel)
+    for s in range(steps):
+        st = o.substep(dt)
+        assert rec["iters"][s] == st["cg_iterations"] and rec["res"][s] == st["residual_inf"], (s, rec[s], st)
+    op, ov = o.particles()
+    assert np.array_equal(gp, op) and np.array_equal(gv, ov)
+    assert np.array_equal(gpg, o.get("pg"))
