@@ -136,6 +136,18 @@ int fs_device_count(void);
 /* setCollider(mesh) (cpu/fluid-simulator.h:62-71) takes a mesh and builds the SDF with
  * manifold2SDF; this entry takes the cell-centred collider SDF (nx*ny*nz doubles) directly. */
 int fs_set_collider_sdf(fs_ctx* ctx, const double* sdf);
+/* setCollider(mesh) itself (cpu/fluid-simulator.h:62-71): the collider SDF of a closed triangle
+ * mesh by manifold2SDF (FluidSim/src/cpu/util.cc:203-317), computed on the device. vertices: AoS
+ * xyz doubles (core::Mesh::vertices, Core/include/Core/mesh.h:9-14); indices: three 0-based vertex
+ * indices per triangle (core::Mesh::indices); exact_band: cells around each triangle's bounding
+ * box that get exact distances before the sweeps (setCollider passes 3). Load an OBJ file with
+ * the reference's core::myLoadObj (Core/src/obj-io.cc:13-64) and pass its arrays.
+ * flags: 0 = the reference's point-triangle distance, which measures |p + q - 2a| instead of
+ * |p - q| when the closest point q lies on edge AB, on edge AC or inside triangle abc
+ * (util.cc:101-125); FS_MESH_EXACT_DISTANCE = |p - q| in every case. */
+enum { FS_MESH_EXACT_DISTANCE = 1 };
+int fs_set_collider_mesh(fs_ctx* ctx, const double* vertices, int64_t nverts, const uint32_t* indices,
+                         int64_t ntris, int32_t exact_band, int32_t flags);
 /* setInitialFluid (cpu/fluid-simulator.h:73-78) draws unseeded random particles; this entry
  * takes explicit positions (and velocities; NULL = zero). Positions must lie in the domain. */
 int fs_set_particles(fs_ctx* ctx, const double* pos_xyz, const double* vel_xyz, int64_t n);
@@ -152,6 +164,17 @@ int fs_substep(fs_ctx* ctx, double dt, fs_stats* stats);
 int fs_step_frame(fs_ctx* ctx, double frame_dt, fs_stats* stats);
 /* CFL() (fluid-simulator.cc:184-203). */
 int fs_cfl(fs_ctx* ctx, double* dt_out);
+
+/* rebuildSurface (FluidSim/src/cpu/rebuild-surface.cc:732-741): a marching-cubes triangle mesh of
+ * the current fluid SDF, built on the device and kept there; returns its vertex and triangle
+ * counts. Vertices lie on the sign-changing edges between cell centres (samples outside the grid
+ * read 1000), with the reference's interpolated positions and normals; the triangles of a cube
+ * follow this library's case table (the reference's comes from an external table), vertices are
+ * numbered in the order of their edges and triangles in the order of their cubes. */
+int fs_rebuild_surface(fs_ctx* ctx, int64_t* nverts, int64_t* ntris);
+/* Copies the last rebuilt surface: vertices and normals AoS xyz (nverts*3 doubles each),
+ * indices three per triangle (ntris*3). Any output may be NULL. */
+int fs_get_surface(fs_ctx* ctx, double* vertices, double* normals, uint32_t* indices);
 
 /* ---- Operator level: the stages of substep, each on the context's device state ---- */
 int fs_clear(fs_ctx* ctx);                                   /* fluid-simulator.cc:20-37 */

@@ -63,6 +63,10 @@ _SIGS = {
     "fso_sample_velocity": (None, [_P, _D, _D]),
     "fso_nearest": (None, [_P, C.c_int, _D, C.POINTER(C.c_int)]),
     "fso_last_iterations": (C.c_int, [_P]),
+    "fso_set_collider_mesh": (C.c_int, [_P, _D, C.c_int64, C.POINTER(C.c_uint32), C.c_int64, C.c_int, C.c_int]),
+    "fso_rebuild_surface": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "fso_get_surface": (C.c_int, [_P, _D, _D, C.POINTER(C.c_uint32)]),
+    "fso_mc_table": (C.c_int, [C.POINTER(C.c_int8), C.POINTER(C.c_int8)]),
     "fso_rollout_begin": (C.c_int, [_P]),
     "fso_rollout_stop": (C.c_int, [_P]),
     "fso_rollout_steps": (C.c_int, [_P]),
@@ -115,6 +119,14 @@ def morton3d(x, y, z) -> int:
 
 def bspline(r) -> float:
     return load().fso_bspline(r)
+
+
+def mc_table():
+    """The marching-cubes case table: triangle counts (256,) and edge triples (256, 12, 3)."""
+    ntri = np.zeros(256, np.int8)
+    tri = np.zeros((256, 36), np.int8)
+    load().fso_mc_table(ntri.ctypes.data_as(C.POINTER(C.c_int8)), tri.ctypes.data_as(C.POINTER(C.c_int8)))
+    return ntri, tri.reshape(256, 12, 3)
 
 
 class OracleSim:
@@ -275,6 +287,24 @@ class OracleSim:
         out = np.empty(3)
         self._lib.fso_sample_velocity(self._ctx, _dp(pa), _dp(out))
         return out
+
+    def set_collider_mesh(self, vertices, indices, exact_band: int = 3, flags: int = 0):
+        """Collider SDF of a closed triangle mesh (manifold2SDF, exact band of 3 as setCollider uses;
+        flags 1 = true point-triangle distances instead of the reference's)."""
+        v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+        ix = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        if self._lib.fso_set_collider_mesh(self._ctx, _dp(v), len(v), ix.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           len(ix) // 3, exact_band, flags) != 0:
+            raise ValueError("bad mesh")
+
+    def rebuild_surface(self):
+        """Marching-cubes surface of the fluid SDF: (vertices (V,3), normals (V,3), triangles (T,3))."""
+        nv, nt = C.c_int64(), C.c_int64()
+        self._lib.fso_rebuild_surface(self._ctx, C.byref(nv), C.byref(nt))
+        v, n = np.zeros((nv.value, 3)), np.zeros((nv.value, 3))
+        ix = np.zeros((nt.value, 3), np.uint32)
+        self._lib.fso_get_surface(self._ctx, _dp(v), _dp(n), ix.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return v, n, ix
 
     def rollout_begin(self):
         """Record the following substeps for rollout_vjp (all of them are kept in memory)."""

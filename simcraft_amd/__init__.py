@@ -12,8 +12,8 @@ import warnings
 import numpy as np
 
 from . import _capi
-from ._capi import (ADVECT_FLIP, ADVECT_PIC, PRECISION_FP64, PRECISION_MIXED, PRECOND_MIC, PROJECT_FVM, RECON_NAIVE,
-                    SOLVER_CG, TOL_ABS_LINF, TOL_REL_L2, fs_config, fs_stats)
+from ._capi import (ADVECT_FLIP, ADVECT_PIC, MESH_EXACT_DISTANCE, PRECISION_FP64, PRECISION_MIXED, PRECOND_MIC,
+                    PROJECT_FVM, RECON_NAIVE, SOLVER_CG, TOL_ABS_LINF, TOL_REL_L2, fs_config, fs_stats)
 
 __all__ = ["FluidSim", "FluidSimError", "default_config", "device_count", "fs_config", "fs_stats"]
 
@@ -89,6 +89,29 @@ class FluidSim:
         if a.shape != self.shape:
             raise ValueError(f"collider SDF must have shape {self.shape}")
         self._check(self._lib.fs_set_collider_sdf(self._ctx, _dptr(a)))
+
+    def set_collider_mesh(self, vertices: np.ndarray, indices: np.ndarray, exact_band: int = 3,
+                          flags: int = 0) -> None:
+        """setCollider(mesh): the collider SDF of a closed triangle mesh (manifold2SDF on the
+        device; exact band of 3 cells as the reference's setCollider uses). flags =
+        MESH_EXACT_DISTANCE replaces the reference's point-triangle distance by the true one."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        ix = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+        if len(ix) % 3:
+            raise ValueError("indices must hold three vertex indices per triangle")
+        self._check(self._lib.fs_set_collider_mesh(self._ctx, _dptr(v), len(v),
+                                                   ix.ctypes.data_as(C.POINTER(C.c_uint32)), len(ix) // 3,
+                                                   exact_band, flags))
+
+    def rebuild_surface(self):
+        """rebuildSurface: marching-cubes mesh of the fluid SDF on the device; returns
+        (vertices (V, 3), normals (V, 3), triangles (T, 3) uint32)."""
+        nv, nt = C.c_int64(), C.c_int64()
+        self._check(self._lib.fs_rebuild_surface(self._ctx, C.byref(nv), C.byref(nt)))
+        v, n = np.zeros((nv.value, 3)), np.zeros((nv.value, 3))
+        ix = np.zeros((nt.value, 3), dtype=np.uint32)
+        self._check(self._lib.fs_get_surface(self._ctx, _dptr(v), _dptr(n), ix.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return v, n, ix
 
     def set_particles(self, pos: np.ndarray, vel: np.ndarray | None = None) -> None:
         p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)

@@ -23,6 +23,7 @@ SOLVER_CG = 0
 RECON_NAIVE = 0
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 TOL_ABS_LINF, TOL_REL_L2 = 0, 1
+MESH_EXACT_DISTANCE = 1
 
 
 class fs_config(C.Structure):
@@ -77,6 +78,9 @@ SIGNATURES = {
     "fs_substep": (C.c_int, [_P, C.c_double, C.POINTER(fs_stats)]),
     "fs_step_frame": (C.c_int, [_P, C.c_double, C.POINTER(fs_stats)]),
     "fs_cfl": (C.c_int, [_P, _D]),
+    "fs_rebuild_surface": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "fs_get_surface": (C.c_int, [_P, _D, _D, C.POINTER(C.c_uint32)]),
+    "fs_set_collider_mesh": (C.c_int, [_P, _D, C.c_int64, C.POINTER(C.c_uint32), C.c_int64, C.c_int32, C.c_int32]),
     "fs_clear": (C.c_int, [_P]),
     "fs_advect": (C.c_int, [_P, C.c_double]),
     "fs_reconstruct": (C.c_int, [_P, C.c_double]),

@@ -82,6 +82,7 @@ void release_all(fs_ctx* c) {
   for (int a = 0; a < 3; a++)
     for (DBuf* b : {&c->g[a], &c->gbuf[a], &c->gvalid[a], &c->gvalidbuf[a], &c->gw[a], &c->glast[a], &c->gweights[a]})
       fs_free(*b);
+  for (DBuf* b : {&c->surf_v, &c->surf_n, &c->surf_i}) fs_free(*b);
   for (DBuf* b : {&c->fsdf, &c->fsdfbuf, &c->sdfvalid, &c->sdfvalidbuf, &c->collider, &c->pg, &c->adiag, &c->rhs,
                   &c->active, &c->keys, &c->slots, &c->perm, &c->counts, &c->offs, &c->scan_tmp, &c->red,
                   &c->recon_tmp, &c->sorted[0], &c->sorted[1], &c->sorted[2], &c->sorted[3], &c->p2g_w,

@@ -155,6 +155,9 @@ struct fs_ctx {
   size_t scan_tmp_bytes = 0;
   size_t bins_cap = 0;
   DBuf red;  // small reduction scratch
+  // the last surface built by fs_rebuild_surface (vertices and normals AoS xyz FP64, triangles)
+  DBuf surf_v, surf_n, surf_i;
+  int64_t surf_nv = 0, surf_nt = 0;
   PcgState pcg;
   hipEvent_t ev[12];
   bool ev_ok = false;

@@ -5,13 +5,13 @@
 // order, so the FP64 build is bit-identical to a correctly rounded CPU evaluation.
 // P2G is a deterministic gather over particles binned by their nearest node (counting sort,
 // particle index ascending inside a bin) instead of the reference's racy TBB scatter.
-#include <hipcub/hipcub.hpp>
 
 #include <cfloat>
 #include <cmath>
 #include <cstring>
 
 #include "fs_ctx.h"
+#include "fs_scan.h"
 
 using namespace fsd;
 
@@ -791,109 +791,134 @@ __global__ __launch_bounds__(TPB) void k_fill_bits(int n, typename Bits<T>::U* _
   if (i < n) d[i] = Bits<T>::inf;
 }
 
-// The per-cell minima are kept in RC_COPIES arrays of ncell entries: one per XCD (XCC_ID 0..7),
-// updated with workgroup-scope atomics, which the XCD's own L2 performs (all CUs of an XCD share
-// it), and one for any other XCC_ID, updated at agent scope; k_recon_final takes the minimum of
-// the copies. Agent-scope atomics on one array would all go out to memory.
-constexpr int RC_COPIES = 9;
-__device__ __forceinline__ int xcc_id() {
-  unsigned v;
-  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
-  return (int)v;
-}
+// A workgroup takes 256 consecutive particles. Particles are stored in a spatially coherent
+// order, so their candidate cells fill a small box: the workgroup lowers the minima of that box
+// in LDS (atomic minimum in LDS, no memory traffic), then lowers each per-cell minimum in memory
+// once per box cell that received a candidate (an agent-scope atomic that skips cells whose
+// minimum is already lower). When the box holds more cells than the LDS table, the workgroup
+// lowers the minima in memory directly.
+constexpr int RC_LDS_BYTES = 16384;
 
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_recon_splat(int n, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
-                              const T* __restrict__ pz, const int* __restrict__ keys, const int* __restrict__ binmax,
-                              T r2, T s_max, typename Bits<T>::U* __restrict__ dmin, int ncell) {
-  using UB = typename Bits<T>::U;
-  int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= n) return;
-  const int xc8 = xcc_id();
-  const bool local = xc8 >= 0 && xc8 < 8;  // uniform over the workgroup
-  UB* const dcopy = dmin + (size_t)(local ? xc8 : 8) * ncell;
-  if (binmax[keys[id]] == id) return;  // forNeighbours stops one short of the inclusive bin end
-  const T p[3] = {px[id], py[id], pz[id]};
+struct ReconCand {  // one particle's candidate cells (empty box when it is skipped)
+  int lo[3], hi[3], bin[3];
+  T p[3];
+};
+
+template <typename T>
+__device__ inline ReconCand<T> recon_candidates(int id, int n, const GridArgs<T>& ga, const T* px, const T* py,
+                                                const T* pz, const int* keys, const int* binmax, T r2) {
+  ReconCand<T> c;
+  const bool skip = id >= n || binmax[keys[id]] == id;  // forNeighbours stops one short of the inclusive bin end
   const int dims[3] = {ga.nx, ga.ny, ga.nz};
-  int bin[3], lo[3], hi[3];
+  c.p[0] = skip ? T(0) : px[id];
+  c.p[1] = skip ? T(0) : py[id];
+  c.p[2] = skip ? T(0) : pz[id];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    int bb = (int)floor(p[d] / ga.h[d]);  // ParticleNeighbourSearcher::update (ns-util.h:52-60)
+    int bb = (int)floor(c.p[d] / ga.h[d]);  // ParticleNeighbourSearcher::update (ns-util.h:52-60)
     bb = bb < 0 ? 0 : bb;
-    bin[d] = bb >= dims[d] ? dims[d] - 1 : bb;
-    lo[d] = max(0, (int)floor((p[d] - r2) / ga.h[d] - T(0.5)) - 1);
-    hi[d] = min(dims[d] - 1, (int)floor((p[d] + r2) / ga.h[d] - T(0.5)) + 1);
+    c.bin[d] = bb >= dims[d] ? dims[d] - 1 : bb;
+    c.lo[d] = max(0, (int)floor((c.p[d] - r2) / ga.h[d] - T(0.5)) - 1);
+    c.hi[d] = min(dims[d] - 1, (int)floor((c.p[d] + r2) / ga.h[d] - T(0.5)) + 1);
+    if (skip) {
+      c.lo[d] = INT_MAX / 4;
+      c.hi[d] = -1;
+    }
   }
-  // cell c keeps bin b on axis d when floor((x_c - 2r)/h) <= b <= ceil((x_c + 2r)/h) (clamped)
+  return c;
+}
+
+// Visits the candidate cells (i, j, k) of one particle with their squared distance:
+// cell c keeps bin b on axis d when floor((x_c - 2r)/h) <= b <= ceil((x_c + 2r)/h) (clamped),
+// and |x_c - x_p| <= 2r as d^2 <= s_max.
+template <typename T, class F>
+__device__ inline void recon_visit(const ReconCand<T>& c, const GridArgs<T>& ga, T r2, T s_max, F&& fn) {
+  const int dims[3] = {ga.nx, ga.ny, ga.nz};
   auto in_range = [&](int d, T xc) {
     int mn = max(0, (int)floor((xc - r2) / ga.h[d]));
     int mx = min(dims[d] - 1, (int)ceil((xc + r2) / ga.h[d]));
-    return bin[d] >= mn && bin[d] <= mx;
+    return c.bin[d] >= mn && c.bin[d] <= mx;
   };
-  for (int i = lo[0]; i <= hi[0]; i++) {
+  for (int i = c.lo[0]; i <= c.hi[0]; i++) {
     const T xc = coord1(i, T(0.5), ga.h[0]);
-    const T dx = xc - p[0];
+    const T dx = xc - c.p[0];
     const T dx2 = dx * dx;
     if (dx2 > s_max || !in_range(0, xc)) continue;
-    for (int j = lo[1]; j <= hi[1]; j++) {
+    for (int j = c.lo[1]; j <= c.hi[1]; j++) {
       const T yc = coord1(j, T(0.5), ga.h[1]);
-      const T dy = yc - p[1];
+      const T dy = yc - c.p[1];
       const T dxy = dx2 + dy * dy;
       if (dxy > s_max || !in_range(1, yc)) continue;
-      // the cells of the z-run in batches of 8: candidates first, then their current minima
-      // (independent loads, one round trip), then the lowering atomics (no return value)
-      UB* const row = dcopy + lin3(i, j, 0, ga.ny, ga.nz);
-      for (int k0 = lo[2]; k0 <= hi[2]; k0 += 8) {
-        UB bq[8], cq[8];
-        bool ok[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const int k = k0 + q;
-          ok[q] = false;
-          bq[q] = 0;
-          if (k > hi[2]) continue;
-          const T zc = coord1(k, T(0.5), ga.h[2]);
-          const T dz = zc - p[2];
-          const T d2 = dxy + dz * dz;
-          if (d2 > s_max || !in_range(2, zc)) continue;
-          ok[q] = true;
-          bq[q] = Bits<T>::of(d2);
-        }
-        if (local) {
-#pragma unroll
-          for (int q = 0; q < 8; q++)
-            cq[q] = ok[q] ? __hip_atomic_load(row + k0 + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0;
-#pragma unroll
-          for (int q = 0; q < 8; q++)
-            if (ok[q] && bq[q] < cq[q])
-              __hip_atomic_fetch_min(row + k0 + q, bq[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; q++)
-            cq[q] = ok[q] ? __hip_atomic_load(row + k0 + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-#pragma unroll
-          for (int q = 0; q < 8; q++)
-            if (ok[q] && bq[q] < cq[q])
-              __hip_atomic_fetch_min(row + k0 + q, bq[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+      for (int k = c.lo[2]; k <= c.hi[2]; k++) {
+        const T zc = coord1(k, T(0.5), ga.h[2]);
+        const T dz = zc - c.p[2];
+        const T d2 = dxy + dz * dz;
+        if (d2 > s_max || !in_range(2, zc)) continue;
+        fn(i, j, k, d2);
       }
     }
   }
 }
 
+__device__ inline void lower_global(unsigned int* a, unsigned int v) {
+  if (v < __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    __hip_atomic_fetch_min(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void lower_global(unsigned long long* a, unsigned long long v) {
+  if (v < __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    __hip_atomic_fetch_min(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_recon_splat(int n, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
+                              const T* __restrict__ pz, const int* __restrict__ keys, const int* __restrict__ binmax,
+                              T r2, T s_max, typename Bits<T>::U* __restrict__ dmin) {
+  using UB = typename Bits<T>::U;
+  constexpr int CAP = RC_LDS_BYTES / (int)sizeof(UB);
+  __shared__ UB table[CAP];
+  __shared__ int box[6];
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const ReconCand<T> c = recon_candidates(id, n, ga, px, py, pz, keys, binmax, r2);
+  if (threadIdx.x < 6) box[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : -1;
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (c.hi[d] >= 0) {
+      atomicMin(&box[d], c.lo[d]);
+      atomicMax(&box[3 + d], c.hi[d]);
+    }
+  }
+  __syncthreads();
+  const int L0 = box[0], L1 = box[1], L2 = box[2];
+  const int e0 = box[3] - L0 + 1, e1 = box[4] - L1 + 1, e2 = box[5] - L2 + 1;
+  if (e0 <= 0 || e1 <= 0 || e2 <= 0) return;  // uniform: no particle of the workgroup is kept
+  const long long vol = (long long)e0 * e1 * e2;
+  if (vol > CAP) {  // uniform: lower the minima in memory directly
+    recon_visit(c, ga, r2, s_max, [&](int i, int j, int k, T d2) { lower_global(dmin + lin3(i, j, k, ga.ny, ga.nz), Bits<T>::of(d2)); });
+    return;
+  }
+  for (int q = threadIdx.x; q < (int)vol; q += TPB) table[q] = Bits<T>::inf;
+  __syncthreads();
+  recon_visit(c, ga, r2, s_max, [&](int i, int j, int k, T d2) {
+    atomicMin(&table[((i - L0) * e1 + (j - L1)) * e2 + (k - L2)], Bits<T>::of(d2));
+  });
+  __syncthreads();
+  for (int q = threadIdx.x; q < (int)vol; q += TPB) {
+    const UB v = table[q];
+    if (v == Bits<T>::inf) continue;
+    const int k = L2 + q % e2, j = L1 + (q / e2) % e1, i = L0 + q / (e2 * e1);
+    lower_global(dmin + lin3(i, j, k, ga.ny, ga.nz), v);
+  }
+}
+
 // sdf.grid.fill(1e9); sdfValid.fill(false); then min(sdf, |d| - r) for the kept particles.
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_recon_final(int ncell, typename Bits<T>::U* __restrict__ dmin, T radius, T* __restrict__ sdf,
-                              uint8_t* __restrict__ valid) {
+__global__ __launch_bounds__(TPB) void k_recon_final(int ncell, const typename Bits<T>::U* __restrict__ dmin, T radius,
+                                                     T* __restrict__ sdf, uint8_t* __restrict__ valid) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
-  typename Bits<T>::U b = dmin[c];
-#pragma unroll
-  for (int q = 1; q < RC_COPIES; q++) {
-    const typename Bits<T>::U o = dmin[(size_t)q * ncell + c];
-    b = o < b ? o : b;
-  }
-  dmin[c] = b;  // the merged minimum, which the reverse pass reads
+  const typename Bits<T>::U b = dmin[c];
   T out = T(1e9);
   uint8_t v = 0;
   if (b != Bits<T>::inf) {
@@ -1287,10 +1312,9 @@ int ensure_bins(fs_ctx* c, size_t nbins) {
   FS_TRY(fs_alloc(c, c->counts, (nbins + 1) * sizeof(int)));
   FS_TRY(fs_alloc(c, c->offs, (nbins + 1) * sizeof(int)));
   FS_TRY(fs_alloc(c, c->offs2, (nbins + 1) * sizeof(int)));
-  size_t tmp = 0;
-  FS_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (int*)nullptr, (int*)nullptr, (int)(nbins + 1), c->stream));
-  FS_TRY(fs_alloc(c, c->scan_tmp, tmp + 256));
-  c->scan_tmp_bytes = tmp + 256;
+  const size_t tmp = (size_t)(fsscan::tiles_for((int64_t)nbins + 1) + 1) * sizeof(int);
+  FS_TRY(fs_alloc(c, c->scan_tmp, tmp));
+  c->scan_tmp_bytes = tmp;
   c->bins_cap = nbins;
   return FS_OK;
 }
@@ -1302,9 +1326,8 @@ int counting_sort(fs_ctx* c, int nbins, DBuf* offs_out = nullptr) {
   int n = (int)c->np;
   FS_HIP(c, hipMemsetAsync(c->counts.p, 0, (size_t)(nbins + 1) * sizeof(int), c->stream));
   if (n > 0) k_count<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->counts.as<int>(), c->slots.as<int>());
-  size_t tmp = c->scan_tmp_bytes;
-  FS_HIP(c, hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.p, tmp, c->counts.as<int>(), offs.as<int>(), nbins + 1,
-                                             c->stream));
+  FS_HIP(c, fsscan::exclusive_scan(c->stream, (int64_t)nbins + 1, c->counts.as<int>(), offs.as<int>(),
+                                   c->scan_tmp.as<int>()));
   if (n > 0) {
     k_scatter<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->slots.as<int>(), offs.as<int>(),
                                                        c->perm.as<int>());
@@ -1350,9 +1373,9 @@ int reconstruct_t(fs_ctx* c, double radius) {
   int nb = morton3D(c->nx - 1, c->ny - 1, c->nz - 1) + 1;
   // own keys and per-bin last particle (rkeys, rbinmax), not the P2G binning buffers
   if (!c->rbinmax.p) FS_TRY(fs_alloc(c, c->rbinmax, (size_t)nb * sizeof(int)));
-  if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)c->n_cell * RC_COPIES * sizeof(double)));
+  if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)c->n_cell * sizeof(double)));
   UB* dmin = c->recon_tmp.as<UB>();
-  k_fill_bits<T><<<fs_blocks(c->n_cell * RC_COPIES, TPB), TPB, 0, c->stream>>>(c->n_cell * RC_COPIES, dmin);
+  k_fill_bits<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->n_cell, dmin);
   T rad = (T)radius;
   T r2 = T(2) * rad;
   T smax;
@@ -1366,7 +1389,7 @@ int reconstruct_t(fs_ctx* c, double radius) {
     k_bin_last<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->rkeys.as<int>(), c->rbinmax.as<int>());
     k_recon_splat<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
                                                                c->pos[2].as<T>(), c->rkeys.as<int>(),
-                                                               c->rbinmax.as<int>(), r2, smax, dmin, c->n_cell);
+                                                               c->rbinmax.as<int>(), r2, smax, dmin);
   }
   k_recon_final<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->n_cell, dmin, rad, c->fsdf.as<T>(),
                                                                      c->sdfvalid.as<uint8_t>());
@@ -1730,11 +1753,11 @@ int adj_sdf_t(fs_ctx* c, AdjStep& t, DBuf& phibar, DBuf& psibar, DBuf* xbar) {
   GridArgs<T> ga = grid_args<T>(c);
   const int nb = morton3D(c->nx - 1, c->ny - 1, c->nz - 1) + 1;
   FS_TRY(ensure_bins(c, (size_t)nb));
-  if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)nc * RC_COPIES * sizeof(double)));
+  if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)nc * sizeof(double)));
   UB* dmin = c->recon_tmp.as<UB>();
   const T rad = (T)(c->cfg.recon_radius_factor * c->h[0]), r2 = T(2) * rad;
   const T smx = sizeof(T) == 8 ? (T)smax_for((double)r2) : (T)smax_for_f((float)r2);
-  k_fill_bits<T><<<fs_blocks(nc * RC_COPIES, TPB), TPB, 0, c->stream>>>(nc * RC_COPIES, dmin);
+  k_fill_bits<T><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(nc, dmin);
   if (n > 0) {
     FS_HIP(c, hipMemsetAsync(c->counts.p, 0xFF, (size_t)nb * sizeof(int), c->stream));
     k_keys_morton<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, t.x1[0].as<T>(), t.x1[1].as<T>(),
@@ -1742,7 +1765,7 @@ int adj_sdf_t(fs_ctx* c, AdjStep& t, DBuf& phibar, DBuf& psibar, DBuf* xbar) {
     k_bin_last<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->counts.as<int>());
     k_recon_splat<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, t.x1[0].as<T>(), t.x1[1].as<T>(),
                                                                t.x1[2].as<T>(), c->keys.as<int>(),
-                                                               c->counts.as<int>(), r2, smx, dmin, nc);
+                                                               c->counts.as<int>(), r2, smx, dmin);
   }
   k_recon_final<T><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(nc, dmin, rad, tp.sA.as<T>(), tp.sV[0].as<uint8_t>());
   FS_HIP(c, hipMemcpyAsync(tp.sB.p, t.sdfbuf0.p, cb, hipMemcpyDeviceToDevice, c->stream));

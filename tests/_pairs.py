@@ -1,6 +1,8 @@
 """Helpers that drive the device library and the oracle through identical calls."""
 from __future__ import annotations
 
+import hashlib
+
 import numpy as np
 
 from simcraft_amd import scenes
@@ -96,3 +98,37 @@ STAGES = [
     ("apply_collider", lambda s, dt, h: s.apply_collider()),
     ("g2p", lambda s, dt, h: s.g2p(dt)),
 ]
+
+
+STRUCTURE_ARRAYS = ["active", "sdf_valid", "u_valid", "v_valid", "w_valid"]
+
+
+def structure_hash(sim) -> str:
+    """Digest of the discrete structure of the last substep. The substep is smooth in its inputs
+    only while these stay the same; where one changes, the result jumps:
+      * active cells and the valid flags of the fluid SDF and the face grids (after extrapolation);
+      * the faces that P2G reached (weight sum > 0): a face a particle's stencil starts to reach
+        takes that particle's velocity instead of an extrapolated one;
+      * each particle's cell: the reconstruction skips the last particle of each cell's bin
+        (ns-util.h:103), so a particle that changes cell can change which one is skipped."""
+    h = hashlib.sha1()
+    for nm in STRUCTURE_ARRAYS:
+        h.update(np.ascontiguousarray(sim.get(nm)).tobytes())
+    for nm in ("uw", "vw", "ww"):
+        h.update(np.ascontiguousarray(sim.get(nm) > 0).tobytes())
+    pos, _ = sim.particles()
+    n = sim.shape[0]
+    h.update(np.ascontiguousarray(np.floor(pos * n).astype(np.int32)).tobytes())
+    return h.hexdigest()
+
+
+def central_difference(run, x0, eps_list):
+    """(f(x0 + e) - f(x0 - e)) / 2e for the first e of eps_list whose two runs pass through the
+    same discrete structure; run(x) returns (f(x), signature). Returns (estimate, e), or
+    (None, None) when every step size straddles a change of structure."""
+    for eps in eps_list:
+        fp, sp = run(x0 + eps)
+        fm, sm = run(x0 - eps)
+        if sp == sm:
+            return (fp - fm) / (2 * eps), eps
+    return None, None

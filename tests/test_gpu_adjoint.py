@@ -2,11 +2,14 @@
 CPU oracle's forward, checkpoint invariance and the torch.autograd binding."""
 from __future__ import annotations
 
+import hashlib
+
 import numpy as np
 import pytest
 
 from oracle import binding as B
 from simcraft_amd import scenes
+from tests._pairs import central_difference, structure_hash
 from tests.conftest import announce
 
 N = 16
@@ -157,3 +160,117 @@ def test_torch_autograd_binding(hip):
     np.testing.assert_allclose(grav.grad.numpy(), dg, rtol=1e-12, atol=1e-15)
     for g, ref in zip(grids, gu):
         np.testing.assert_allclose(g.grad.numpy(), ref, rtol=1e-12, atol=1e-15)
+
+
+def _paired_rollout(hip, cfg_kw, pos, vel, col, steps, dt, every, dpos_fn):
+    """Records the same rollout on the device (checkpoints every `every`) and in the oracle,
+    seeds both reverse passes with dpos_fn(pos_end, vel_end) and returns both results."""
+    cfg = hip.default_config(**cfg_kw)
+    sim = hip.FluidSim(cfg)
+    o = B.OracleSim(B.default_config(**cfg_kw), dot_mode=B.DOT_TREE)
+    for s in (sim, o):
+        s.set_collider_sdf(col)
+        s.set_particles(pos, vel)
+    sim.rollout_begin(every)
+    o.rollout_begin()
+    for _ in range(steps):
+        sim.substep(dt)
+        o.substep(dt)
+    sim.rollout_stop()
+    o.rollout_stop()
+    gp, gv = sim.particles()
+    op, ov = o.particles()
+    assert np.array_equal(gp, op) and np.array_equal(gv, ov)  # the forward is bit-exact
+    dp, dv = dpos_fn(gp, gv)
+    dev = sim.rollout_vjp(dp, dv)
+    cpu = o.rollout_vjp(dp, dv)
+    sim.close()
+    o.close()
+    return dev, cpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,advection", [("dam", 1), ("dam", 0), ("sphere", 1)],
+                         ids=["dam-flip", "dam-pic", "sphere-flip"])
+def test_device_adjoint_matches_cpu_adjoint(hip, kind, advection):
+    """The device's reverse pass against the oracle's FP64 CPU reverse pass on the same recorded
+    rollout (16^3, 3 substeps). The sphere scene exercises the collider's reverse, whose normal
+    is held fixed in both."""
+    announce(f"adjoint device vs cpu {kind} advection={advection}")
+    if kind == "sphere":
+        X, Y, Z = scenes.cell_centres(N)
+        pos = scenes.particles_in_cells((X > 0.45) & (X < 0.95) & (Y < 0.55) & (Z > 0.25) & (Z < 0.75), 3, 1.0 / N)
+        col = scenes.sphere_sdf(N)
+    else:
+        pos, _ = scenes.dam_break(N, 3)
+        col = scenes.no_collider(N)
+    vel = np.random.default_rng(3).uniform(-0.3, 0.3, pos.shape)
+    rng = np.random.default_rng(11)
+    cx, cv = rng.standard_normal(pos.shape), rng.standard_normal(pos.shape)
+    kw = dict(nx=N, ny=N, nz=N, flip_alpha=0.95, advection=advection)
+    dev, cpu = _paired_rollout(hip, kw, pos, vel, col, 3, DT, 2, lambda p, v: (cx, cv))
+    errs = [_rel(d, c) for d, c in zip(dev, cpu)]
+    print(f"[adjoint {kind}] device vs CPU rel L2: dpos0 {errs[0]:.2e} dvel0 {errs[1]:.2e} dG {errs[2]:.2e}")
+    assert max(errs) <= 1e-9, errs
+
+
+@pytest.mark.gpu
+def test_long_rollout_adjoint_32_against_cpu(hip):
+    """The long-rollout workload on its 32^3 downsample: dam break, 40 substeps of dt 0.001,
+    checkpoints every 8 on the device; loss = kinetic energy of the particles with x > 0.3 at the
+    end (the region x > 0.7 holds no particle after 40 substeps). dL/dG and dL/dv0 from the device
+    (FP64 and mixed) against the oracle's FP64 CPU adjoint, and dL/dg against central differences
+    of the forward with a tight pressure tolerance, taken with a step that leaves the discrete
+    structure of every substep unchanged (the reverse pass holds it fixed; across a change the
+    loss jumps and no difference quotient estimates the derivative)."""
+    n, steps, dt, x_min = 32, 40, 1e-3, 0.3
+    announce(f"long rollout adjoint n={n} steps={steps}")
+    pos, vel = scenes.dam_break(n)
+    col = scenes.no_collider(n)
+    kw = dict(nx=n, ny=n, nz=n, flip_alpha=0.95, tol_mode=1, cg_tolerance=1e-12, cg_max_iterations=1000)
+
+    def ke_seed(p, v):
+        return None, np.where((p[:, 0] > x_min)[:, None], v, 0.0)
+
+    dev, cpu = _paired_rollout(hip, kw, pos, vel, col, steps, dt, 8, ke_seed)
+    errs = [_rel(d, c) for d, c in zip(dev[1:], cpu[1:])]
+    print(f"[long rollout 32] device vs CPU rel L2: dvel0 {errs[0]:.2e} dG {errs[1]:.2e}; dG {cpu[2]}")
+    assert max(errs) <= 1e-8, errs
+
+    def loss(g):
+        s = hip.FluidSim(hip.default_config(**dict(kw, gravity=g)))
+        s.set_collider_sdf(col)
+        s.set_particles(pos, vel)
+        sig = []
+        for _ in range(steps):
+            s.substep(dt)
+            sig.append(structure_hash(s))
+        p, v = s.particles()
+        s.close()
+        m = p[:, 0] > x_min
+        sig.append(hashlib.sha1(m.tobytes()).hexdigest())
+        return 0.5 * float(np.sum(v[m] ** 2)), sig
+
+    fd, eps = central_difference(loss, 9.8, (1e-5, 3e-6, 1e-6, 3e-7, 1e-7, 3e-8))
+    print(f"[long rollout 32] dL/dg: FD {fd} (step {eps}), CPU adjoint {-cpu[2][1]:.8e}")
+    assert fd is not None, "every step size changed the discrete structure"
+    assert -cpu[2][1] == pytest.approx(fd, rel=1e-6)
+    # the mixed-precision device adjoint of the mixed forward, against the FP64 CPU adjoint
+    sim = hip.FluidSim(hip.default_config(**dict(kw, precision=hip.PRECISION_MIXED, tol_mode=0, cg_tolerance=1e-6,
+                                                cg_max_iterations=300)))
+    sim.set_collider_sdf(col)
+    sim.set_particles(pos, vel)
+    sim.rollout_begin(8)
+    for _ in range(steps):
+        sim.substep(dt)
+    sim.rollout_stop()
+    _, dv = ke_seed(*sim.particles())
+    mixed = sim.rollout_vjp(None, dv)
+    sim.close()
+    merr = [_rel(d, c) for d, c in zip(mixed[1:], cpu[1:])]
+    print(f"[long rollout 32] mixed device vs FP64 CPU rel L2: dvel0 {merr[0]:.2e} dG {merr[1]:.2e}")
+    assert max(merr) <= 1e-2, merr

@@ -18,11 +18,17 @@ def _run(sim, pos, vel, col, dt, steps):
     return stats
 
 
+# Bounds about ten times the errors measured on an MI355X: pressure rel. L2, particle velocity rel.
+# L2, particle position max (32^3: 2.4e-7, 7.1e-7, 2.4e-7; 48^3: 1.2e-7, 4.3e-6, 1.1e-7;
+# 128^3: 8.7e-7, 3.6e-5, 1.2e-7).
+BOUNDS = {32: (3e-6, 1e-5, 3e-6), 48: (2e-6, 5e-5, 2e-6), 128: (1e-5, 4e-4, 2e-6)}
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,steps,collider", [(32, 10, "none"), (48, 4, "sphere")])
+@pytest.mark.parametrize("n,steps,collider", [(32, 10, "none"), (48, 4, "sphere"), (128, 3, "none")])
 def test_mixed_against_fp64_oracle(hip, n, steps, collider):
     announce(f"mixed n={n} steps={steps} collider={collider}")
-    dt = 0.004 if n == 32 else 0.002
+    dt = {32: 0.004, 48: 0.002, 128: 0.001}[n]
     pos, vel = scenes.dam_break(n)
     col = scenes.collider_for(collider, n)
     cfg = hip.default_config(nx=n, ny=n, nz=n, flip_alpha=0.95, precision=hip.PRECISION_MIXED)
@@ -44,4 +50,5 @@ def test_mixed_against_fp64_oracle(hip, n, steps, collider):
     xerr = np.abs(p32 - p64).max()
     print(f"[mixed n={n}] pressure rel L2 {perr:.2e}, velocity rel L2 {verr:.2e}, position max {xerr:.2e}; "
           f"iterations mixed {[s['cg_iterations'] for s in st]} fp64 {[s['cg_iterations'] for s in ost]}")
-    assert perr < 1e-3 and verr < 1e-3 and xerr < 1e-4
+    pb, vb, xb = BOUNDS[n]
+    assert perr < pb and verr < vb and xerr < xb, (perr, verr, xerr)

@@ -122,3 +122,25 @@ def test_iteration_cap_and_empty(hip, oracle):
     st = e.substep(0.004)
     assert st["cg_iterations"] == 0 and st["status"] == 0
     assert not e.get("active").any()
+
+
+def test_cfl_and_frame_driver_bitexact(hip, oracle):
+    """CFL() = 5 max(1e-3, min h/|u|) (fluid-simulator.cc:184-203) and the frame loop of
+    step(Frame&) (:205-220): the same substep count, dt sequence and final state as the oracle."""
+    n = 32
+    announce(f"cfl / step_frame n={n}")
+    g, o = make_pair(hip, oracle, n, "overlap", flip_alpha=0.95)
+    assert g.cfl() == o.cfl() == 5.0 * (1.0 / n) / 1e-6  # particles carry velocity; grids are still zero
+    for s in (g, o):
+        pos, _ = s.particles()
+        s.set_particles(pos, np.random.default_rng(8).uniform(-3.0, 3.0, pos.shape))
+        s.substep(0.004)
+    cg, co = g.cfl(), o.cfl()
+    assert cg == co, (cg, co)
+    umax = max(np.abs(o.get(nm)).max() for nm in ("u", "v", "w"))
+    assert cg == 5.0 * max(1e-3, min((1.0 / n) / 1e-6, (1.0 / n) / umax))
+    frame = 2.5 * cg  # three substeps: two of the CFL step (if it holds) and the remainder
+    sg, so = g.step_frame(frame), o.step_frame(frame)
+    assert sg["substeps"] == so["substeps"] >= 2, (sg, so)
+    assert_same(g, o, ["pg", "u", "v", "w", "fluid_sdf", "active"], "after step_frame")
+    assert g.cfl() == o.cfl()
