@@ -262,6 +262,19 @@ int fs_sample_grid(fs_ctx* ctx, const double* pos_xyz, const double* u, const do
 int fs_sample_grid_vjp(fs_ctx* ctx, const double* pos_xyz, const double* dvel_xyz, double* du, double* dv,
                        double* dw);
 
+/* ---- Allocation-free entry (no counterpart in the reference) ----
+ * For callers that cannot allocate device memory: code under HIP stream/graph capture and
+ * serving loops that forbid hipMalloc. CFL() (fluid-simulator.cc:184-203, as fs_cfl) of face
+ * grids the caller holds in device memory (u, v, w: dense reference layout, FP64, for an
+ * nx*ny*nz cell grid of spacing h); the time step cfl_factor * max(1e-3, min(h/1e-6, h/max|u|))
+ * is written to the device address dt_out on `stream` (a hipStream_t; NULL = the null stream),
+ * asynchronously. Nothing is allocated: the reduction's partial maxima live in a fixed 8 KiB
+ * device array of the library (one per GPU, held until the process ends), which every call
+ * rewrites before reading. Calls that overlap on one GPU race on that array. Each face grid
+ * must have fewer than 2^31 faces (FS_BAD_ARG otherwise, before anything is launched). */
+int fs_cfl_device_noalloc(const double* u, const double* v, const double* w, int32_t nx, int32_t ny, int32_t nz,
+                          double h, double cfl_factor, double* dt_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -112,6 +112,8 @@ SIGNATURES = {
     "fs_rollout_vjp": (C.c_int, [_P, _D, _D, _D, _D, _D]),
     "fs_sample_grid": (C.c_int, [_P, _D, _D, _D, _D, _D]),
     "fs_sample_grid_vjp": (C.c_int, [_P, _D, _D, _D, _D, _D]),
+    "fs_cfl_device_noalloc": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, _P,
+                                        _P]),
 }
 
 KERNEL_NAMES = ["spmv", "mic_fwd", "mic_bwd", "pcg_update", "mic_factor", "p2g", "g2p", "advect", "reconstruct"]
