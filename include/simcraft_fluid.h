@@ -39,7 +39,7 @@ typedef enum fs_status {
   FS_NONFINITE = 3,     /* NaN/Inf residual */
   FS_BAD_ARG = 4,
   FS_DEVICE_ERROR = 5,  /* HIP error, or a device-side wait that exceeded its bound */
-  FS_UNSUPPORTED = 6,   /* e.g. APIC, Multigrid: declared but unimplemented in the reference */
+  FS_UNSUPPORTED = 6,   /* e.g. Multigrid: declared but unimplemented in the reference */
   FS_NO_DEVICE = 7
 } fs_status;
 
@@ -153,7 +153,17 @@ int fs_set_collider_mesh(fs_ctx* ctx, const double* vertices, int64_t nverts, co
 int fs_set_particles(fs_ctx* ctx, const double* pos_xyz, const double* vel_xyz, int64_t n);
 int fs_get_particles(fs_ctx* ctx, double* pos_xyz, double* vel_xyz);
 int64_t fs_num_particles(const fs_ctx* ctx);
-int fs_set_advector(fs_ctx* ctx, int32_t method);                       /* cpu/fluid-simulator.h:80-98 */
+/* cpu/fluid-simulator.h:80-98. FS_ADVECT_APIC, which the reference declares without an
+ * implementation (setAdvector reaches ERROR at :96-97), is affine particle-in-cell: each particle
+ * carries per velocity component a the row c_a; P2G transfers v_a + c_a.(x_n - x_p) to every
+ * node of its stencil, G2P sets v_a to the sampled velocity and c_a = D^-1 b (b = sum w g (x_n -
+ * x_p), D = sum w (x_n - x_p)(x_n - x_p)^T, over the same nodes and weights). Not differentiated
+ * by the reverse mode. */
+int fs_set_advector(fs_ctx* ctx, int32_t method);
+/* The APIC rows of every particle: 9 doubles per particle, [particle][component u, v, w][x, y, z];
+ * zero after fs_set_particles. */
+int fs_get_particle_affine(fs_ctx* ctx, double* c9);
+int fs_set_particle_affine(fs_ctx* ctx, const double* c9);
 int fs_set_projector(fs_ctx* ctx, int32_t method);                      /* cpu/fluid-simulator.h:100-107 */
 int fs_set_compressed_solver(fs_ctx* ctx, int32_t solver, int32_t precond); /* :109-113 */
 int fs_set_reconstructor(fs_ctx* ctx, int32_t method);                  /* :168-179 */

@@ -67,6 +67,8 @@ _SIGS = {
     "fso_rebuild_surface": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "fso_get_surface": (C.c_int, [_P, _D, _D, C.POINTER(C.c_uint32)]),
     "fso_mc_table": (C.c_int, [C.POINTER(C.c_int8), C.POINTER(C.c_int8)]),
+    "fso_get_apic": (C.c_int, [_P, _D]),
+    "fso_set_apic": (C.c_int, [_P, _D]),
     "fso_rollout_begin": (C.c_int, [_P]),
     "fso_rollout_stop": (C.c_int, [_P]),
     "fso_rollout_steps": (C.c_int, [_P]),
@@ -305,6 +307,16 @@ class OracleSim:
         ix = np.zeros((nt.value, 3), np.uint32)
         self._lib.fso_get_surface(self._ctx, _dp(v), _dp(n), ix.ctypes.data_as(C.POINTER(C.c_uint32)))
         return v, n, ix
+
+    def apic_matrices(self):
+        """APIC affine rows (N, 3, 3): [particle, component u/v/w, x/y/z]."""
+        c = np.zeros((self.num_particles(), 3, 3))
+        self._lib.fso_get_apic(self._ctx, _dp(c))
+        return c
+
+    def set_apic_matrices(self, c):
+        a = np.ascontiguousarray(c, np.float64)
+        self._lib.fso_set_apic(self._ctx, _dp(a))
 
     def rollout_begin(self):
         """Record the following substeps for rollout_vjp (all of them are kept in memory)."""

@@ -373,6 +373,9 @@ struct Sim {
   PcgLayout layout;
 
   enum Dir { Left, Right, Up, Down, Front, Back };  // project-solver.cc:7-14
+  // APIC (advection == FS_ADVECT_APIC): per particle the affine velocity matrix, one row per
+  // velocity component (c[a] = the gradient of component a near the particle); zero when set
+  std::vector<std::array<V3, 3>> apicC;
   // reverse mode: substeps recorded since fso_rollout_begin
   bool recording = false;
   // the last surface built by fso_rebuild_surface (defined after the marching cubes)
@@ -693,6 +696,8 @@ struct Sim {
   // One face grid of FlipAdvectionSolver3D::solveP2G (advect-solver.cc:188-271).
   void p2g_grid(int axis, RArr& G, RArr& Wt) {
     int n = (int)pos.size();
+    const bool apic = cfg.advection == FS_ADVECT_APIC;
+    if (apic && apicC.size() != pos.size()) apicC.assign(pos.size(), {V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}});
     std::vector<std::array<int, 3>> nb(n);
     for (int i = 0; i < n; i++) {
       nb[i] = {nearest1(pos[i].x, face_off(axis, 0), h[0]), nearest1(pos[i].y, face_off(axis, 1), h[1]),
@@ -715,6 +720,7 @@ struct Sim {
         if (b0 + 2 < xlo || b0 - 1 >= xhi) continue;
         V3 p = pos[idx];
         double vc = axis == 0 ? vel[idx].x : (axis == 1 ? vel[idx].y : vel[idx].z);
+        const V3 ca = apic ? apicC[idx][axis] : V3{0, 0, 0};
         for (int i = -1; i <= 2; i++) {
           int x = b0 + i;
           if (x < xlo || x >= xhi) continue;
@@ -724,7 +730,9 @@ struct Sim {
               if (x >= 0 && x < G.nx && y >= 0 && y < G.ny && zz >= 0 && zz < G.nz) {
                 V3 pn{coord1(x, o0, h[0]), coord1(y, o1, h[1]), coord1(zz, o2, h[2])};
                 double w = kweight(h[0], h[1], h[2], p - pn);
-                G(x, y, zz) += vc * w;
+                // APIC: the particle's affine velocity at the node, v + c . (x_n - x_p)
+                const double val = apic ? vc + gdot(ca, pn - p) : vc;
+                G(x, y, zz) += val * w;
                 Wt(x, y, zz) += w;
               }
             }
@@ -1299,10 +1307,66 @@ struct Sim {
   }
 
   // FlipAdvectionSolver3D::solveG2P (advect-solver.cc:157-169), PicAdvector3D::solveG2P (:7-17)
+  // APIC G2P (Jiang et al. 2015, affine particle-in-cell) of one component a at p: the velocity
+  // is sampleVelocity's (util.cc:40-85: the same nodes, weights and order) and its affine row
+  // c = D^-1 b with b = sum w g (x_n - p), D = sum w (x_n - p)(x_n - p)^T over the same nodes.
+  void apicSample(V3 p, int a, const RArr& G, double& v, V3& c) const {
+    const double o[3] = {face_off(a, 0), face_off(a, 1), face_off(a, 2)};
+    int b[3] = {nearest1(p.x, o[0], h[0]), nearest1(p.y, o[1], h[1]), nearest1(p.z, o[2], h[2])};
+    const double hs = h[0];
+    double acc = 0.0, ws = 0.0;
+    V3 bb{0, 0, 0};
+    double D00 = 0, D01 = 0, D02 = 0, D11 = 0, D12 = 0, D22 = 0;
+    for (int i = -1; i <= 2; i++)
+      for (int j = -1; j <= 2; j++)
+        for (int k = -1; k <= 2; k++) {
+          int x = b[0] + i, y = b[1] + j, zz = b[2] + k;
+          if (!(x >= 0 && x < G.nx && y >= 0 && y < G.ny && zz >= 0 && zz < G.nz)) continue;
+          const double gn = G(x, y, zz);
+          const V3 pn{coord1(x, o[0], h[0]), coord1(y, o[1], h[1]), coord1(zz, o[2], h[2])};
+          const double w = kweight(hs, hs, hs, p - pn);
+          acc += gn * w;
+          ws += w;
+          const V3 d = pn - p;
+          const double gw = gn * w;
+          bb.x += gw * d.x;
+          bb.y += gw * d.y;
+          bb.z += gw * d.z;
+          const double wx = w * d.x, wy = w * d.y, wz = w * d.z;
+          D00 += wx * d.x;
+          D01 += wx * d.y;
+          D02 += wx * d.z;
+          D11 += wy * d.y;
+          D12 += wy * d.z;
+          D22 += wz * d.z;
+        }
+    v = ws > 0.0 ? acc / ws : 0.0;
+    c = {0, 0, 0};
+    if (!(ws > 0.0)) return;
+    const double i00 = D11 * D22 - D12 * D12, i01 = D02 * D12 - D01 * D22, i02 = D01 * D12 - D02 * D11;
+    const double i11 = D00 * D22 - D02 * D02, i12 = D01 * D02 - D00 * D12, i22 = D00 * D11 - D01 * D01;
+    const double det = D00 * i00 + D01 * i01 + D02 * i02;
+    if (det == 0.0) return;
+    c.x = (i00 * bb.x + i01 * bb.y + i02 * bb.z) / det;
+    c.y = (i01 * bb.x + i11 * bb.y + i12 * bb.z) / det;
+    c.z = (i02 * bb.x + i12 * bb.y + i22 * bb.z) / det;
+  }
+
   void g2p() {
     int n = (int)pos.size();
     double alpha = cfg.flip_alpha;
     bool flip = cfg.advection == FS_ADVECT_FLIP;
+    if (cfg.advection == FS_ADVECT_APIC) {
+      if (apicC.size() != pos.size()) apicC.assign(pos.size(), {V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}});
+      const RArr* G[3] = {ug.get(), vg.get(), wg.get()};
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < n; i++) {
+        double v[3];
+        for (int a = 0; a < 3; a++) apicSample(pos[i], a, *G[a], v[a], apicC[i][a]);
+        vel[i] = {v[0], v[1], v[2]};
+      }
+      return;
+    }
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++) {
       V3 pic = sampleVelocity(pos[i], *ug, *vg, *wg);
@@ -2487,6 +2551,7 @@ int fso_set_particles(void* sp, const double* p, const double* v, int64_t n) {
   Sim* s = static_cast<Sim*>(sp);
   s->pos.resize(n);
   s->vel.resize(n);
+  s->apicC.assign(n, {fso::V3{0, 0, 0}, fso::V3{0, 0, 0}, fso::V3{0, 0, 0}});
   for (int64_t i = 0; i < n; i++) {
     s->pos[i] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
     s->vel[i] = v ? fso::V3{v[3 * i], v[3 * i + 1], v[3 * i + 2]} : fso::V3{0, 0, 0};
@@ -2705,11 +2770,32 @@ int fso_mc_table(int8_t* ntri, int8_t* tri) {
   return FS_OK;
 }
 
+// APIC affine matrices, 9 doubles per particle: the rows for u, v and w (each x, y, z).
+int fso_get_apic(void* sp, double* c) {
+  Sim* s = static_cast<Sim*>(sp);
+  for (size_t i = 0; i < s->pos.size(); i++)
+    for (int a = 0; a < 3; a++) {
+      const fso::V3 r = i < s->apicC.size() ? s->apicC[i][a] : fso::V3{0, 0, 0};
+      c[9 * i + 3 * a] = r.x;
+      c[9 * i + 3 * a + 1] = r.y;
+      c[9 * i + 3 * a + 2] = r.z;
+    }
+  return FS_OK;
+}
+int fso_set_apic(void* sp, const double* c) {
+  Sim* s = static_cast<Sim*>(sp);
+  s->apicC.resize(s->pos.size());
+  for (size_t i = 0; i < s->pos.size(); i++)
+    for (int a = 0; a < 3; a++) s->apicC[i][a] = {c[9 * i + 3 * a], c[9 * i + 3 * a + 1], c[9 * i + 3 * a + 2]};
+  return FS_OK;
+}
+
 // Reverse mode (same meaning as fs_rollout_* of the device library). The oracle keeps every
 // recorded substep in memory (no checkpoints), so it is meant for small grids.
 int fso_rollout_begin(void* sp) {
   Sim* s = static_cast<Sim*>(sp);
-  if (!s->cfg.advect_on_cleared_grid || s->cfg.sdf_smooth_iters < 1) return FS_UNSUPPORTED;
+  if (!s->cfg.advect_on_cleared_grid || s->cfg.sdf_smooth_iters < 1 || s->cfg.advection == FS_ADVECT_APIC)
+    return FS_UNSUPPORTED;
   s->tape.clear();
   s->recording = true;
   return FS_OK;

@@ -28,6 +28,8 @@ int fso_get_array(void* s, const char* name, void* out);
 int fso_rollout_begin(void* s);
 int fso_rollout_stop(void* s);
 int fso_rollout_vjp(void* s, const double* dpos, const double* dvel, double* dpos0, double* dvel0, double* dgrav);
+int fso_get_apic(void* s, double* c);
+int fso_set_apic(void* s, const double* c);
 }
 
 static int check(int st, const char* what) {
@@ -63,7 +65,8 @@ int main() {
                                      cx, cy - r, cz, cx, cy, cz + r, cx, cy, cz - r};
   const std::vector<uint32_t> tris = {0, 2, 4, 2, 1, 4, 1, 3, 4, 3, 0, 4, 2, 0, 5, 1, 2, 5, 3, 1, 5, 0, 3, 5};
   int bad = 0;
-  for (int advection : {(int)FS_ADVECT_FLIP, (int)FS_ADVECT_PIC}) {
+  for (int advection : {(int)FS_ADVECT_FLIP, (int)FS_ADVECT_PIC, (int)FS_ADVECT_APIC}) {
+    const bool apic = advection == FS_ADVECT_APIC;
     fs_config cfg;
     fso_config_default(&cfg);
     cfg.nx = cfg.ny = cfg.nz = n;
@@ -75,12 +78,22 @@ int main() {
     for (int flags : {0, (int)FS_MESH_EXACT_DISTANCE})
       bad |= check(fso_set_collider_mesh(s, verts.data(), 6, tris.data(), 8, 3, flags), "set_collider_mesh");
     bad |= check(fso_set_particles(s, pos.data(), vel.data(), np), "set_particles");
-    bad |= check(fso_rollout_begin(s), "rollout_begin");
+    std::vector<double> cm(9 * np);
+    for (size_t e = 0; e < cm.size(); e++) cm[e] = 0.5 * std::sin(0.37 * (double)e);
+    if (apic) {
+      bad |= check(fso_set_apic(s, cm.data()), "set_apic");
+      if (fso_rollout_begin(s) != FS_UNSUPPORTED) bad = 1;  // APIC has no reverse pass
+    } else {
+      bad |= check(fso_rollout_begin(s), "rollout_begin");
+    }
     fs_stats st;
     for (int step = 0; step < 3; step++) bad |= check(fso_substep(s, 0.004, &st), "substep");
     fso_rollout_stop(s);
     std::vector<double> dp(3 * np, 1.0), dv(3 * np, -0.5), dp0(3 * np), dv0(3 * np), dg(3);
-    bad |= check(fso_rollout_vjp(s, dp.data(), dv.data(), dp0.data(), dv0.data(), dg.data()), "rollout_vjp");
+    if (apic)
+      bad |= check(fso_get_apic(s, cm.data()), "get_apic");
+    else
+      bad |= check(fso_rollout_vjp(s, dp.data(), dv.data(), dp0.data(), dv0.data(), dg.data()), "rollout_vjp");
     double dt = 0;
     bad |= check(fso_cfl(s, &dt), "cfl");
     bad |= check(fso_step_frame(s, 0.01, &st), "step_frame");

@@ -12,7 +12,7 @@ import warnings
 import numpy as np
 
 from . import _capi
-from ._capi import (ADVECT_FLIP, ADVECT_PIC, MESH_EXACT_DISTANCE, PRECISION_FP64, PRECISION_MIXED, PRECOND_MIC,
+from ._capi import (ADVECT_APIC, ADVECT_FLIP, ADVECT_PIC, MESH_EXACT_DISTANCE, PRECISION_FP64, PRECISION_MIXED, PRECOND_MIC,
                     PROJECT_FVM, RECON_NAIVE, SOLVER_CG, TOL_ABS_LINF, TOL_REL_L2, fs_config, fs_stats)
 
 __all__ = ["FluidSim", "FluidSimError", "default_config", "device_count", "fs_config", "fs_stats"]
@@ -102,6 +102,16 @@ class FluidSim:
         self._check(self._lib.fs_set_collider_mesh(self._ctx, _dptr(v), len(v),
                                                    ix.ctypes.data_as(C.POINTER(C.c_uint32)), len(ix) // 3,
                                                    exact_band, flags))
+
+    def particle_affine(self) -> np.ndarray:
+        """APIC rows (N, 3, 3): [particle, velocity component u/v/w, x/y/z]."""
+        c = np.zeros((self.num_particles(), 3, 3))
+        self._check(self._lib.fs_get_particle_affine(self._ctx, _dptr(c)))
+        return c
+
+    def set_particle_affine(self, c9: np.ndarray) -> None:
+        a = np.ascontiguousarray(c9, dtype=np.float64).reshape(self.num_particles(), 3, 3)
+        self._check(self._lib.fs_set_particle_affine(self._ctx, _dptr(a)))
 
     def rebuild_surface(self):
         """rebuildSurface: marching-cubes mesh of the fluid SDF on the device; returns

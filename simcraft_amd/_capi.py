@@ -78,6 +78,8 @@ SIGNATURES = {
     "fs_substep": (C.c_int, [_P, C.c_double, C.POINTER(fs_stats)]),
     "fs_step_frame": (C.c_int, [_P, C.c_double, C.POINTER(fs_stats)]),
     "fs_cfl": (C.c_int, [_P, _D]),
+    "fs_get_particle_affine": (C.c_int, [_P, _D]),
+    "fs_set_particle_affine": (C.c_int, [_P, _D]),
     "fs_rebuild_surface": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "fs_get_surface": (C.c_int, [_P, _D, _D, C.POINTER(C.c_uint32)]),
     "fs_set_collider_mesh": (C.c_int, [_P, _D, C.c_int64, C.POINTER(C.c_uint32), C.c_int64, C.c_int32, C.c_int32]),

@@ -766,6 +766,10 @@ int fs_rollout_begin(fs_ctx* c, int32_t checkpoint_every) {
     c->set_error("the reverse pass needs advection on the cleared grid (advect_on_cleared_grid = 1)");
     return FS_UNSUPPORTED;
   }
+  if (c->cfg.advection == FS_ADVECT_APIC) {
+    c->set_error("the reverse pass covers FLIP and PIC, not APIC");
+    return FS_UNSUPPORTED;
+  }
   fs_adj_release(c);
   c->tape.on = true;
   c->tape.recording = true;

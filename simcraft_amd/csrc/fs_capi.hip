@@ -36,6 +36,14 @@ int fs_zero(fs_ctx* c, DBuf& b) {
   return FS_OK;
 }
 
+int fs_ensure_apic(fs_ctx* c) {
+  const size_t bytes = (size_t)std::max<int64_t>(c->np, 1) * 9 * c->rsz;
+  if (c->apicc.p && c->apicc.bytes == bytes) return FS_OK;
+  fs_free(c->apicc);
+  FS_TRY(fs_alloc(c, c->apicc, bytes));
+  return fs_zero(c, c->apicc);
+}
+
 int fs_prof_begin(fs_ctx* c, int kind, double bytes) {
   if (c->cfg.profile_stages < 2 || !c->pev_ok || !c->ktime_on) return -1;
   if (c->pev_used + 2 > fs_ctx::NPROF_EV) {
@@ -82,7 +90,7 @@ void release_all(fs_ctx* c) {
   for (int a = 0; a < 3; a++)
     for (DBuf* b : {&c->g[a], &c->gbuf[a], &c->gvalid[a], &c->gvalidbuf[a], &c->gw[a], &c->glast[a], &c->gweights[a]})
       fs_free(*b);
-  for (DBuf* b : {&c->surf_v, &c->surf_n, &c->surf_i}) fs_free(*b);
+  for (DBuf* b : {&c->surf_v, &c->surf_n, &c->surf_i, &c->apicc}) fs_free(*b);
   for (DBuf* b : {&c->fsdf, &c->fsdfbuf, &c->sdfvalid, &c->sdfvalidbuf, &c->collider, &c->pg, &c->adiag, &c->rhs,
                   &c->active, &c->keys, &c->slots, &c->perm, &c->counts, &c->offs, &c->scan_tmp, &c->red,
                   &c->recon_tmp, &c->sorted[0], &c->sorted[1], &c->sorted[2], &c->sorted[3], &c->p2g_w,
@@ -154,6 +162,21 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 }  // namespace
+
+// APIC affine rows: device layout [3a + d][particle] at storage width <-> AoS doubles
+// [particle][a][d] across the ABI.
+template <typename T>
+__global__ void k_apic_to_aos(int64_t n, const T* __restrict__ cm, double* __restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int q = 0; q < 9; q++) out[9 * i + q] = (double)cm[q * n + i];
+}
+template <typename T>
+__global__ void k_apic_from_aos(int64_t n, const double* __restrict__ in, T* __restrict__ cm) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int q = 0; q < 9; q++) cm[q * n + i] = (T)in[9 * i + q];
+}
 
 extern "C" {
 
@@ -334,6 +357,7 @@ int fs_set_particles(fs_ctx* c, const double* pos, const double* vel, int64_t n)
     }
   FS_HIP(c, hipStreamSynchronize(c->stream));
   fs_adj_release(c);  // a recorded rollout refers to the previous particles
+  fs_free(c->apicc);  // APIC affine rows restart at zero
   if (n != c->np) {
     for (int d = 0; d < 3; d++) {
       fs_free(c->pos[d]);
@@ -372,14 +396,59 @@ int fs_get_particles(fs_ctx* c, double* pos, double* vel) {
 
 int64_t fs_num_particles(const fs_ctx* c) { return c ? c->np : -1; }
 
+int fs_get_particle_affine(fs_ctx* c, double* c9) {
+  FS_TRY(check_ctx(c));
+  if (!c9) return FS_BAD_ARG;
+  if (c->np == 0) return FS_OK;
+  FS_TRY(fs_ensure_apic(c));
+  DBuf tmp;
+  FS_TRY(fs_alloc(c, tmp, (size_t)c->np * 9 * sizeof(double)));
+  const int nb = (int)((c->np + 255) / 256);
+  if (c->mixed) k_apic_to_aos<float><<<nb, 256, 0, c->stream>>>(c->np, c->apicc.as<float>(), tmp.as<double>());
+  else k_apic_to_aos<double><<<nb, 256, 0, c->stream>>>(c->np, c->apicc.as<double>(), tmp.as<double>());
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c9, tmp.p, (size_t)c->np * 9 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  fs_free(tmp);
+  if (e != hipSuccess) {
+    c->set_error(hipGetErrorString(e));
+    return FS_DEVICE_ERROR;
+  }
+  return FS_OK;
+}
+
+int fs_set_particle_affine(fs_ctx* c, const double* c9) {
+  FS_TRY(check_ctx(c));
+  if (!c9) return FS_BAD_ARG;
+  if (c->np == 0) return FS_OK;
+  FS_TRY(fs_ensure_apic(c));
+  DBuf tmp;
+  FS_TRY(fs_alloc(c, tmp, (size_t)c->np * 9 * sizeof(double)));
+  hipError_t e = hipMemcpyAsync(tmp.p, c9, (size_t)c->np * 9 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  const int nb = (int)((c->np + 255) / 256);
+  if (e == hipSuccess) {
+    if (c->mixed) k_apic_from_aos<float><<<nb, 256, 0, c->stream>>>(c->np, tmp.as<double>(), c->apicc.as<float>());
+    else k_apic_from_aos<double><<<nb, 256, 0, c->stream>>>(c->np, tmp.as<double>(), c->apicc.as<double>());
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  fs_free(tmp);
+  if (e != hipSuccess) {
+    c->set_error(hipGetErrorString(e));
+    return FS_DEVICE_ERROR;
+  }
+  return FS_OK;
+}
+
 int fs_set_advector(fs_ctx* c, int32_t method) {
   FS_TRY(check_ctx(c));
-  if (method == FS_ADVECT_PIC || method == FS_ADVECT_FLIP) {
+  if (method == FS_ADVECT_PIC || method == FS_ADVECT_FLIP || method == FS_ADVECT_APIC) {
     c->cfg.advection = method;
     return FS_OK;
   }
-  c->set_error("advection method not implemented by the reference (APIC)");
-  return FS_UNSUPPORTED;
+  c->set_error("unknown advection method");
+  return FS_BAD_ARG;
 }
 
 int fs_set_projector(fs_ctx* c, int32_t method) {

@@ -155,6 +155,9 @@ struct fs_ctx {
   size_t scan_tmp_bytes = 0;
   size_t bins_cap = 0;
   DBuf red;  // small reduction scratch
+  // APIC affine rows, 9 per particle, component-major: [3a + d][particle] (allocated and zeroed
+  // by fs_ensure_apic when APIC is in use; freed when the particles change, so they restart at 0)
+  DBuf apicc;
   // the last surface built by fs_rebuild_surface (vertices and normals AoS xyz FP64, triangles)
   DBuf surf_v, surf_n, surf_i;
   int64_t surf_nv = 0, surf_nt = 0;
@@ -183,6 +186,8 @@ struct fs_ctx {
 int fs_alloc(fs_ctx* c, DBuf& b, size_t bytes);
 void fs_free(DBuf& b);
 int fs_zero(fs_ctx* c, DBuf& b);
+// APIC affine rows of the current particles (9 * np values at storage width), zeroed on first use
+int fs_ensure_apic(fs_ctx* c);
 
 // Stages implemented in the .hip files.
 int fs_stage_clear(fs_ctx* c);

@@ -1436,8 +1436,164 @@ int ensure_side(fs_ctx* c) {
 // a + 1 is binned on the main stream (two buffer sets, alternating); the main stream waits for the
 // gathers before the stage ends. With per-launch event timing on, everything stays on the main
 // stream, so that the timed intervals hold the whole stage.
+// ---- APIC (affine particle-in-cell, Jiang et al. 2015) ----------------------------------------
+// The reference declares AdvectionMethod::APIC without an implementation (setAdvector reaches
+// ERROR, cpu/fluid-simulator.h:96-97). Here each particle carries, per velocity component a, the
+// affine row c_a (the component's gradient near the particle); P2G transfers v_a + c_a.(x_n - x_p)
+// to every node, and G2P sets v_a to the sampled velocity and c_a = D^-1 b over the same nodes, with
+// b = sum w g (x_n - x_p) and D = sum w (x_n - x_p)(x_n - x_p)^T.
+
+// P2G, one thread per node of face grid a: the particles of the bins that reach the node (nearest
+// node b - 1 .. b + 2 around the node, so b in [i-2, i+1] per axis), bins in key order and
+// particles in index order inside a bin - every node sums in (key, index) order, the order of
+// FLIP's P2G. Writes the normalised velocity, the weight sum and the valid flag.
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_p2g_apic(int a, GridArgs<T> ga, int gx, int gy, int gz,
+                                                  const int* __restrict__ offs, const int* __restrict__ perm,
+                                                  const T* __restrict__ px, const T* __restrict__ py,
+                                                  const T* __restrict__ pz, const T* __restrict__ vc,
+                                                  const T* __restrict__ c0, const T* __restrict__ c1,
+                                                  const T* __restrict__ c2, T* __restrict__ G, T* __restrict__ Wt,
+                                                  uint8_t* __restrict__ valid) {
+  const int64_t id = blockIdx.x * (int64_t)TPB + threadIdx.x;
+  if (id >= (int64_t)gx * gy * gz) return;
+  const int k = (int)(id % gz), j = (int)((id / gz) % gy), i = (int)(id / ((int64_t)gz * gy));
+  const T off[3] = {a == 0 ? T(0) : T(0.5), a == 1 ? T(0) : T(0.5), a == 2 ? T(0) : T(0.5)};
+  const T pn[3] = {coord1(i, off[0], ga.h[0]), coord1(j, off[1], ga.h[1]), coord1(k, off[2], ga.h[2])};
+  const int nbk[3] = {ga.nx + 1, ga.ny + 1, ga.nz + 1};  // bin index range of the nearest-node keys
+  const int node[3] = {i, j, k};
+  int lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    lo[d] = max(node[d] - 2, 0);
+    hi[d] = min(node[d] + 1, nbk[d] - 1);
+  }
+  T g = T(0), w_sum = T(0);
+  for (int b0 = lo[0]; b0 <= hi[0]; b0++)
+    for (int b1 = lo[1]; b1 <= hi[1]; b1++)
+      for (int b2 = lo[2]; b2 <= hi[2]; b2++) {
+        const int bin = lin3(b0, b1, b2, nbk[1], nbk[2]);
+        for (int q = offs[bin]; q < offs[bin + 1]; q++) {
+          const int pi = perm[q];
+          const T d0 = px[pi] - pn[0], d1 = py[pi] - pn[1], d2 = pz[pi] - pn[2];  // p - x_n
+          const T w = bspline(wdiv(fabs(d0), ga.h[0])) * bspline(wdiv(fabs(d1), ga.h[1])) *
+                      bspline(wdiv(fabs(d2), ga.h[2]));
+          // x_n - p = -(p - x_n), exactly
+          const T val = vc[pi] + (c0[pi] * (-d0) + c1[pi] * (-d1) + c2[pi] * (-d2));
+          g = g + val * w;
+          w_sum = w_sum + w;
+        }
+      }
+  const bool ok = w_sum > T(0);
+  G[id] = ok ? g / w_sum : T(0);
+  Wt[id] = w_sum;
+  valid[id] = ok ? 1 : 0;
+}
+
+// G2P: per particle and component, the sampled velocity and the affine row (same nodes, weights
+// and order as sampleVelocity, util.cc:40-85).
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_g2p_apic(int n, GridArgs<T> ga, Grids3<T> G, const T* __restrict__ px,
+                                                  const T* __restrict__ py, const T* __restrict__ pz,
+                                                  T* __restrict__ vx, T* __restrict__ vy, T* __restrict__ vz,
+                                                  T* __restrict__ cm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T p[3] = {px[i], py[i], pz[i]};
+  const T hs[3] = {ga.h[0], ga.h[0], ga.h[0]};
+  T* vout[3] = {vx, vy, vz};
+#pragma unroll 1
+  for (int a = 0; a < 3; a++) {
+    const T off[3] = {a == 0 ? T(0) : T(0.5), a == 1 ? T(0) : T(0.5), a == 2 ? T(0) : T(0.5)};
+    int b[3];
+    T dn[3][4], wf[3][4];  // x_n - p per axis node, kernel factors
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      b[c] = nearest1(p[c], off[c], ga.h[c]) - 1;
+#pragma unroll
+      for (int o = 0; o < 4; o++) {
+        const T xn = coord1(b[c] + o, off[c], ga.h[c]);
+        dn[c][o] = xn - p[c];
+        wf[c][o] = bspline(wdiv(fabs(p[c] - xn), hs[c]));
+      }
+    }
+    const int gx = G.d[a][0], gy = G.d[a][1], gz = G.d[a][2];
+    const T* __restrict__ g = G.g[a];
+    T acc = T(0), ws = T(0), b0 = T(0), b1 = T(0), b2 = T(0);
+    T D00 = T(0), D01 = T(0), D02 = T(0), D11 = T(0), D12 = T(0), D22 = T(0);
+    for (int ii = 0; ii < 4; ii++) {
+      const int x = b[0] + ii;
+      if (x < 0 || x >= gx) continue;
+      for (int jj = 0; jj < 4; jj++) {
+        const int y = b[1] + jj;
+        if (y < 0 || y >= gy) continue;
+        for (int kk = 0; kk < 4; kk++) {
+          const int z = b[2] + kk;
+          if (z < 0 || z >= gz) continue;
+          const T gn = g[lin3(x, y, z, gy, gz)];
+          const T w = wf[0][ii] * wf[1][jj] * wf[2][kk];
+          acc = acc + gn * w;
+          ws = ws + w;
+          const T dx = dn[0][ii], dy = dn[1][jj], dz = dn[2][kk];
+          const T gw = gn * w;
+          b0 = b0 + gw * dx;
+          b1 = b1 + gw * dy;
+          b2 = b2 + gw * dz;
+          const T wx = w * dx, wy = w * dy, wz = w * dz;
+          D00 = D00 + wx * dx;
+          D01 = D01 + wx * dy;
+          D02 = D02 + wx * dz;
+          D11 = D11 + wy * dy;
+          D12 = D12 + wy * dz;
+          D22 = D22 + wz * dz;
+        }
+      }
+    }
+    const bool ok = ws > T(0);
+    vout[a][i] = ok ? acc / ws : T(0);
+    T r0 = T(0), r1 = T(0), r2 = T(0);
+    if (ok) {
+      const T i00 = D11 * D22 - D12 * D12, i01 = D02 * D12 - D01 * D22, i02 = D01 * D12 - D02 * D11;
+      const T i11 = D00 * D22 - D02 * D02, i12 = D01 * D02 - D00 * D12, i22 = D00 * D11 - D01 * D01;
+      const T det = D00 * i00 + D01 * i01 + D02 * i02;
+      if (det != T(0)) {
+        r0 = (i00 * b0 + i01 * b1 + i02 * b2) / det;
+        r1 = (i01 * b0 + i11 * b1 + i12 * b2) / det;
+        r2 = (i02 * b0 + i12 * b1 + i22 * b2) / det;
+      }
+    }
+    cm[(int64_t)(3 * a) * n + i] = r0;
+    cm[(int64_t)(3 * a + 1) * n + i] = r1;
+    cm[(int64_t)(3 * a + 2) * n + i] = r2;
+  }
+}
+
+template <typename T>
+int p2g_apic_t(fs_ctx* c) {
+  GridArgs<T> ga = grid_args<T>(c);
+  const int n = (int)c->np;
+  const int nb = (c->nx + 1) * (c->ny + 1) * (c->nz + 1);
+  FS_TRY(fs_ensure_apic(c));
+  const T* cm = c->apicc.as<T>();
+  for (int a = 0; a < 3; a++) {
+    if (n > 0)
+      k_keys_nearest<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, a, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
+                                                                  c->pos[2].as<T>(), c->keys.as<int>());
+    FS_TRY(counting_sort(c, nb, &c->offs));
+    const int64_t nf = c->n_face[a];
+    k_p2g_apic<T><<<fs_blocks(nf, TPB), TPB, 0, c->stream>>>(
+        a, ga, c->fdim[a][0], c->fdim[a][1], c->fdim[a][2], c->offs.as<int>(), c->perm.as<int>(), c->pos[0].as<T>(),
+        c->pos[1].as<T>(), c->pos[2].as<T>(), c->vel[a].as<T>(), cm + (int64_t)(3 * a) * n,
+        cm + (int64_t)(3 * a + 1) * n, cm + (int64_t)(3 * a + 2) * n, c->g[a].as<T>(), c->gw[a].as<T>(),
+        c->gvalid[a].as<uint8_t>());
+    FS_HIP(c, hipGetLastError());
+  }
+  return FS_OK;
+}
+
 template <typename T>
 int p2g_t(fs_ctx* c) {
+  if (c->cfg.advection == FS_ADVECT_APIC) return p2g_apic_t<T>(c);
   GridArgs<T> ga = grid_args<T>(c);
   int n = (int)c->np;
   int nb = (c->nx + 1) * (c->ny + 1) * (c->nz + 1);
@@ -1562,6 +1718,14 @@ int g2p_t(fs_ctx* c) {
   int n = (int)c->np;
   if (n == 0) return FS_OK;
   GridArgs<T> ga = grid_args<T>(c);
+  if (c->cfg.advection == FS_ADVECT_APIC) {
+    FS_TRY(fs_ensure_apic(c));
+    k_g2p_apic<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, grids_of<T>(c, c->g), c->pos[0].as<T>(),
+                                                            c->pos[1].as<T>(), c->pos[2].as<T>(), c->vel[0].as<T>(),
+                                                            c->vel[1].as<T>(), c->vel[2].as<T>(), c->apicc.as<T>());
+    FS_HIP(c, hipGetLastError());
+    return FS_OK;
+  }
   const bool flip = c->cfg.advection == FS_ADVECT_FLIP;
   const int pf = fs_prof_begin(c, FS_KERNEL_G2P,
                                (double)n * 9 * sizeof(T) +
