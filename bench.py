@@ -21,6 +21,8 @@ The JSON line carries:
                 reference's ||r||_inf < 1e-6 with its cap of 300 (--tol-mode abs, the default
                 headline), or ||r||_2 <= 1e-6 ||rhs||_2 with a cap of 1000 (--tol-mode rel).
   fp64_step     the same workload with FP64 arithmetic throughout, and the mixed/FP64 ratio.
+  other_preconditioner  the same workload with the other preconditioner (multigrid for the
+                MIC(0) headline, --precond mg for the reverse) and its speedup over the timed run.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
 """
@@ -48,9 +50,10 @@ def log(msg: str) -> None:
 def workload_config(simcraft_amd, wl: dict, tol_mode: int, profile: int):
     n = wl["n"]
     prec = simcraft_amd.PRECISION_MIXED if wl["precision"] == "mixed" else simcraft_amd.PRECISION_FP64
+    pre = simcraft_amd.PRECOND_MULTIGRID if wl.get("precond", "mic") == "mg" else simcraft_amd.PRECOND_MIC
     return simcraft_amd.default_config(nx=n, ny=n, nz=n, flip_alpha=wl["flip_alpha"], precision=prec,
                                        tol_mode=tol_mode, cg_max_iterations=wl.get("max_iters", 300),
-                                       profile_stages=profile)
+                                       profile_stages=profile, preconditioner=pre)
 
 
 REL_MAX_ITERS = 1000
@@ -98,6 +101,7 @@ def timed_run(simcraft_amd, scenes, wl: dict, tol_mode: str, steps: int, warmup:
     sim.close()
     return {"value": steps / elapsed, "unit": "steps/s", "steps": steps, "warmup": warmup,
             "ms_per_step": 1000.0 * elapsed / steps, "precision": wl["precision"],
+            "preconditioner": "multigrid" if wl.get("precond", "mic") == "mg" else "mic",
             "tol_mode": "abs_linf" if tol_mode == "abs" else "rel_l2", "pcg_max_iterations": wl["max_iters"],
             "pcg_iterations_mean": float(np.mean([s["cg_iterations"] for s in stats])), **solve_summary(stats)}
 
@@ -231,6 +235,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="dam_break_sphere_256_mixed")
     ap.add_argument("--tol-mode", choices=["abs", "rel"], default="abs")
+    ap.add_argument("--precond", choices=["mic", "mg"], default="mic",
+                    help="preconditioner of the timed run: the reference's MIC(0) (the headline) or multigrid")
     ap.add_argument("--max-iters", type=int, default=None,
                     help="PCG iteration cap (default: the reference's 300 for abs, 1000 for rel)")
     ap.add_argument("--no-secondary", action="store_true",
@@ -260,7 +266,7 @@ def main() -> int:
     import simcraft_amd
     from simcraft_amd import scenes
 
-    wl = dict(scenes.WORKLOADS[args.workload])
+    wl = dict(scenes.WORKLOADS[args.workload], precond=args.precond)
     tol_mode = simcraft_amd.TOL_REL_L2 if args.tol_mode == "rel" else simcraft_amd.TOL_ABS_LINF
     log(f"workload {args.workload}: {wl}")
     t0 = time.perf_counter()
@@ -319,7 +325,8 @@ def main() -> int:
                         "us_per_launch": kd["us_per_launch"], "bytes_per_launch": kd["bytes_per_launch"]}
             log("kernels: " + json.dumps(kernels))
     out = {
-        "metric": "FLIP steps/sec at 256^3 (MIC-PCG to 1e-6) + adjoint/forward time ratio",
+        "metric": ("FLIP steps/sec at 256^3 (MIC-PCG to 1e-6) + adjoint/forward time ratio" if args.precond == "mic"
+                   else "FLIP steps/sec at 256^3 (multigrid-PCG to 1e-6)"),
         "value": sps,
         "unit": "steps/s",
         "n_gpus": 1,
@@ -343,6 +350,7 @@ def main() -> int:
             "pcg_tolerance": 1e-6,
             "tol_mode": "abs_linf" if args.tol_mode == "abs" else "rel_l2",
             "pcg_max_iterations": wl.get("max_iters", 300),
+            "preconditioner": "mic" if args.precond == "mic" else "multigrid",
             "pcg_iterations_mean": float(np.mean(iters)),
             **solve_summary(stats),
         },
@@ -352,6 +360,7 @@ def main() -> int:
         "adjoint": None,
         "other_stopping_rule": None,
         "fp64_step": None,
+        "other_preconditioner": None,
     }
     sim.close()
     del sim
@@ -367,6 +376,13 @@ def main() -> int:
             r64["mixed_over_fp64"] = round(sps / r64["value"], 3)
             out["fp64_step"] = r64
             log(f"secondary: {r64}")
+        other_pc = "mg" if args.precond == "mic" else "mic"
+        log(f"secondary: the same workload with the {other_pc} preconditioner ...")
+        rpc = timed_run(simcraft_amd, scenes, dict(wl, precond=other_pc), args.tol_mode, args.secondary_steps,
+                        args.warmup)
+        rpc["speedup"] = round(rpc["value"] / sps, 3)
+        out["other_preconditioner"] = rpc
+        log(f"secondary: {rpc}")
     if not args.no_adjoint:
         log("adjoint rollout ...")
         out["adjoint"] = adjoint_ratio(simcraft_amd, scenes, args.adjoint_n, args.adjoint_steps, args.adjoint_every,

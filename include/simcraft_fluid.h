@@ -39,7 +39,7 @@ typedef enum fs_status {
   FS_NONFINITE = 3,     /* NaN/Inf residual */
   FS_BAD_ARG = 4,
   FS_DEVICE_ERROR = 5,  /* HIP error, or a device-side wait that exceeded its bound */
-  FS_UNSUPPORTED = 6,   /* e.g. Multigrid: declared but unimplemented in the reference */
+  FS_UNSUPPORTED = 6,   /* a configuration the library does not implement */
   FS_NO_DEVICE = 7
 } fs_status;
 
@@ -101,7 +101,10 @@ typedef struct fs_config {
                                      their time); 2 = also per-launch kernel timing (see
                                      fs_kernel_profile); while that timing is switched off, the SDF
                                      stages overlap P2G and ms_reconstruct is not separable */
-  int32_t reserved[7];
+  int32_t preconditioner;         /* FS_PRECOND_* (setCompressedSolver, cpu/fluid-simulator.h:109-113):
+                                     FS_PRECOND_MIC (default) or FS_PRECOND_MULTIGRID, this
+                                     library's aggregation multigrid (fs_mg_apply) */
+  int32_t reserved[6];
 } fs_config;
 
 typedef struct fs_stats {
@@ -208,6 +211,10 @@ int fs_spmv(fs_ctx* ctx, const double* x, double* y);
 /* ModifiedIncompleteCholesky3D::precond (project-solver.cc:230-306): z = M^-1 r over active
  * cells of the assembled system. */
 int fs_mic_apply(fs_ctx* ctx, const double* r, double* z);
+/* The multigrid preconditioner of FS_PRECOND_MULTIGRID (declared without an implementation in the
+ * reference, project-solver.h:117-118): one V-cycle of the assembled system, z = M^-1 r on every
+ * cell (r read on active cells; z is 0 on inactive ones). */
+int fs_mg_apply(fs_ctx* ctx, const double* r, double* z);
 
 /* ---- State access (dense reference layout) ----
  * Names: "u","v","w","ubuf","vbuf","wbuf","u_valid","v_valid","w_valid","u_valid_buf",
@@ -237,7 +244,9 @@ enum {
   FS_KERNEL_G2P = 6,
   FS_KERNEL_ADVECT = 7,
   FS_KERNEL_RECONSTRUCT = 8,
-  FS_KERNEL_COUNT = 9
+  FS_KERNEL_MG_LEVEL0 = 9,  /* multigrid cycle on the grid: sweeps, restriction, correction, z.r */
+  FS_KERNEL_MG_COARSE = 10, /* multigrid cycle below the grid (levels of 2x2x2 aggregates) */
+  FS_KERNEL_COUNT = 11
 };
 int fs_kernel_profile(fs_ctx* ctx, int32_t kind, int64_t* launches, double* total_ms, double* total_bytes);
 int fs_kernel_profile_reset(fs_ctx* ctx);

@@ -52,6 +52,7 @@ _SIGS = {
     "fso_cfl": (C.c_int, [_P, _D]),
     "fso_spmv": (C.c_int, [_P, _D, _D]),
     "fso_mic_apply": (C.c_int, [_P, _D, _D]),
+    "fso_mg_apply": (C.c_int, [_P, _D, _D]),
     "fso_array_info": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "fso_get_array": (C.c_int, [_P, C.c_char_p, _P]),
     "fso_set_array": (C.c_int, [_P, C.c_char_p, _P]),
@@ -252,6 +253,12 @@ class OracleSim:
         ra = np.ascontiguousarray(r, dtype=np.float64)
         za = np.zeros(self.shape) if z is None else np.array(z, dtype=np.float64, copy=True)
         self._lib.fso_mic_apply(self._ctx, _dp(ra), _dp(za))
+        return za
+
+    def mg_apply(self, r):
+        ra = np.ascontiguousarray(r, dtype=np.float64)
+        za = np.zeros(self.shape)
+        self._lib.fso_mg_apply(self._ctx, _dp(ra), _dp(za))
         return za
 
     def array_info(self, name):

@@ -1126,6 +1126,7 @@ struct Sim {
 
   // CgSolver3D::solve (project-solver.cc:308-346). Returns the final ||r||_inf.
   double solvePressure() {
+    if (cfg.preconditioner == FS_PRECOND_MULTIGRID) return solvePressureMg();
     const double tol = cfg.cg_tolerance;
     const int max_iterations = cfg.cg_max_iterations;
     bool tree = dot_mode == 1;
@@ -1174,6 +1175,239 @@ struct Sim {
       double sigma_new = tree ? tree_dot_sweep(layout, z, r) : dotSerial(z, r);
       double beta = sigma_new / sigma;
       for (size_t e = 0; e < s.size(); e++)  // scaleAndAdd (util.cc:404-411)
+        if (active.d[e]) s.d[e] = z.d[e] + beta * s.d[e];
+      sigma = sigma_new;
+    }
+    if (last_status == FS_OK && !conv) last_status = FS_NOT_CONVERGED;
+    last_iters = iter;
+    last_residual = residual;
+    return residual;
+  }
+
+  // ---- Multigrid-preconditioned CG --------------------------------------------------------------
+  // PreconditionerMethod::Multigrid is declared in the reference without an implementation
+  // (project-solver.h:117-118, setCompressedSolver reaches ERROR); this is the library's own
+  // multigrid, restated here as the device computes it (fs_mg.hip), for FP64 parity.
+  //   Level 0: the assembled system on the grid. D = Adiag on active cells; Cx/Cy/Cz = the
+  //     coupling to the -x/-y/-z neighbour (An[Left], An[Down], An[Back]) where both cells are
+  //     active, else 0; the +x coupling of a cell is Cx of its +x neighbour (the assembly is
+  //     symmetric). W = omega / D where D > 0, else 0.
+  //   Level l+1: 2x2x2 aggregates of level l, the Galerkin product P^T A P with piecewise-constant
+  //     P: D sums the children's D plus twice their couplings inside the aggregate, Cx sums the
+  //     -x couplings of the children on the aggregate's -x face. Down to a single cell.
+  //   V-cycle on b: two Jacobi sweeps from zero, the residual restricted by summation, the coarse
+  //     level's cycle, its solution added with weight alpha (over-correction for the
+  //     piecewise-constant prolongation), two Jacobi sweeps. The coarsest level runs four sweeps.
+  //     Every step is symmetric, so the cycle is a symmetric preconditioner.
+  //   CG: the reference's loop (project-solver.cc:308-346) with this preconditioner; vectors FP64;
+  //     dot products over the dense grid in chunks of 4096 cells (dense_tree_dot).
+  static constexpr double MG_OMEGA = 0.9, MG_ALPHA = 1.8;
+  struct MgLevel {
+    int nx = 0, ny = 0, nz = 0;
+    std::vector<double> D, Cx, Cy, Cz, W, b, x, xs;
+    size_t size() const { return (size_t)nx * ny * nz; }
+    void alloc(int a, int b_, int c) {
+      nx = a;
+      ny = b_;
+      nz = c;
+      for (auto* v : {&D, &Cx, &Cy, &Cz, &W, &b, &x, &xs}) v->assign(size(), 0.0);
+    }
+  };
+  std::vector<MgLevel> mg;
+
+  // sum order of a row: diagonal, -x, -y, -z, +x, +y, +z (applyCompressedMatrix's order,
+  // project-solver.cc:23-35); terms outside the grid are dropped
+  template <typename F>
+  static double mgRow(const MgLevel& L, int i, int j, int k, F xo) {
+    const size_t sx = (size_t)L.ny * L.nz, sy = (size_t)L.nz;
+    const size_t c = (size_t)k + sy * ((size_t)j + (size_t)L.ny * i);
+    double t = L.D[c] * xo(c);
+    if (i > 0) t += L.Cx[c] * xo(c - sx);
+    if (j > 0) t += L.Cy[c] * xo(c - sy);
+    if (k > 0) t += L.Cz[c] * xo(c - 1);
+    if (i < L.nx - 1) t += L.Cx[c + sx] * xo(c + sx);
+    if (j < L.ny - 1) t += L.Cy[c + sy] * xo(c + sy);
+    if (k < L.nz - 1) t += L.Cz[c + 1] * xo(c + 1);
+    return t;
+  }
+
+  void mgBuild() {
+    mg.clear();
+    mg.emplace_back();
+    MgLevel& L0 = mg.back();
+    L0.alloc(nx, ny, nz);
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++) {
+          const size_t c = Adiag.id(i, j, k);
+          if (!active(i, j, k)) continue;
+          L0.D[c] = Adiag.d[c];
+          if (i > 0 && active(i - 1, j, k)) L0.Cx[c] = An[Left].d[c];
+          if (j > 0 && active(i, j - 1, k)) L0.Cy[c] = An[Down].d[c];
+          if (k > 0 && active(i, j, k - 1)) L0.Cz[c] = An[Back].d[c];
+          L0.W[c] = L0.D[c] > 0.0 ? MG_OMEGA / L0.D[c] : 0.0;
+        }
+    while (mg.back().nx > 1 || mg.back().ny > 1 || mg.back().nz > 1) {
+      const MgLevel& F = mg.back();
+      MgLevel C;
+      C.alloc((F.nx + 1) / 2, (F.ny + 1) / 2, (F.nz + 1) / 2);
+      for (int I = 0; I < C.nx; I++)
+        for (int J = 0; J < C.ny; J++)
+          for (int K = 0; K < C.nz; K++) {
+            double d = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+            for (int di = 0; di < 2; di++)
+              for (int dj = 0; dj < 2; dj++)
+                for (int dk = 0; dk < 2; dk++) {
+                  const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
+                  if (i >= F.nx || j >= F.ny || k >= F.nz) continue;
+                  const size_t f = (size_t)k + (size_t)F.nz * ((size_t)j + (size_t)F.ny * i);
+                  d += F.D[f];
+                  if (di) d += 2.0 * F.Cx[f];
+                  if (dj) d += 2.0 * F.Cy[f];
+                  if (dk) d += 2.0 * F.Cz[f];
+                  if (!di) cx += F.Cx[f];
+                  if (!dj) cy += F.Cy[f];
+                  if (!dk) cz += F.Cz[f];
+                }
+            const size_t c = (size_t)K + (size_t)C.nz * ((size_t)J + (size_t)C.ny * I);
+            C.D[c] = d;
+            C.Cx[c] = cx;
+            C.Cy[c] = cy;
+            C.Cz[c] = cz;
+            C.W[c] = d > 0.0 ? MG_OMEGA / d : 0.0;
+          }
+      mg.push_back(std::move(C));
+    }
+  }
+
+  // one Jacobi sweep out = xo + W (b - A xo) on live cells, out = xo elsewhere
+  template <typename F>
+  static void mgSweep(MgLevel& L, F xo, std::vector<double>& out) {
+    for (int i = 0; i < L.nx; i++)
+      for (int j = 0; j < L.ny; j++)
+        for (int k = 0; k < L.nz; k++) {
+          const size_t c = (size_t)k + (size_t)L.nz * ((size_t)j + (size_t)L.ny * i);
+          const double w = L.W[c];
+          out[c] = w > 0.0 ? xo(c) + w * (L.b[c] - mgRow(L, i, j, k, xo)) : xo(c);
+        }
+  }
+
+  void mgCycle(size_t l) {
+    MgLevel& L = mg[l];
+    // two sweeps from zero: the first gives W b
+    mgSweep(L, [&](size_t n) { return L.W[n] * L.b[n]; }, L.x);
+    for (size_t c = 0; c < L.size(); c++)
+      if (!(L.W[c] > 0.0)) L.x[c] = 0.0;
+    if (l + 1 == mg.size()) {
+      mgSweep(L, [&](size_t n) { return L.x[n]; }, L.xs);
+      mgSweep(L, [&](size_t n) { return L.xs[n]; }, L.x);
+      return;
+    }
+    MgLevel& C = mg[l + 1];
+    for (int I = 0; I < C.nx; I++)
+      for (int J = 0; J < C.ny; J++)
+        for (int K = 0; K < C.nz; K++) {
+          double acc = 0.0;
+          for (int di = 0; di < 2; di++)
+            for (int dj = 0; dj < 2; dj++)
+              for (int dk = 0; dk < 2; dk++) {
+                const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
+                if (i >= L.nx || j >= L.ny || k >= L.nz) continue;
+                const size_t f = (size_t)k + (size_t)L.nz * ((size_t)j + (size_t)L.ny * i);
+                if (L.W[f] > 0.0) acc += L.b[f] - mgRow(L, i, j, k, [&](size_t n) { return L.x[n]; });
+              }
+          C.b[(size_t)K + (size_t)C.nz * ((size_t)J + (size_t)C.ny * I)] = acc;
+        }
+    mgCycle(l + 1);
+    // coarse correction (on live cells), then the first post-sweep
+    auto parent = [&](size_t n) {
+      const int k = (int)(n % L.nz), j = (int)((n / L.nz) % L.ny), i = (int)(n / ((size_t)L.nz * L.ny));
+      return (size_t)(k / 2) + (size_t)C.nz * ((size_t)(j / 2) + (size_t)C.ny * (i / 2));
+    };
+    auto corrected = [&](size_t n) { return L.W[n] > 0.0 ? L.x[n] + MG_ALPHA * C.x[parent(n)] : L.x[n]; };
+    mgSweep(L, corrected, L.xs);
+    mgSweep(L, [&](size_t n) { return L.xs[n]; }, L.x);
+  }
+
+  // z = M^-1 r over the grid (the multigrid V-cycle of the current system; mgBuild first)
+  void mgApply(const RArr& rr, RArr& zz) {
+    MgLevel& L = mg[0];
+    for (size_t c = 0; c < L.size(); c++) L.b[c] = active.d[c] ? rr.d[c] : 0.0;
+    mgCycle(0);
+    for (size_t c = 0; c < L.size(); c++) zz.d[c] = L.x[c];
+  }
+
+  // Dot product over the dense grid as the device reduces it: chunks of 4096 cells
+  // (block_tree), then block_tree over the chunk partials.
+  double dense_tree_dot(const std::vector<double>& a, const std::vector<double>& b) const {
+    const size_t n = a.size(), nc = (n + 4095) / 4096;
+    std::vector<double> parts(nc), chunk(4096);
+    for (size_t c = 0; c < nc; c++) {
+      for (size_t m = 0; m < 4096; m++) {
+        const size_t e = c * 4096 + m;
+        chunk[m] = e < n ? a[e] * b[e] : 0.0;
+      }
+      parts[c] = block_tree(chunk);
+    }
+    return block_tree(parts);
+  }
+
+  double solvePressureMg() {
+    const double tol = cfg.cg_tolerance;
+    const int max_iterations = cfg.cg_max_iterations;
+    mgBuild();
+    const MgLevel& L0 = mg[0];
+    pg.fill(0);
+    for (size_t e = 0; e < r.size(); e++) r.d[e] = active.d[e] ? rhs.d[e] : 0.0;
+    const double bnorm2 = dense_tree_dot(r.d, r.d);
+    auto converged = [&](double res) {
+      if (cfg.tol_mode == FS_TOL_REL_L2) return std::sqrt(dense_tree_dot(r.d, r.d)) <= tol * std::sqrt(bnorm2);
+      return res < tol;
+    };
+    double residual = linf(r);
+    last_status = FS_OK;
+    if (converged(residual)) {
+      last_iters = 0;
+      last_residual = residual;
+      return residual;
+    }
+    RArr t(nx, ny, nz);
+    mgApply(r, z);
+    s.d = z.d;
+    double sigma = dense_tree_dot(z.d, r.d);
+    int iter = 1;
+    bool conv = false;
+    for (; iter < max_iterations; iter++) {
+      for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+          for (int k = 0; k < nz; k++) {
+            const size_t c = t.id(i, j, k);
+            t.d[c] = active.d[c] ? mgRow(L0, i, j, k, [&](size_t n) { return s.d[n]; }) : 0.0;
+          }
+      const double sdott = dense_tree_dot(s.d, t.d);
+      if (sdott == 0.0) {
+        last_status = FS_BREAKDOWN;
+        break;
+      }
+      const double alpha = sigma / sdott;
+      for (size_t e = 0; e < pg.size(); e++)
+        if (active.d[e]) {
+          pg.d[e] += alpha * s.d[e];
+          r.d[e] += -alpha * t.d[e];
+        }
+      residual = linf(r);
+      if (!(residual <= 1.7976931348623157e308)) {
+        last_status = FS_NONFINITE;
+        break;
+      }
+      if (converged(residual)) {
+        conv = true;
+        break;
+      }
+      mgApply(r, z);
+      const double sigma_new = dense_tree_dot(z.d, r.d);
+      const double beta = sigma_new / sigma;
+      for (size_t e = 0; e < s.size(); e++)
         if (active.d[e]) s.d[e] = z.d[e] + beta * s.d[e];
       sigma = sigma_new;
     }
@@ -2532,6 +2766,7 @@ void fso_config_default(fs_config* cfg) {
   cfg->inner_tolerance = 1e-3;
   cfg->max_refinements = 8;
   cfg->check_interval = 8;
+  cfg->preconditioner = FS_PRECOND_MIC;
 }
 
 void* fso_create(const fs_config* cfg) {
@@ -2651,6 +2886,16 @@ int fso_spmv(void* sp, const double* x, double* y) {
   std::memcpy(Y.d.data(), y, Y.size() * sizeof(double));
   s->applyCompressedMatrix(X, Y);
   std::memcpy(y, Y.d.data(), Y.size() * sizeof(double));
+  return 0;
+}
+// The multigrid V-cycle of the current system: z = M^-1 r on every cell (0 on inactive ones).
+int fso_mg_apply(void* sp, const double* r, double* z) {
+  Sim* s = static_cast<Sim*>(sp);
+  fso::RArr R(s->nx, s->ny, s->nz), Z(s->nx, s->ny, s->nz);
+  std::memcpy(R.d.data(), r, R.size() * sizeof(double));
+  s->mgBuild();
+  s->mgApply(R, Z);
+  std::memcpy(z, Z.d.data(), Z.size() * sizeof(double));
   return 0;
 }
 int fso_mic_apply(void* sp, const double* r, double* z) {

@@ -13,6 +13,7 @@ import numpy as np
 
 from . import _capi
 from ._capi import (ADVECT_APIC, ADVECT_FLIP, ADVECT_PIC, MESH_EXACT_DISTANCE, PRECISION_FP64, PRECISION_MIXED, PRECOND_MIC,
+                    PRECOND_MULTIGRID,
                     PROJECT_FVM, RECON_NAIVE, SOLVER_CG, TOL_ABS_LINF, TOL_REL_L2, fs_config, fs_stats)
 
 __all__ = ["FluidSim", "FluidSimError", "default_config", "device_count", "fs_config", "fs_stats"]
@@ -230,6 +231,13 @@ class FluidSim:
         ra = np.ascontiguousarray(r, dtype=np.float64)
         za = np.zeros(self.shape) if z is None else np.array(z, dtype=np.float64, copy=True)
         self._check(self._lib.fs_mic_apply(self._ctx, _dptr(ra), _dptr(za)))
+        return za
+
+    def mg_apply(self, r: np.ndarray) -> np.ndarray:
+        """One multigrid V-cycle of the assembled system (FS_PRECOND_MULTIGRID's preconditioner)."""
+        ra = np.ascontiguousarray(r, dtype=np.float64)
+        za = np.zeros(self.shape)
+        self._check(self._lib.fs_mg_apply(self._ctx, _dptr(ra), _dptr(za)))
         return za
 
     # -- named state ---------------------------------------------------------------------------

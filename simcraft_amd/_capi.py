@@ -40,7 +40,8 @@ class fs_config(C.Structure):
         ("cfl_factor", C.c_double),
         ("inner_tolerance", C.c_double),
         ("max_refinements", C.c_int32), ("check_interval", C.c_int32), ("profile_stages", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("preconditioner", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -99,6 +100,7 @@ SIGNATURES = {
     "fs_g2p": (C.c_int, [_P, C.c_double]),
     "fs_spmv": (C.c_int, [_P, _D, _D]),
     "fs_mic_apply": (C.c_int, [_P, _D, _D]),
+    "fs_mg_apply": (C.c_int, [_P, _D, _D]),
     "fs_array_info": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "fs_get_array": (C.c_int, [_P, C.c_char_p, _P]),
     "fs_set_array": (C.c_int, [_P, C.c_char_p, _P]),
@@ -118,14 +120,16 @@ SIGNATURES = {
                                         _P]),
 }
 
-KERNEL_NAMES = ["spmv", "mic_fwd", "mic_bwd", "pcg_update", "mic_factor", "p2g", "g2p", "advect", "reconstruct"]
+KERNEL_NAMES = ["spmv", "mic_fwd", "mic_bwd", "pcg_update", "mic_factor", "p2g", "g2p", "advect", "reconstruct",
+                "mg_level0", "mg_coarse"]
 # HIP kernel symbols behind each class (for matching rocprofv3 traces)
 KERNEL_SYMBOLS = {
-    "spmv": ["k_spmv"], "mic_fwd": ["k_fwd"], "mic_bwd": ["k_bwd"], "pcg_update": ["k_update"],
+    "spmv": ["k_spmv", "k_mg_spmv"], "mic_fwd": ["k_fwd"], "mic_bwd": ["k_bwd"], "pcg_update": ["k_update", "k_mg_update"],
     "mic_factor": ["k_factor_sweep"], "p2g": ["k_p2g_records", "k_p2g_gather", "k_keys_nearest", "k_count",
                                              "k_scatter", "k_sort_bins"],
     "g2p": ["k_g2p"], "advect": ["k_advect"], "reconstruct": ["k_keys_morton", "k_bin_last", "k_recon_splat",
                                                                "k_recon_final"],
+    "mg_level0": ["k_mg_pre", "k_mg_restrict", "k_mg_post1", "k_mg_post2"], "mg_coarse": [],
 }
 
 _lib = None

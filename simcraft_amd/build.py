@@ -18,7 +18,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OBJDIR = PKG / "_obj"
 LIB = PKG / "libsimcraft_fluid.so"
-SOURCES = ["fs_grid.hip", "fs_pcg.hip", "fs_capi.hip", "fs_adjoint.hip", "fs_noalloc.hip", "fs_mesh.hip", "fs_surface.hip"]
+SOURCES = ["fs_grid.hip", "fs_pcg.hip", "fs_capi.hip", "fs_adjoint.hip", "fs_noalloc.hip", "fs_mesh.hip", "fs_surface.hip", "fs_mg.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 COMMON = [

@@ -98,6 +98,7 @@ void release_all(fs_ctx* c) {
     fs_free(*b);
   for (int q = 0; q < 6; q++) fs_free(c->an[q]);
   fs_pcg_release(c);
+  fs_mg_release(c);
   fs_adj_release(c);
 }
 
@@ -204,6 +205,7 @@ void fs_config_default(fs_config* cfg) {
   cfg->max_refinements = 8;
   cfg->check_interval = 8;
   cfg->profile_stages = 0;
+  cfg->preconditioner = FS_PRECOND_MIC;
 }
 
 int fs_device_count(void) {
@@ -235,6 +237,10 @@ int fs_create(const fs_config* cfg, fs_ctx** out) {
   }
   if (cfg->precision != FS_PRECISION_FP64 && cfg->precision != FS_PRECISION_MIXED) {
     g_create_error = "unknown precision";
+    return FS_BAD_ARG;
+  }
+  if (cfg->preconditioner != FS_PRECOND_MIC && cfg->preconditioner != FS_PRECOND_MULTIGRID) {
+    g_create_error = "unknown preconditioner";
     return FS_BAD_ARG;
   }
   int ndev = 0;
@@ -464,10 +470,11 @@ int fs_set_compressed_solver(fs_ctx* c, int32_t solver, int32_t precond) {
     c->set_error("unknown compressed solver");
     return FS_UNSUPPORTED;
   }
-  if (precond != FS_PRECOND_MIC) {
-    c->set_error("preconditioner not implemented by the reference (Multigrid)");
-    return FS_UNSUPPORTED;
+  if (precond != FS_PRECOND_MIC && precond != FS_PRECOND_MULTIGRID) {
+    c->set_error("unknown preconditioner");
+    return FS_BAD_ARG;
   }
+  c->cfg.preconditioner = precond;
   return FS_OK;
 }
 
@@ -553,6 +560,12 @@ int fs_mic_apply(fs_ctx* c, const double* r, double* z) {
   FS_TRY(check_ctx(c));
   if (!r || !z) return FS_BAD_ARG;
   return fs_pcg_mic_dense(c, r, z);
+}
+
+int fs_mg_apply(fs_ctx* c, const double* r, double* z) {
+  FS_TRY(check_ctx(c));
+  if (!r || !z) return FS_BAD_ARG;
+  return fs_mg_apply_dense(c, r, z);
 }
 
 // cpu::FluidSimulator::substep (FluidSim/src/cpu/fluid-simulator.cc:140-182).

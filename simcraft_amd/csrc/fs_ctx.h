@@ -51,6 +51,7 @@ struct PcgScalars {
   int32_t conv_iter;
   int32_t status;
   int32_t wait_timeouts;
+  unsigned long long n_active;  // active cells (multigrid solve; integer sum, order-free)
 };
 
 struct PcgState {
@@ -78,6 +79,22 @@ struct PcgState {
   // mixed-precision copies
   DBuf f_adiag, f_al, f_ad, f_ab, f_e, f_cl, f_cd, f_cb, f_dr, f_du, f_df;
   DBuf f_r, f_s, f_t, f_q, f_z, f_x;
+};
+
+// Multigrid-preconditioned CG (fs_mg.hip). Levels are dense grids, k fastest, storage type of the
+// context; level 0 is the assembled system, level l+1 the 2x2x2 aggregation of level l.
+struct MgLevelBuf {
+  int nx = 0, ny = 0, nz = 0;
+  int64_t n = 0;
+  DBuf D, Cx, Cy, Cz, W;  // diagonal, couplings to the -x/-y/-z neighbour, Jacobi weight omega/D
+  DBuf b, x, xs;          // right-hand side (levels >= 1), cycle solution and its ping-pong copy
+};
+struct MgState {
+  std::vector<MgLevelBuf> lv;
+  DBuf x, r, s0, s1, t;  // CG vectors (FP64); z is level 0's x
+  DBuf partials, partials2, scal;
+  PcgScalars* host_scal = nullptr;  // pinned
+  int nchunks = 0;
 };
 
 // Reverse mode. One recorded substep (device copies, context precision) and the rollout tape:
@@ -162,6 +179,7 @@ struct fs_ctx {
   DBuf surf_v, surf_n, surf_i;
   int64_t surf_nv = 0, surf_nt = 0;
   PcgState pcg;
+  MgState mg;
   hipEvent_t ev[12];
   bool ev_ok = false;
   // kernel timing (profile_stages >= 2): event pairs recorded around launches, resolved at the
@@ -224,6 +242,10 @@ int fs_pcg_spmv_dense(fs_ctx* c, const double* x, double* y);
 int fs_pcg_mic_dense(fs_ctx* c, const double* r, double* z);
 int fs_pcg_export_e(fs_ctx* c, double* out);
 void fs_pcg_release(fs_ctx* c);
+// Multigrid-preconditioned CG of the assembled system (fs_mg.hip).
+int fs_mg_solve(fs_ctx* c, fs_stats* st);
+int fs_mg_apply_dense(fs_ctx* c, const double* r, double* z);
+void fs_mg_release(fs_ctx* c);
 int fs_particles_upload(fs_ctx* c, const double* pos, const double* vel);
 int fs_particles_download(fs_ctx* c, double* pos, double* vel);
 
