@@ -58,17 +58,18 @@ __device__ __forceinline__ void decode3(int64_t c, int ny, int nz, int& i, int& 
 }
 
 // Row c of the level's operator applied to xo: diagonal, -x, -y, -z, +x, +y, +z (the order of
-// applyCompressedMatrix, project-solver.cc:23-35); terms outside the grid are dropped.
+// applyCompressedMatrix, project-solver.cc:23-35); terms outside the grid are dropped. xo(q, i, j, k)
+// gives the operand at cell q = (i, j, k).
 template <typename R, typename S, typename XO>
 __device__ __forceinline__ R row(const Lv<S>& L, int64_t c, int i, int j, int k, XO xo) {
   const int64_t sx = (int64_t)L.ny * L.nz, sy = L.nz;
-  R t = (R)L.D[c] * xo(c);
-  if (i > 0) t = t + (R)L.Cx[c] * xo(c - sx);
-  if (j > 0) t = t + (R)L.Cy[c] * xo(c - sy);
-  if (k > 0) t = t + (R)L.Cz[c] * xo(c - 1);
-  if (i < L.nx - 1) t = t + (R)L.Cx[c + sx] * xo(c + sx);
-  if (j < L.ny - 1) t = t + (R)L.Cy[c + sy] * xo(c + sy);
-  if (k < L.nz - 1) t = t + (R)L.Cz[c + 1] * xo(c + 1);
+  R t = (R)L.D[c] * xo(c, i, j, k);
+  if (i > 0) t = t + (R)L.Cx[c] * xo(c - sx, i - 1, j, k);
+  if (j > 0) t = t + (R)L.Cy[c] * xo(c - sy, i, j - 1, k);
+  if (k > 0) t = t + (R)L.Cz[c] * xo(c - 1, i, j, k - 1);
+  if (i < L.nx - 1) t = t + (R)L.Cx[c + sx] * xo(c + sx, i + 1, j, k);
+  if (j < L.ny - 1) t = t + (R)L.Cy[c + sy] * xo(c + sy, i, j + 1, k);
+  if (k < L.nz - 1) t = t + (R)L.Cz[c + 1] * xo(c + 1, i, j, k + 1);
   return t;
 }
 
@@ -132,40 +133,20 @@ __global__ __launch_bounds__(256) void k_mg_coarsen(Lv<S> F, int cx_, int cy_, i
 // ---- V-cycle ------------------------------------------------------------------------------------
 // B is the type of the level's right-hand side: FP64 on level 0 (the CG residual), S below.
 // sc (may be null): the kernels of a CG iteration return at once after the solve has stopped.
+// A cell with W = 0 is never written: the level's x and xs are zeroed when the hierarchy is
+// built, and stay 0 there for the whole solve.
 
-// x = two Jacobi sweeps from zero: x1 = W b, x = x1 + W (b - A x1); 0 where W = 0.
+// x = two Jacobi sweeps from zero at a live cell: x1 = W b, x = x1 + W (b - A x1).
 template <typename S, typename B>
-__global__ __launch_bounds__(256) void k_mg_pre(Lv<S> L, const B* __restrict__ b, S* __restrict__ x,
-                                                const PcgScalars* sc) {
-  if (sc && sc->converged != STOP_RUN) return;
-  const int64_t n = (int64_t)L.nx * L.ny * L.nz;
-  auto xo = [&](int64_t q) { return L.W[q] * (S)b[q]; };
-#pragma unroll 1
-  for (int m = 0; m < 16; m++) {
-    const int64_t c = (int64_t)blockIdx.x * MCH + m * 256 + threadIdx.x;
-    if (c >= n) break;
-    const S w = L.W[c];
-    S out = S(0);
-    if (w > S(0)) {
-      int i, j, k;
-      decode3(c, L.ny, L.nz, i, j, k);
-      out = xo(c) + w * ((S)b[c] - row<S>(L, c, i, j, k, xo));
-    }
-    x[c] = out;
-  }
+__device__ __forceinline__ S pre_cell(const Lv<S>& L, const B* __restrict__ b, int64_t c, int i, int j, int k) {
+  auto xo = [&](int64_t q, int, int, int) { return L.W[q] * (S)b[q]; };
+  return xo(c, i, j, k) + L.W[c] * ((S)b[c] - row<S>(L, c, i, j, k, xo));
 }
-
-// bc(I) = sum over the live children f of aggregate I (lexicographic order) of b(f) - (A x)(f).
+// b(f) - (A x)(f) summed over the live children f of aggregate (I, J, K), in lexicographic order.
 template <typename S, typename B>
-__global__ __launch_bounds__(256) void k_mg_restrict(Lv<S> L, const B* __restrict__ b, const S* __restrict__ x,
-                                                     int cnx, int cny, int cnz, S* __restrict__ bc,
-                                                     const PcgScalars* sc) {
-  if (sc && sc->converged != STOP_RUN) return;
-  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (c >= (int64_t)cnx * cny * cnz) return;
-  int I, J, K;
-  decode3(c, cny, cnz, I, J, K);
-  auto xo = [&](int64_t q) { return x[q]; };
+__device__ __forceinline__ S restrict_cell(const Lv<S>& L, const B* __restrict__ b, const S* __restrict__ x, int I,
+                                           int J, int K) {
+  auto xo = [&](int64_t q, int, int, int) { return x[q]; };
   S acc = S(0);
   for (int di = 0; di < 2; di++)
     for (int dj = 0; dj < 2; dj++)
@@ -175,40 +156,73 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Lv<S> L, const B* __restric
         const int64_t f = ((int64_t)i * L.ny + j) * L.nz + k;
         if (L.W[f] > S(0)) acc = acc + ((S)b[f] - row<S>(L, f, i, j, k, xo));
       }
-  bc[c] = acc;
+  return acc;
+}
+// The coarse correction x' = x + alpha xc(parent) on live cells, then one sweep at a live cell.
+template <typename S, typename B>
+__device__ __forceinline__ S post1_cell(const Lv<S>& L, const B* __restrict__ b, const S* __restrict__ x,
+                                        const S* __restrict__ xc, int cny, int cnz, S alpha, int64_t c, int i, int j,
+                                        int k) {
+  auto xo = [&](int64_t q, int qi, int qj, int qk) {
+    return L.W[q] > S(0) ? x[q] + alpha * xc[((int64_t)(qi >> 1) * cny + (qj >> 1)) * cnz + (qk >> 1)] : x[q];
+  };
+  return xo(c, i, j, k) + L.W[c] * ((S)b[c] - row<S>(L, c, i, j, k, xo));
+}
+// One sweep x = xs + W (b - A xs) at a live cell.
+template <typename S, typename B>
+__device__ __forceinline__ S sweep_cell(const Lv<S>& L, const B* __restrict__ b, const S* __restrict__ xs, int64_t c,
+                                        int i, int j, int k) {
+  auto xo = [&](int64_t q, int, int, int) { return xs[q]; };
+  return xs[c] + L.W[c] * ((S)b[c] - row<S>(L, c, i, j, k, xo));
 }
 
-// Coarse correction and the first post-sweep: x' = x + alpha xc(parent) on live cells,
-// xs = x' + W (b - A x').
+template <typename S, typename B>
+__global__ __launch_bounds__(256) void k_mg_pre(Lv<S> L, const B* __restrict__ b, S* __restrict__ x,
+                                                const PcgScalars* sc) {
+  if (sc && sc->converged != STOP_RUN) return;
+  const int64_t n = (int64_t)L.nx * L.ny * L.nz;
+#pragma unroll 1
+  for (int m = 0; m < 16; m++) {
+    const int64_t c = (int64_t)blockIdx.x * MCH + m * 256 + threadIdx.x;
+    if (c >= n) break;
+    if (!(L.W[c] > S(0))) continue;
+    int i, j, k;
+    decode3(c, L.ny, L.nz, i, j, k);
+    x[c] = pre_cell<S, B>(L, b, c, i, j, k);
+  }
+}
+
+template <typename S, typename B>
+__global__ __launch_bounds__(256) void k_mg_restrict(Lv<S> L, const B* __restrict__ b, const S* __restrict__ x,
+                                                     int cnx, int cny, int cnz, S* __restrict__ bc,
+                                                     const PcgScalars* sc) {
+  if (sc && sc->converged != STOP_RUN) return;
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= (int64_t)cnx * cny * cnz) return;
+  int I, J, K;
+  decode3(c, cny, cnz, I, J, K);
+  bc[c] = restrict_cell<S, B>(L, b, x, I, J, K);
+}
+
 template <typename S, typename B>
 __global__ __launch_bounds__(256) void k_mg_post1(Lv<S> L, const B* __restrict__ b, const S* __restrict__ x,
                                                   const S* __restrict__ xc, int cny, int cnz, S alpha,
                                                   S* __restrict__ xs, const PcgScalars* sc) {
   if (sc && sc->converged != STOP_RUN) return;
   const int64_t n = (int64_t)L.nx * L.ny * L.nz;
-  auto xo = [&](int64_t q) {
-    if (!(L.W[q] > S(0))) return x[q];
-    int i, j, k;
-    decode3(q, L.ny, L.nz, i, j, k);
-    return x[q] + alpha * xc[((int64_t)(i >> 1) * cny + (j >> 1)) * cnz + (k >> 1)];
-  };
 #pragma unroll 1
   for (int m = 0; m < 16; m++) {
     const int64_t c = (int64_t)blockIdx.x * MCH + m * 256 + threadIdx.x;
     if (c >= n) break;
-    const S w = L.W[c];
-    S out = x[c];
-    if (w > S(0)) {
-      int i, j, k;
-      decode3(c, L.ny, L.nz, i, j, k);
-      out = xo(c) + w * ((S)b[c] - row<S>(L, c, i, j, k, xo));
-    }
-    xs[c] = out;
+    if (!(L.W[c] > S(0))) continue;
+    int i, j, k;
+    decode3(c, L.ny, L.nz, i, j, k);
+    xs[c] = post1_cell<S, B>(L, b, x, xc, cny, cnz, alpha, c, i, j, k);
   }
 }
 
-// One sweep x = xs + W (b - A xs). DOT (level 0 of a CG cycle): z.r over the chunk, reduced by the
-// last block into sigma (FIRST: the initial sigma; else beta = sigma_new / sigma, sigma = sigma_new).
+// One sweep from xs into x. DOT (level 0 of a CG cycle): z.r over the chunk, reduced by the last
+// block into sigma (FIRST: the initial sigma; else beta = sigma_new / sigma, sigma = sigma_new).
 template <typename S, typename B, bool DOT, bool FIRST>
 __global__ __launch_bounds__(256) void k_mg_post2(Lv<S> L, const B* __restrict__ b, const S* __restrict__ xs,
                                                   S* __restrict__ x, double* __restrict__ part, int nchunks,
@@ -216,19 +230,15 @@ __global__ __launch_bounds__(256) void k_mg_post2(Lv<S> L, const B* __restrict__
   __shared__ double sm[4];
   if (sc && sc->converged != STOP_RUN) return;
   const int64_t n = (int64_t)L.nx * L.ny * L.nz;
-  auto xo = [&](int64_t q) { return xs[q]; };
   double acc = 0.0;
 #pragma unroll 1
   for (int m = 0; m < 16; m++) {
     const int64_t c = (int64_t)blockIdx.x * MCH + m * 256 + threadIdx.x;
     if (c >= n) break;
-    const S w = L.W[c];
-    S out = xs[c];
-    if (w > S(0)) {
-      int i, j, k;
-      decode3(c, L.ny, L.nz, i, j, k);
-      out = xs[c] + w * ((S)b[c] - row<S>(L, c, i, j, k, xo));
-    }
+    if (!(L.W[c] > S(0))) continue;  // z = 0 there, and so is its product with r
+    int i, j, k;
+    decode3(c, L.ny, L.nz, i, j, k);
+    const S out = sweep_cell<S, B>(L, b, xs, c, i, j, k);
     x[c] = out;
     if (DOT) acc += (double)out * (double)b[c];
   }
@@ -242,6 +252,71 @@ __global__ __launch_bounds__(256) void k_mg_post2(Lv<S> L, const B* __restrict__
       if (!FIRST) sc->beta = sig / sc->sigma;
       sc->sigma = sig;
     }
+  }
+}
+
+// The levels from l0 down to the single cell and back up, in one workgroup (the small levels,
+// where separate launches would cost more than the work): the same cell functions, the passes
+// separated by workgroup barriers.
+constexpr int MG_TAIL_CELLS = 4096;  // levels at or below this size run in the tail kernel
+constexpr int MG_MAX_LEVELS = 16;
+template <typename S>
+struct TailLevels {
+  Lv<S> L[MG_MAX_LEVELS];
+  S* b[MG_MAX_LEVELS];
+  S* x[MG_MAX_LEVELS];
+  S* xs[MG_MAX_LEVELS];
+};
+template <typename S>
+__global__ __launch_bounds__(1024) void k_mg_tail(TailLevels<S> T, int l0, int nl, S alpha, const PcgScalars* sc) {
+  if (sc && sc->converged != STOP_RUN) return;
+  const int tid = threadIdx.x;
+  for (int l = l0; l < nl; l++) {
+    const Lv<S>& L = T.L[l];
+    const int n = L.nx * L.ny * L.nz;
+    for (int c = tid; c < n; c += 1024) {
+      if (!(L.W[c] > S(0))) continue;
+      int i, j, k;
+      decode3(c, L.ny, L.nz, i, j, k);
+      T.x[l][c] = pre_cell<S, S>(L, T.b[l], c, i, j, k);
+    }
+    __syncthreads();
+    if (l + 1 < nl) {
+      const Lv<S>& C = T.L[l + 1];
+      const int nc = C.nx * C.ny * C.nz;
+      for (int c = tid; c < nc; c += 1024) {
+        int I, J, K;
+        decode3(c, C.ny, C.nz, I, J, K);
+        T.b[l + 1][c] = restrict_cell<S, S>(L, T.b[l], T.x[l], I, J, K);
+      }
+      __syncthreads();
+    }
+  }
+  auto sweep = [&](int l, const S* from, S* to) {
+    const Lv<S>& L = T.L[l];
+    const int n = L.nx * L.ny * L.nz;
+    for (int c = tid; c < n; c += 1024) {
+      if (!(L.W[c] > S(0))) continue;
+      int i, j, k;
+      decode3(c, L.ny, L.nz, i, j, k);
+      to[c] = sweep_cell<S, S>(L, T.b[l], from, c, i, j, k);
+    }
+    __syncthreads();
+  };
+  sweep(nl - 1, T.x[nl - 1], T.xs[nl - 1]);
+  sweep(nl - 1, T.xs[nl - 1], T.x[nl - 1]);
+  for (int l = nl - 2; l >= l0; l--) {
+    const Lv<S>& L = T.L[l];
+    const Lv<S>& C = T.L[l + 1];
+    const int n = L.nx * L.ny * L.nz;
+    for (int c = tid; c < n; c += 1024) {
+      if (!(L.W[c] > S(0))) continue;
+      int i, j, k;
+      decode3(c, L.ny, L.nz, i, j, k);
+      T.xs[l][c] = post1_cell<S, S>(L, T.b[l], T.x[l], T.x[l + 1], C.ny, C.nz, alpha, c, i, j, k);
+    }
+    __syncthreads();
+    sweep(l, T.xs[l], T.x[l]);
   }
 }
 
@@ -295,7 +370,8 @@ __global__ __launch_bounds__(256) void k_mg_init(int64_t n, const uint8_t* __res
 }
 
 // s = z (iteration 1) or z + beta s_old, recomputed for the six neighbours; t = A s (the assembled
-// system in FP64 on active cells, 0 elsewhere); s.t reduced by the last block into alpha.
+// system in FP64 on active cells); s.t reduced by the last block into alpha. Inactive cells are not
+// written: s and t are zeroed when the solve starts and stay 0 there.
 template <typename S>
 __global__ __launch_bounds__(256) void k_mg_spmv(Lv<S> L, const uint8_t* __restrict__ act, const S* __restrict__ z,
                                                  const double* __restrict__ s_old, double* __restrict__ s_new,
@@ -306,19 +382,17 @@ __global__ __launch_bounds__(256) void k_mg_spmv(Lv<S> L, const uint8_t* __restr
   const int64_t n = (int64_t)L.nx * L.ny * L.nz;
   const bool first = iter == 1;
   const double beta = sc->beta;
-  auto so = [&](int64_t q) { return first ? (double)z[q] : (double)z[q] + beta * s_old[q]; };
+  auto so = [&](int64_t q, int, int, int) { return first ? (double)z[q] : (double)z[q] + beta * s_old[q]; };
   double acc = 0.0;
 #pragma unroll 1
   for (int m = 0; m < 16; m++) {
     const int64_t c = (int64_t)blockIdx.x * MCH + m * 256 + threadIdx.x;
     if (c >= n) break;
-    double sv = 0.0, tv = 0.0;
-    if (act[c]) {
-      int i, j, k;
-      decode3(c, L.ny, L.nz, i, j, k);
-      sv = so(c);
-      tv = row<double>(L, c, i, j, k, so);
-    }
+    if (!act[c]) continue;
+    int i, j, k;
+    decode3(c, L.ny, L.nz, i, j, k);
+    const double sv = so(c, i, j, k);
+    const double tv = row<double>(L, c, i, j, k, so);
     s_new[c] = sv;
     t[c] = tv;
     acc += sv * tv;
@@ -355,14 +429,12 @@ __global__ __launch_bounds__(256) void k_mg_update(int64_t n, const uint8_t* __r
   for (int m = 0; m < 16; m++) {
     const int64_t c = (int64_t)blockIdx.x * MCH + m * 256 + threadIdx.x;
     if (c >= n) break;
-    double rv = r[c];
-    if (act[c]) {
-      x[c] = x[c] + alpha * s[c];
-      rv = rv + (-alpha) * t[c];
-      r[c] = rv;
-      const double av = fabs(rv);
-      mx = (mx < av) ? av : mx;
-    }
+    if (!act[c]) continue;  // r = 0 there
+    x[c] = x[c] + alpha * s[c];
+    const double rv = r[c] + (-alpha) * t[c];
+    r[c] = rv;
+    const double av = fabs(rv);
+    mx = (mx < av) ? av : mx;
     ss += rv * rv;
   }
   const double bm = block_max256(mx, sm);
@@ -431,6 +503,13 @@ int mg_build(fs_ctx* c) {
     const size_t bytes = (size_t)B.n * sizeof(S);
     for (DBuf* d : {&B.D, &B.Cx, &B.Cy, &B.Cz, &B.W, &B.x, &B.xs}) FS_TRY(ensure(c, *d, bytes));
     if (l > 0) FS_TRY(ensure(c, B.b, bytes));
+    // the cycle never writes a cell with W = 0: its iterates stay 0 there for the solve
+    FS_HIP(c, hipMemsetAsync(B.x.p, 0, bytes, c->stream));
+    FS_HIP(c, hipMemsetAsync(B.xs.p, 0, bytes, c->stream));
+  }
+  if ((int)M.lv.size() > MG_MAX_LEVELS) {
+    c->set_error("multigrid: too many levels");
+    return FS_BAD_ARG;
   }
   MgLevelBuf& L0 = M.lv[0];
   k_mg_level0<S><<<fs_blocks(L0.n, 256), 256, 0, c->stream>>>(
@@ -495,6 +574,14 @@ int mg_cycle(fs_ctx* c, const double* r0, PcgScalars* sc, bool first) {
                                                                      M.partials.as<double>(), (int)nb, sc);
     }
   };
+  // the levels from lt on run in one workgroup (the first level at or below MG_TAIL_CELLS cells,
+  // never level 0, whose right-hand side is the FP64 residual)
+  int lt = nl;
+  for (int l = 1; l < nl; l++)
+    if (M.lv[l].n <= MG_TAIL_CELLS) {
+      lt = l;
+      break;
+    }
   // down
   int pf = fs_prof_begin(c, FS_KERNEL_MG_LEVEL0, (16.0 + 12.0 * sizeof(S)) * na);
   pre(0);
@@ -502,18 +589,23 @@ int mg_cycle(fs_ctx* c, const double* r0, PcgScalars* sc, bool first) {
   fs_prof_end(c, pf);
   // coarse levels hold about 1/7 of the active cells; 31 values per cell over the four passes
   pf = fs_prof_begin(c, FS_KERNEL_MG_COARSE, 31.0 * sizeof(S) * na / 7.0);
-  for (int l = 1; l < nl; l++) {
+  for (int l = 1; l < lt; l++) {
     pre(l);
-    if (l + 1 < nl) restrict_(l);
+    restrict_(l);
   }
-  // the coarsest level: two more sweeps
-  if (nl > 1) {
-    MgLevelBuf& B = M.lv[nl - 1];
-    sweep(nl - 1, B.x, B.xs, false);
-    sweep(nl - 1, B.xs, B.x, false);
+  if (lt < nl) {
+    TailLevels<S> T{};
+    for (int l = lt; l < nl; l++) {
+      MgLevelBuf& B = M.lv[l];
+      T.L[l] = lv_of<S>(B);
+      T.b[l] = B.b.as<S>();
+      T.x[l] = B.x.as<S>();
+      T.xs[l] = B.xs.as<S>();
+    }
+    k_mg_tail<S><<<1, 1024, 0, c->stream>>>(T, lt, nl, (S)MG_ALPHA, sc);
   }
   // up: correction and post-smoothing
-  for (int l = nl - 2; l >= 1; l--) {
+  for (int l = lt - 1; l >= 1; l--) {
     post1(l);
     sweep(l, M.lv[l].xs, M.lv[l].x, false);
   }
@@ -573,6 +665,8 @@ int mg_solve_t(fs_ctx* c, fs_stats* st) {
   const double tol = c->cfg.cg_tolerance;
   const uint8_t* act = c->active.as<uint8_t>();
   FS_HIP(c, hipMemsetAsync(M.scal.p, 0, sizeof(PcgScalars), c->stream));
+  // s and t are written on active cells only
+  for (DBuf* d : {&M.s0, &M.s1, &M.t}) FS_HIP(c, hipMemsetAsync(d->p, 0, (size_t)n * sizeof(double), c->stream));
   k_mg_init<S><<<nb, 256, 0, c->stream>>>(n, act, c->rhs.as<S>(), M.x.as<double>(), M.r.as<double>(),
                                            M.partials.as<double>(), M.partials2.as<double>(), nc, sc, tmode, tol);
   FS_HIP(c, hipGetLastError());
