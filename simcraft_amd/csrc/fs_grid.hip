@@ -819,8 +819,11 @@ __device__ inline ReconCand<T> recon_candidates(int id, int n, const GridArgs<T>
     int bb = (int)floor(c.p[d] / ga.h[d]);  // ParticleNeighbourSearcher::update (ns-util.h:52-60)
     bb = bb < 0 ? 0 : bb;
     c.bin[d] = bb >= dims[d] ? dims[d] - 1 : bb;
-    c.lo[d] = max(0, (int)floor((c.p[d] - r2) / ga.h[d] - T(0.5)) - 1);
-    c.hi[d] = min(dims[d] - 1, (int)floor((c.p[d] + r2) / ga.h[d] - T(0.5)) + 1);
+    // cells whose centre (i + 1/2) h can lie within r2 of p: (p - r2)/h - 1/2 <= i <= (p + r2)/h - 1/2,
+    // widened by a hundredth of a cell so that rounding of these bounds never drops a cell the
+    // exact distance test below would keep (that test alone decides)
+    c.lo[d] = max(0, (int)floor((c.p[d] - r2) / ga.h[d] - T(0.5) - T(0.01)) + 1);
+    c.hi[d] = min(dims[d] - 1, (int)ceil((c.p[d] + r2) / ga.h[d] - T(0.5) + T(0.01)) - 1);
     if (skip) {
       c.lo[d] = INT_MAX / 4;
       c.hi[d] = -1;
@@ -831,30 +834,57 @@ __device__ inline ReconCand<T> recon_candidates(int id, int n, const GridArgs<T>
 
 // Visits the candidate cells (i, j, k) of one particle with their squared distance:
 // cell c keeps bin b on axis d when floor((x_c - 2r)/h) <= b <= ceil((x_c + 2r)/h) (clamped),
-// and |x_c - x_p| <= 2r as d^2 <= s_max.
+// and |x_c - x_p| <= 2r as d^2 <= s_max, with d^2 = (dx^2 + dy^2) + dz^2. The bin test is
+// evaluated once per row of the box; for each (i, j) the k range is narrowed to the cells whose
+// dz^2 can fit under s_max - (dx^2 + dy^2), widened by a hundredth of a cell so that rounding never
+// drops a cell the exact test keeps. The x rows are walked from a lane-dependent start, so
+// particles of one cell (neighbouring lanes, same box) do not update the same LDS minimum at the
+// same step. The set of visited cells and each distance do not depend on any of this.
 template <typename T, class F>
-__device__ inline void recon_visit(const ReconCand<T>& c, const GridArgs<T>& ga, T r2, T s_max, F&& fn) {
+__device__ inline void recon_visit(const ReconCand<T>& c, const GridArgs<T>& ga, T r2, T s_max, int rot, F&& fn) {
   const int dims[3] = {ga.nx, ga.ny, ga.nz};
+  const int ni = c.hi[0] - c.lo[0] + 1, nj = c.hi[1] - c.lo[1] + 1, nk = c.hi[2] - c.lo[2] + 1;
+  if (ni <= 0 || nj <= 0 || nk <= 0) return;
   auto in_range = [&](int d, T xc) {
     int mn = max(0, (int)floor((xc - r2) / ga.h[d]));
     int mx = min(dims[d] - 1, (int)ceil((xc + r2) / ga.h[d]));
     return c.bin[d] >= mn && c.bin[d] <= mx;
   };
-  for (int i = c.lo[0]; i <= c.hi[0]; i++) {
+  // bin test per row (bit o: cell lo + o); boxes wider than 32 cells (radii above ~15 cells)
+  // test every candidate instead
+  const bool rows = ni <= 32 && nj <= 32 && nk <= 32;
+  unsigned ok[3] = {0u, 0u, 0u};
+  const int nn[3] = {ni, nj, nk};
+  if (rows) {
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+      for (int o = 0; o < nn[d]; o++)
+        if (in_range(d, coord1(c.lo[d] + o, T(0.5), ga.h[d]))) ok[d] |= 1u << o;
+  }
+  auto pass = [&](int d, int o, T xc) { return rows ? ((ok[d] >> o) & 1u) != 0u : in_range(d, xc); };
+  const float ihz = 1.0f / (float)ga.h[2];
+  int oi = rot % ni;
+  for (int t = 0; t < ni; t++, oi = (oi + 1 == ni) ? 0 : oi + 1) {
+    const int i = c.lo[0] + oi;
     const T xc = coord1(i, T(0.5), ga.h[0]);
     const T dx = xc - c.p[0];
     const T dx2 = dx * dx;
-    if (dx2 > s_max || !in_range(0, xc)) continue;
+    if (dx2 > s_max || !pass(0, oi, xc)) continue;
     for (int j = c.lo[1]; j <= c.hi[1]; j++) {
       const T yc = coord1(j, T(0.5), ga.h[1]);
       const T dy = yc - c.p[1];
       const T dxy = dx2 + dy * dy;
-      if (dxy > s_max || !in_range(1, yc)) continue;
-      for (int k = c.lo[2]; k <= c.hi[2]; k++) {
+      if (dxy > s_max || !pass(1, j - c.lo[1], yc)) continue;
+      // bounds only (the margin absorbs the approximate square root and reciprocal)
+      const float R = __builtin_amdgcn_sqrtf((float)(s_max - dxy));
+      const float pz = (float)c.p[2] * ihz;
+      const int kl = max(c.lo[2], (int)floorf(pz - R * ihz - 0.51f) + 1);
+      const int kh = min(c.hi[2], (int)ceilf(pz + R * ihz - 0.49f) - 1);
+      for (int k = kl; k <= kh; k++) {
         const T zc = coord1(k, T(0.5), ga.h[2]);
         const T dz = zc - c.p[2];
         const T d2 = dxy + dz * dz;
-        if (d2 > s_max || !in_range(2, zc)) continue;
+        if (d2 > s_max || !pass(2, k - c.lo[2], zc)) continue;
         fn(i, j, k, d2);
       }
     }
@@ -882,11 +912,26 @@ __global__ __launch_bounds__(TPB) void k_recon_splat(int n, GridArgs<T> ga, cons
   const ReconCand<T> c = recon_candidates(id, n, ga, px, py, pz, keys, binmax, r2);
   if (threadIdx.x < 6) box[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : -1;
   __syncthreads();
+  // the workgroup's box: reduced within each wave first (a skipped particle's empty box, lo =
+  // INT_MAX / 4 and hi = -1, changes neither bound), then one LDS atomic per wave and bound
+  int blo[3], bhi[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    if (c.hi[d] >= 0) {
-      atomicMin(&box[d], c.lo[d]);
-      atomicMax(&box[3 + d], c.hi[d]);
+    blo[d] = c.lo[d];
+    bhi[d] = c.hi[d];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      blo[d] = min(blo[d], __shfl_xor(blo[d], off, 64));
+      bhi[d] = max(bhi[d], __shfl_xor(bhi[d], off, 64));
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      if (bhi[d] >= 0) {
+        atomicMin(&box[d], blo[d]);
+        atomicMax(&box[3 + d], bhi[d]);
+      }
     }
   }
   __syncthreads();
@@ -895,12 +940,12 @@ __global__ __launch_bounds__(TPB) void k_recon_splat(int n, GridArgs<T> ga, cons
   if (e0 <= 0 || e1 <= 0 || e2 <= 0) return;  // uniform: no particle of the workgroup is kept
   const long long vol = (long long)e0 * e1 * e2;
   if (vol > CAP) {  // uniform: lower the minima in memory directly
-    recon_visit(c, ga, r2, s_max, [&](int i, int j, int k, T d2) { lower_global(dmin + lin3(i, j, k, ga.ny, ga.nz), Bits<T>::of(d2)); });
+    recon_visit(c, ga, r2, s_max, threadIdx.x & 7, [&](int i, int j, int k, T d2) { lower_global(dmin + lin3(i, j, k, ga.ny, ga.nz), Bits<T>::of(d2)); });
     return;
   }
   for (int q = threadIdx.x; q < (int)vol; q += TPB) table[q] = Bits<T>::inf;
   __syncthreads();
-  recon_visit(c, ga, r2, s_max, [&](int i, int j, int k, T d2) {
+  recon_visit(c, ga, r2, s_max, threadIdx.x & 7, [&](int i, int j, int k, T d2) {
     atomicMin(&table[((i - L0) * e1 + (j - L1)) * e2 + (k - L2)], Bits<T>::of(d2));
   });
   __syncthreads();
