@@ -258,7 +258,7 @@ int fs_set_kernel_timing(fs_ctx* ctx, int32_t on);
  * Vector-Jacobian products of a rollout of fs_substep calls, for differentiable simulation.
  * The discrete structure of each substep (fluid SDF, marking, face weights, active set,
  * extrapolation masks, Poisson matrix) is held fixed; the transfers (including their kernel
- * weights' dependence on positions), collider, projection (through the same MIC-PCG on the
+ * weights' dependence on positions), collider, projection (through the same preconditioned CG on the
  * symmetric matrix), velocity extrapolation, gravity and advection are differentiated exactly.
  * Requires advect_on_cleared_grid = 1 (the reference order). Arrays are AoS xyz doubles. */
 int fs_rollout_begin(fs_ctx* ctx, int32_t checkpoint_every); /* record the following substeps */

@@ -449,7 +449,7 @@ int adj_step_t(fs_ctx* c, AdjStep& t, AdjWork& w, double grav[3]) {
     k_project_faces_adj<T><<<blocks(c->n_face[a]), TPB, 0, c->stream>>>(a, dims_of(c, a), t.weights[a].as<T>(),
                                                                        w.gb5[a].as<T>());
   FS_HIP(c, hipGetLastError());
-  // 5. lambda = A^-1 pbar with the same MIC-PCG (A is symmetric); the system of this step
+  // 5. lambda = A^-1 pbar with the forward's preconditioned CG (A is symmetric); the system of this step
   FS_TRY(copy_buf(c, c->active, t.active));
   FS_TRY(copy_buf(c, c->adiag, t.adiag));
   for (int q = 0; q < 6; q++) FS_TRY(copy_buf(c, c->an[q], t.an[q]));

@@ -194,13 +194,14 @@ def _rel(a, b):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind,advection", [("dam", 1), ("dam", 0), ("sphere", 1)],
-                         ids=["dam-flip", "dam-pic", "sphere-flip"])
-def test_device_adjoint_matches_cpu_adjoint(hip, kind, advection):
+@pytest.mark.parametrize("kind,advection,precond", [("dam", 1, 1), ("dam", 0, 1), ("sphere", 1, 1), ("dam", 1, 0)],
+                         ids=["dam-flip", "dam-pic", "sphere-flip", "dam-flip-multigrid"])
+def test_device_adjoint_matches_cpu_adjoint(hip, kind, advection, precond):
     """The device's reverse pass against the oracle's FP64 CPU reverse pass on the same recorded
     rollout (16^3, 3 substeps). The sphere scene exercises the collider's reverse, whose normal
-    is held fixed in both."""
-    announce(f"adjoint device vs cpu {kind} advection={advection}")
+    is held fixed in both; the last case runs forward and adjoint solves with the multigrid
+    preconditioner."""
+    announce(f"adjoint device vs cpu {kind} advection={advection} precond={precond}")
     if kind == "sphere":
         X, Y, Z = scenes.cell_centres(N)
         pos = scenes.particles_in_cells((X > 0.45) & (X < 0.95) & (Y < 0.55) & (Z > 0.25) & (Z < 0.75), 3, 1.0 / N)
@@ -211,7 +212,7 @@ def test_device_adjoint_matches_cpu_adjoint(hip, kind, advection):
     vel = np.random.default_rng(3).uniform(-0.3, 0.3, pos.shape)
     rng = np.random.default_rng(11)
     cx, cv = rng.standard_normal(pos.shape), rng.standard_normal(pos.shape)
-    kw = dict(nx=N, ny=N, nz=N, flip_alpha=0.95, advection=advection)
+    kw = dict(nx=N, ny=N, nz=N, flip_alpha=0.95, advection=advection, preconditioner=precond)
     dev, cpu = _paired_rollout(hip, kw, pos, vel, col, 3, DT, 2, lambda p, v: (cx, cv))
     errs = [_rel(d, c) for d, c in zip(dev, cpu)]
     print(f"[adjoint {kind}] device vs CPU rel L2: dpos0 {errs[0]:.2e} dvel0 {errs[1]:.2e} dG {errs[2]:.2e}")

@@ -35,17 +35,17 @@ def _scene(kind, seed=3):
     return pos, vel, col
 
 
-def _sim(oracle, pos, vel, col, alpha, gravity=9.8, advection=1):
+def _sim(oracle, pos, vel, col, alpha, gravity=9.8, advection=1, precond=1):
     cfg = oracle.default_config(nx=N, ny=N, nz=N, flip_alpha=alpha, gravity=gravity, advection=advection,
-                                cg_tolerance=1e-12, tol_mode=1, cg_max_iterations=1000)
+                                cg_tolerance=1e-12, tol_mode=1, cg_max_iterations=1000, preconditioner=precond)
     o = oracle.OracleSim(cfg, dot_mode=oracle.DOT_TREE)
     o.set_collider_sdf(col)
     o.set_particles(pos, vel)
     return o
 
 
-def _loss(oracle, pos, vel, col, cx, cv, alpha, advection, gravity=9.8):
-    o = _sim(oracle, pos, vel, col, alpha, gravity, advection)
+def _loss(oracle, pos, vel, col, cx, cv, alpha, advection, gravity=9.8, precond=1):
+    o = _sim(oracle, pos, vel, col, alpha, gravity, advection, precond)
     for _ in range(STEPS):
         o.substep(DT)
     p, v = o.particles()
@@ -53,13 +53,15 @@ def _loss(oracle, pos, vel, col, cx, cv, alpha, advection, gravity=9.8):
     return float(np.sum(cx * p) + np.sum(cv * v))
 
 
-@pytest.mark.parametrize("kind,alpha,advection", [("dam", 0.95, 1), ("dam", 0.0, 0), ("full", 0.95, 1)],
-                         ids=["dam-flip", "dam-pic", "full-flip"])
-def test_oracle_adjoint_matches_finite_differences(oracle, kind, alpha, advection):
+@pytest.mark.parametrize("kind,alpha,advection,precond",
+                         [("dam", 0.95, 1, 1), ("dam", 0.0, 0, 1), ("full", 0.95, 1, 1), ("dam", 0.95, 1, 0)],
+                         ids=["dam-flip", "dam-pic", "full-flip", "dam-flip-multigrid"])
+def test_oracle_adjoint_matches_finite_differences(oracle, kind, alpha, advection, precond):
+    """precond 1: MIC(0); 0: the multigrid preconditioner (forward and adjoint solves)."""
     pos, vel, col = _scene(kind)
     rng = np.random.default_rng(7)
     cx, cv = rng.standard_normal(pos.shape), rng.standard_normal(pos.shape)
-    o = _sim(oracle, pos, vel, col, alpha, advection=advection)
+    o = _sim(oracle, pos, vel, col, alpha, advection=advection, precond=precond)
     o.rollout_begin()
     for _ in range(STEPS):
         o.substep(DT)
@@ -73,13 +75,13 @@ def test_oracle_adjoint_matches_finite_differences(oracle, kind, alpha, advectio
             d *= (pos > 2e-6) & (pos < 1 - 2e-6)  # stay inside the domain
         args_p = (pos + eps * d, vel) if which == "pos" else (pos, vel + eps * d)
         args_m = (pos - eps * d, vel) if which == "pos" else (pos, vel - eps * d)
-        fd = (_loss(oracle, *args_p, col, cx, cv, alpha, advection) -
-              _loss(oracle, *args_m, col, cx, cv, alpha, advection)) / (2 * eps)
+        fd = (_loss(oracle, *args_p, col, cx, cv, alpha, advection, precond=precond) -
+              _loss(oracle, *args_m, col, cx, cv, alpha, advection, precond=precond)) / (2 * eps)
         an = float(np.sum(grad * d))
         assert an == pytest.approx(fd, rel=1e-5, abs=1e-7 * max(1.0, abs(fd))), (which, an, fd)
     if kind == "full":
         return  # in a closed full box the projection cancels gravity
     eps = 1e-6
-    fd = (_loss(oracle, pos, vel, col, cx, cv, alpha, advection, gravity=9.8 + eps) -
-          _loss(oracle, pos, vel, col, cx, cv, alpha, advection, gravity=9.8 - eps)) / (2 * eps)
+    fd = (_loss(oracle, pos, vel, col, cx, cv, alpha, advection, gravity=9.8 + eps, precond=precond) -
+          _loss(oracle, pos, vel, col, cx, cv, alpha, advection, gravity=9.8 - eps, precond=precond)) / (2 * eps)
     assert -dg[1] == pytest.approx(fd, rel=1e-5, abs=1e-8), (dg, fd)  # G = (0, -g, 0)
