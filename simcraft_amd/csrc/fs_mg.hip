@@ -21,8 +21,9 @@
 //
 // Layout: every level is a dense grid, k fastest, in the context's storage type S (FP32 in mixed
 // precision); the CG vectors x, r, s, t are FP64 in both modes, so the stopping test is always
-// made on an FP64 recurrence of the FP64 system. A block of 256 threads takes one chunk of 4096
-// consecutive cells (thread t: cells t + 256 m, m = 0..15); a cell with W = 0 (inactive) reads no
+// made on an FP64 recurrence of the FP64 system. Kernels that reduce (dot products, norms) give a
+// block of 256 threads one chunk of 4096 consecutive cells (thread t: cells t + 256 m, m = 0..15),
+// the fixed tree; the others take one cell per thread. A cell with W = 0 (inactive) reads no
 // neighbour. Sweeps that need the previous iterate of a neighbour recompute it from what is
 // stored (W b for the first sweep, x + alpha x_coarse after the coarse correction), so each
 // level costs four passes: two sweeps, restriction, correction with a sweep, a sweep. Dot
@@ -176,20 +177,18 @@ __device__ __forceinline__ S sweep_cell(const Lv<S>& L, const B* __restrict__ b,
   return xs[c] + L.W[c] * ((S)b[c] - row<S>(L, c, i, j, k, xo));
 }
 
+// The sweeps without a dot product take one cell per thread: the coarse levels are small, and
+// sixteen cells per thread would leave most of the chip idle while each thread waits on its
+// loads in turn.
 template <typename S, typename B>
 __global__ __launch_bounds__(256) void k_mg_pre(Lv<S> L, const B* __restrict__ b, S* __restrict__ x,
                                                 const PcgScalars* sc) {
   if (sc && sc->converged != STOP_RUN) return;
-  const int64_t n = (int64_t)L.nx * L.ny * L.nz;
-#pragma unroll 1
-  for (int m = 0; m < 16; m++) {
-    const int64_t c = (int64_t)blockIdx.x * MCH + m * 256 + threadIdx.x;
-    if (c >= n) break;
-    if (!(L.W[c] > S(0))) continue;
-    int i, j, k;
-    decode3(c, L.ny, L.nz, i, j, k);
-    x[c] = pre_cell<S, B>(L, b, c, i, j, k);
-  }
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= (int64_t)L.nx * L.ny * L.nz || !(L.W[c] > S(0))) return;
+  int i, j, k;
+  decode3(c, L.ny, L.nz, i, j, k);
+  x[c] = pre_cell<S, B>(L, b, c, i, j, k);
 }
 
 template <typename S, typename B>
@@ -209,20 +208,28 @@ __global__ __launch_bounds__(256) void k_mg_post1(Lv<S> L, const B* __restrict__
                                                   const S* __restrict__ xc, int cny, int cnz, S alpha,
                                                   S* __restrict__ xs, const PcgScalars* sc) {
   if (sc && sc->converged != STOP_RUN) return;
-  const int64_t n = (int64_t)L.nx * L.ny * L.nz;
-#pragma unroll 1
-  for (int m = 0; m < 16; m++) {
-    const int64_t c = (int64_t)blockIdx.x * MCH + m * 256 + threadIdx.x;
-    if (c >= n) break;
-    if (!(L.W[c] > S(0))) continue;
-    int i, j, k;
-    decode3(c, L.ny, L.nz, i, j, k);
-    xs[c] = post1_cell<S, B>(L, b, x, xc, cny, cnz, alpha, c, i, j, k);
-  }
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= (int64_t)L.nx * L.ny * L.nz || !(L.W[c] > S(0))) return;
+  int i, j, k;
+  decode3(c, L.ny, L.nz, i, j, k);
+  xs[c] = post1_cell<S, B>(L, b, x, xc, cny, cnz, alpha, c, i, j, k);
 }
 
-// One sweep from xs into x. DOT (level 0 of a CG cycle): z.r over the chunk, reduced by the last
-// block into sigma (FIRST: the initial sigma; else beta = sigma_new / sigma, sigma = sigma_new).
+// One sweep from xs into x.
+template <typename S, typename B>
+__global__ __launch_bounds__(256) void k_mg_sweep(Lv<S> L, const B* __restrict__ b, const S* __restrict__ xs,
+                                                  S* __restrict__ x, const PcgScalars* sc) {
+  if (sc && sc->converged != STOP_RUN) return;
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= (int64_t)L.nx * L.ny * L.nz || !(L.W[c] > S(0))) return;
+  int i, j, k;
+  decode3(c, L.ny, L.nz, i, j, k);
+  x[c] = sweep_cell<S, B>(L, b, xs, c, i, j, k);
+}
+
+// The sweep that ends a CG cycle on level 0, from xs into x = z, with z.r over the chunk, reduced by
+// the last block into sigma (FIRST: the initial sigma; else beta = sigma_new / sigma,
+// sigma = sigma_new). Chunks of 4096 cells per block: the fixed reduction tree.
 template <typename S, typename B, bool DOT, bool FIRST>
 __global__ __launch_bounds__(256) void k_mg_post2(Lv<S> L, const B* __restrict__ b, const S* __restrict__ xs,
                                                   S* __restrict__ x, double* __restrict__ part, int nchunks,
@@ -534,8 +541,8 @@ int mg_cycle(fs_ctx* c, const double* r0, PcgScalars* sc, bool first) {
   const double na = (double)c->n_active_last;
   auto pre = [&](int l) {
     MgLevelBuf& B = M.lv[l];
-    if (l == 0) k_mg_pre<S, double><<<chunks_of(B.n), 256, 0, c->stream>>>(lv_of<S>(B), r0, B.x.as<S>(), sc);
-    else k_mg_pre<S, S><<<chunks_of(B.n), 256, 0, c->stream>>>(lv_of<S>(B), B.b.as<S>(), B.x.as<S>(), sc);
+    if (l == 0) k_mg_pre<S, double><<<fs_blocks(B.n, 256), 256, 0, c->stream>>>(lv_of<S>(B), r0, B.x.as<S>(), sc);
+    else k_mg_pre<S, S><<<fs_blocks(B.n, 256), 256, 0, c->stream>>>(lv_of<S>(B), B.b.as<S>(), B.x.as<S>(), sc);
   };
   auto restrict_ = [&](int l) {
     MgLevelBuf &B = M.lv[l], &C = M.lv[l + 1];
@@ -549,11 +556,12 @@ int mg_cycle(fs_ctx* c, const double* r0, PcgScalars* sc, bool first) {
   auto post1 = [&](int l) {
     MgLevelBuf &B = M.lv[l], &C = M.lv[l + 1];
     if (l == 0)
-      k_mg_post1<S, double><<<chunks_of(B.n), 256, 0, c->stream>>>(lv_of<S>(B), r0, B.x.as<S>(), C.x.as<S>(), C.ny,
-                                                                    C.nz, (S)MG_ALPHA, B.xs.as<S>(), sc);
+      k_mg_post1<S, double><<<fs_blocks(B.n, 256), 256, 0, c->stream>>>(lv_of<S>(B), r0, B.x.as<S>(), C.x.as<S>(),
+                                                                         C.ny, C.nz, (S)MG_ALPHA, B.xs.as<S>(), sc);
     else
-      k_mg_post1<S, S><<<chunks_of(B.n), 256, 0, c->stream>>>(lv_of<S>(B), B.b.as<S>(), B.x.as<S>(), C.x.as<S>(),
-                                                               C.ny, C.nz, (S)MG_ALPHA, B.xs.as<S>(), sc);
+      k_mg_post1<S, S><<<fs_blocks(B.n, 256), 256, 0, c->stream>>>(lv_of<S>(B), B.b.as<S>(), B.x.as<S>(),
+                                                                    C.x.as<S>(), C.ny, C.nz, (S)MG_ALPHA, B.xs.as<S>(),
+                                                                    sc);
   };
   // one sweep from `from` into `to`; on level 0 of a CG cycle the sweep that ends the cycle
   // (into x) also forms sigma
@@ -561,11 +569,11 @@ int mg_cycle(fs_ctx* c, const double* r0, PcgScalars* sc, bool first) {
     MgLevelBuf& B = M.lv[l];
     const unsigned nb = chunks_of(B.n);
     const Lv<S> L = lv_of<S>(B);
+    const unsigned nc1 = fs_blocks(B.n, 256);
     if (l > 0) {
-      k_mg_post2<S, S, false, false><<<nb, 256, 0, c->stream>>>(L, B.b.as<S>(), from.as<S>(), to.as<S>(), nullptr, 0,
-                                                                 sc);
+      k_mg_sweep<S, S><<<nc1, 256, 0, c->stream>>>(L, B.b.as<S>(), from.as<S>(), to.as<S>(), sc);
     } else if (!last || !sc) {
-      k_mg_post2<S, double, false, false><<<nb, 256, 0, c->stream>>>(L, r0, from.as<S>(), to.as<S>(), nullptr, 0, sc);
+      k_mg_sweep<S, double><<<nc1, 256, 0, c->stream>>>(L, r0, from.as<S>(), to.as<S>(), sc);
     } else if (first) {
       k_mg_post2<S, double, true, true><<<nb, 256, 0, c->stream>>>(L, r0, from.as<S>(), to.as<S>(),
                                                                     M.partials.as<double>(), (int)nb, sc);
