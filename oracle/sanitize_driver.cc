@@ -29,6 +29,7 @@ int fso_rollout_begin(void* s);
 int fso_rollout_stop(void* s);
 int fso_rollout_vjp(void* s, const double* dpos, const double* dvel, double* dpos0, double* dvel0, double* dgrav);
 int fso_get_apic(void* s, double* c);
+int fso_mg_apply(void* s, const double* r, double* z);
 int fso_set_apic(void* s, const double* c);
 }
 
@@ -101,9 +102,17 @@ int main() {
     for (size_t e = 0; e < x.size(); e++) x[e] = std::sin(0.1 * (double)e);
     bad |= check(fso_spmv(s, x.data(), y.data()), "spmv");
     bad |= check(fso_mic_apply(s, x.data(), z.data()), "mic_apply");
+    bad |= check(fso_mg_apply(s, x.data(), z.data()), "mg_apply");
     bad |= check(fso_get_array(s, "fluid_sdf", y.data()), "get_array");
     fso_set_p2g_order(s, 2);
     bad |= check(fso_substep(s, 0.004, &st), "substep (parallel P2G)");
+    fso_destroy(s);
+    cfg.preconditioner = FS_PRECOND_MULTIGRID;  // the multigrid-preconditioned CG
+    s = fso_create(&cfg);
+    if (!s) return 1;
+    fso_set_dot_mode(s, 1);
+    bad |= check(fso_set_particles(s, pos.data(), vel.data(), np), "set_particles (multigrid)");
+    for (int step = 0; step < 2; step++) bad |= check(fso_substep(s, 0.004, &st), "substep (multigrid)");
     bad |= check(fso_get_particles(s, dp.data(), dv.data()), "get_particles");
     fso_destroy(s);
   }
