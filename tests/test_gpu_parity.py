@@ -59,6 +59,31 @@ def test_substeps_bitexact_32(hip, oracle, scene):
     assert_same(g, o, ["pg", "fluid_sdf", "active", "u", "v", "w"], "after 10 substeps")
 
 
+def test_two_contexts_interleaved_bitexact(hip, oracle):
+    """Two live contexts of different sizes, substeps interleaved, then a third created after the
+    first is destroyed: each keeps its own exchange entries (the sweeps' uncached buffers come
+    from a process-wide pool that is reused, never freed) and stays bit-identical to its oracle."""
+    announce("two contexts interleaved n=16, 24")
+    pairs = [make_pair(hip, oracle, n, "dam", flip_alpha=0.95) for n in (16, 24)]
+    for step in range(3):
+        for g, o in pairs:
+            sg, so = g.substep(0.004), o.substep(0.004)
+            assert sg["cg_iterations"] == so["cg_iterations"], (step, sg, so)
+            assert np.array_equal(g.get("pg"), o.get("pg")), step
+    for s in pairs[0]:
+        s.close()
+    g3, o3 = make_pair(hip, oracle, 20, "overlap", flip_alpha=0.95)
+    for step in range(2):
+        for g, o in (pairs[1], (g3, o3)):
+            sg, so = g.substep(0.004), o.substep(0.004)
+            assert sg["cg_iterations"] == so["cg_iterations"], (step, sg, so)
+            gp, _ = g.particles()
+            op, _ = o.particles()
+            assert np.array_equal(gp, op), step
+    for s in (*pairs[1], g3, o3):
+        s.close()
+
+
 def test_operators_spmv_mic_bitexact(hip, oracle):
     n, dt = 32, 0.004
     announce("spmv / mic / factor n=32 overlap")
