@@ -94,7 +94,7 @@ void release_all(fs_ctx* c) {
   for (DBuf* b : {&c->fsdf, &c->fsdfbuf, &c->sdfvalid, &c->sdfvalidbuf, &c->collider, &c->pg, &c->adiag, &c->rhs,
                   &c->active, &c->keys, &c->slots, &c->perm, &c->counts, &c->offs, &c->scan_tmp, &c->red,
                   &c->recon_tmp, &c->sorted[0], &c->sorted[1], &c->sorted[2], &c->sorted[3], &c->p2g_w,
-                  &c->p2g_tiles, &c->offs2, &c->p2g_w2, &c->p2g_tiles2, &c->rkeys, &c->rbinmax})
+                  &c->p2g_tiles, &c->offs2, &c->p2g_w2, &c->p2g_tiles2, &c->rkeys, &c->rbinmax, &c->rbins})
     fs_free(*b);
   for (int q = 0; q < 6; q++) fs_free(c->an[q]);
   fs_pcg_release(c);

@@ -166,6 +166,9 @@ struct fs_ctx {
   // reconstruction's own keys and per-bin last particle, so that the fluid-SDF stages (on side2)
   // can run beside P2G and the velocity stages (on the main stream) within a substep
   DBuf rkeys, rbinmax;
+  // reconstruction: per axis and bin b, the cells i whose bin range (ns-util.h forNeighbours)
+  // holds b, as one interval [imin, imax] (int2[3][max(nx, ny, nz)])
+  DBuf rbins;
   hipStream_t side2 = nullptr;
   hipEvent_t sdf_ev[2];  // fork (main), join (side2)
   bool sdf_ev_ok = false;

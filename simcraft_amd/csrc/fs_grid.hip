@@ -802,12 +802,46 @@ constexpr int RC_LDS_BYTES = 16384;
 template <typename T>
 struct ReconCand {  // one particle's candidate cells (empty box when it is skipped)
   int lo[3], hi[3], bin[3];
+  int2 in[3];  // per axis, the cells whose bin range holds this particle's bin
   T p[3];
 };
 
+// Does cell i's bin range on axis d hold bin b? Cell c keeps bin b when
+// floor((x_c - 2r)/h) <= b <= ceil((x_c + 2r)/h), clamped to the grid (ns-util.h:80-109).
+template <typename T>
+__device__ inline bool recon_in_range(int d, int i, int b, const GridArgs<T>& ga, T r2) {
+  const int dims[3] = {ga.nx, ga.ny, ga.nz};
+  const T xc = coord1(i, T(0.5), ga.h[d]);
+  const int mn = max(0, (int)floor((xc - r2) / ga.h[d]));
+  const int mx = min(dims[d] - 1, (int)ceil((xc + r2) / ga.h[d]));
+  return b >= mn && b <= mx;
+}
+
+// Both bounds of a cell's bin range are non-decreasing in the cell index (each step - centre,
+// shift, quotient, floor or ceiling, clamp - is monotone under rounding to nearest), so the cells
+// that hold bin b form one interval [imin, imax]. One thread per axis and bin finds it with the
+// test above (empty: imin > imax).
+template <typename T>
+__global__ void k_recon_bins(GridArgs<T> ga, T r2, int dmax, int2* __restrict__ tab) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * dmax) return;
+  const int d = t / dmax, b = t % dmax;
+  const int dims[3] = {ga.nx, ga.ny, ga.nz};
+  int lo = dims[d], hi = -1;
+  if (b < dims[d]) {
+    for (int i = 0; i < dims[d]; i++)
+      if (recon_in_range(d, i, b, ga, r2)) {
+        lo = min(lo, i);
+        hi = i;
+      }
+  }
+  tab[t] = make_int2(lo, hi);
+}
+
 template <typename T>
 __device__ inline ReconCand<T> recon_candidates(int id, int n, const GridArgs<T>& ga, const T* px, const T* py,
-                                                const T* pz, const int* keys, const int* binmax, T r2) {
+                                                const T* pz, const int* keys, const int* binmax, T r2,
+                                                const int2* bins, int dmax) {
   ReconCand<T> c;
   const bool skip = id >= n || binmax[keys[id]] == id;  // forNeighbours stops one short of the inclusive bin end
   const int dims[3] = {ga.nx, ga.ny, ga.nz};
@@ -819,11 +853,16 @@ __device__ inline ReconCand<T> recon_candidates(int id, int n, const GridArgs<T>
     int bb = (int)floor(c.p[d] / ga.h[d]);  // ParticleNeighbourSearcher::update (ns-util.h:52-60)
     bb = bb < 0 ? 0 : bb;
     c.bin[d] = bb >= dims[d] ? dims[d] - 1 : bb;
+    c.in[d] = bins[d * dmax + c.bin[d]];
     // cells whose centre (i + 1/2) h can lie within r2 of p: (p - r2)/h - 1/2 <= i <= (p + r2)/h - 1/2,
-    // widened by a hundredth of a cell so that rounding of these bounds never drops a cell the
-    // exact distance test below would keep (that test alone decides)
-    c.lo[d] = max(0, (int)floor((c.p[d] - r2) / ga.h[d] - T(0.5) - T(0.01)) + 1);
-    c.hi[d] = min(dims[d] - 1, (int)ceil((c.p[d] + r2) / ga.h[d] - T(0.5) + T(0.01)) - 1);
+    // widened by a hundredth of a cell so that rounding of these bounds (and the reciprocal in
+    // place of the quotient) never drops a cell the exact distance test below would keep (that
+    // test alone decides)
+    const T ih = T(1) / ga.h[d];
+    c.lo[d] = max(0, (int)floor((c.p[d] - r2) * ih - T(0.5) - T(0.01)) + 1);
+    c.hi[d] = min(dims[d] - 1, (int)ceil((c.p[d] + r2) * ih - T(0.5) + T(0.01)) - 1);
+    c.lo[d] = max(c.lo[d], c.in[d].x);  // and whose bin range holds the particle's bin
+    c.hi[d] = min(c.hi[d], c.in[d].y);
     if (skip) {
       c.lo[d] = INT_MAX / 4;
       c.hi[d] = -1;
@@ -832,59 +871,44 @@ __device__ inline ReconCand<T> recon_candidates(int id, int n, const GridArgs<T>
   return c;
 }
 
-// Visits the candidate cells (i, j, k) of one particle with their squared distance:
-// cell c keeps bin b on axis d when floor((x_c - 2r)/h) <= b <= ceil((x_c + 2r)/h) (clamped),
-// and |x_c - x_p| <= 2r as d^2 <= s_max, with d^2 = (dx^2 + dy^2) + dz^2. The bin test is
-// evaluated once per row of the box; for each (i, j) the k range is narrowed to the cells whose
-// dz^2 can fit under s_max - (dx^2 + dy^2), widened by a hundredth of a cell so that rounding never
-// drops a cell the exact test keeps. The x rows are walked from a lane-dependent start, so
-// particles of one cell (neighbouring lanes, same box) do not update the same LDS minimum at the
-// same step. The set of visited cells and each distance do not depend on any of this.
+// Visits the candidate cells (i, j, k) of one particle with their squared distance: cell c keeps
+// the particle's bin on every axis (the cell index lies in the particle's interval c.in) and
+// |x_c - x_p| <= 2r as d^2 <= s_max, with d^2 = (dx^2 + dy^2) + dz^2. For each (i, j) the k range
+// is narrowed to the cells whose dz^2 can fit under s_max - (dx^2 + dy^2), widened by a hundredth
+// of a cell so that rounding never drops a cell the exact test keeps. The x rows are walked from a
+// lane-dependent start, so particles of one cell (neighbouring lanes, same box) do not update the
+// same LDS minimum at the same step. The set of visited cells and each distance do not depend on
+// any of this.
 template <typename T, class F>
 __device__ inline void recon_visit(const ReconCand<T>& c, const GridArgs<T>& ga, T r2, T s_max, int rot, F&& fn) {
-  const int dims[3] = {ga.nx, ga.ny, ga.nz};
-  const int ni = c.hi[0] - c.lo[0] + 1, nj = c.hi[1] - c.lo[1] + 1, nk = c.hi[2] - c.lo[2] + 1;
-  if (ni <= 0 || nj <= 0 || nk <= 0) return;
-  auto in_range = [&](int d, T xc) {
-    int mn = max(0, (int)floor((xc - r2) / ga.h[d]));
-    int mx = min(dims[d] - 1, (int)ceil((xc + r2) / ga.h[d]));
-    return c.bin[d] >= mn && c.bin[d] <= mx;
-  };
-  // bin test per row (bit o: cell lo + o); boxes wider than 32 cells (radii above ~15 cells)
-  // test every candidate instead
-  const bool rows = ni <= 32 && nj <= 32 && nk <= 32;
-  unsigned ok[3] = {0u, 0u, 0u};
-  const int nn[3] = {ni, nj, nk};
-  if (rows) {
-#pragma unroll
-    for (int d = 0; d < 3; d++)
-      for (int o = 0; o < nn[d]; o++)
-        if (in_range(d, coord1(c.lo[d] + o, T(0.5), ga.h[d]))) ok[d] |= 1u << o;
-  }
-  auto pass = [&](int d, int o, T xc) { return rows ? ((ok[d] >> o) & 1u) != 0u : in_range(d, xc); };
+  const int ilo = max(c.lo[0], c.in[0].x), ihi = min(c.hi[0], c.in[0].y);
+  const int jlo = max(c.lo[1], c.in[1].x), jhi = min(c.hi[1], c.in[1].y);
+  const int klo = max(c.lo[2], c.in[2].x), khi = min(c.hi[2], c.in[2].y);
+  const int ni = ihi - ilo + 1;
+  if (ni <= 0 || jhi < jlo || khi < klo) return;
   const float ihz = 1.0f / (float)ga.h[2];
   int oi = rot % ni;
   for (int t = 0; t < ni; t++, oi = (oi + 1 == ni) ? 0 : oi + 1) {
-    const int i = c.lo[0] + oi;
+    const int i = ilo + oi;
     const T xc = coord1(i, T(0.5), ga.h[0]);
     const T dx = xc - c.p[0];
     const T dx2 = dx * dx;
-    if (dx2 > s_max || !pass(0, oi, xc)) continue;
-    for (int j = c.lo[1]; j <= c.hi[1]; j++) {
+    if (dx2 > s_max) continue;
+    for (int j = jlo; j <= jhi; j++) {
       const T yc = coord1(j, T(0.5), ga.h[1]);
       const T dy = yc - c.p[1];
       const T dxy = dx2 + dy * dy;
-      if (dxy > s_max || !pass(1, j - c.lo[1], yc)) continue;
+      if (dxy > s_max) continue;
       // bounds only (the margin absorbs the approximate square root and reciprocal)
       const float R = __builtin_amdgcn_sqrtf((float)(s_max - dxy));
       const float pz = (float)c.p[2] * ihz;
-      const int kl = max(c.lo[2], (int)floorf(pz - R * ihz - 0.51f) + 1);
-      const int kh = min(c.hi[2], (int)ceilf(pz + R * ihz - 0.49f) - 1);
+      const int kl = max(klo, (int)floorf(pz - R * ihz - 0.51f) + 1);
+      const int kh = min(khi, (int)ceilf(pz + R * ihz - 0.49f) - 1);
       for (int k = kl; k <= kh; k++) {
         const T zc = coord1(k, T(0.5), ga.h[2]);
         const T dz = zc - c.p[2];
         const T d2 = dxy + dz * dz;
-        if (d2 > s_max || !pass(2, k - c.lo[2], zc)) continue;
+        if (d2 > s_max) continue;
         fn(i, j, k, d2);
       }
     }
@@ -903,13 +927,14 @@ __device__ inline void lower_global(unsigned long long* a, unsigned long long v)
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_recon_splat(int n, GridArgs<T> ga, const T* __restrict__ px, const T* __restrict__ py,
                               const T* __restrict__ pz, const int* __restrict__ keys, const int* __restrict__ binmax,
-                              T r2, T s_max, typename Bits<T>::U* __restrict__ dmin) {
+                              T r2, T s_max, const int2* __restrict__ bins, int dmax,
+                              typename Bits<T>::U* __restrict__ dmin) {
   using UB = typename Bits<T>::U;
   constexpr int CAP = RC_LDS_BYTES / (int)sizeof(UB);
   __shared__ UB table[CAP];
   __shared__ int box[6];
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  const ReconCand<T> c = recon_candidates(id, n, ga, px, py, pz, keys, binmax, r2);
+  const ReconCand<T> c = recon_candidates(id, n, ga, px, py, pz, keys, binmax, r2, bins, dmax);
   if (threadIdx.x < 6) box[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : -1;
   __syncthreads();
   // the workgroup's box: reduced within each wave first (a skipped particle's empty box, lo =
@@ -1486,6 +1511,8 @@ int reconstruct_t(fs_ctx* c, double radius) {
   // own keys and per-bin last particle (rkeys, rbinmax), not the P2G binning buffers
   if (!c->rbinmax.p) FS_TRY(fs_alloc(c, c->rbinmax, (size_t)nb * sizeof(int)));
   if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)c->n_cell * sizeof(double)));
+  const int dmax = std::max(c->nx, std::max(c->ny, c->nz));
+  if (!c->rbins.p) FS_TRY(fs_alloc(c, c->rbins, (size_t)3 * dmax * sizeof(int2)));
   UB* dmin = c->recon_tmp.as<UB>();
   k_fill_bits<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->n_cell, dmin);
   T rad = (T)radius;
@@ -1499,9 +1526,11 @@ int reconstruct_t(fs_ctx* c, double radius) {
     k_keys_morton<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
                                                                c->pos[2].as<T>(), c->rkeys.as<int>());
     k_bin_last<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->rkeys.as<int>(), c->rbinmax.as<int>());
+    k_recon_bins<T><<<fs_blocks(3 * dmax, 64), 64, 0, c->stream>>>(ga, r2, dmax, c->rbins.as<int2>());
     k_recon_splat<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
                                                                c->pos[2].as<T>(), c->rkeys.as<int>(),
-                                                               c->rbinmax.as<int>(), r2, smax, dmin);
+                                                               c->rbinmax.as<int>(), r2, smax,
+                                                               c->rbins.as<int2>(), dmax, dmin);
   }
   k_recon_final<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->n_cell, dmin, rad, c->fsdf.as<T>(),
                                                                      c->sdfvalid.as<uint8_t>());
@@ -2034,6 +2063,8 @@ int adj_sdf_t(fs_ctx* c, AdjStep& t, DBuf& phibar, DBuf& psibar, DBuf* xbar) {
   const int nb = morton3D(c->nx - 1, c->ny - 1, c->nz - 1) + 1;
   FS_TRY(ensure_bins(c, (size_t)nb));
   if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)nc * sizeof(double)));
+  const int dmax = std::max(c->nx, std::max(c->ny, c->nz));
+  if (!c->rbins.p) FS_TRY(fs_alloc(c, c->rbins, (size_t)3 * dmax * sizeof(int2)));
   UB* dmin = c->recon_tmp.as<UB>();
   const T rad = (T)(c->cfg.recon_radius_factor * c->h[0]), r2 = T(2) * rad;
   const T smx = sizeof(T) == 8 ? (T)smax_for((double)r2) : (T)smax_for_f((float)r2);
@@ -2043,9 +2074,11 @@ int adj_sdf_t(fs_ctx* c, AdjStep& t, DBuf& phibar, DBuf& psibar, DBuf* xbar) {
     k_keys_morton<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, t.x1[0].as<T>(), t.x1[1].as<T>(),
                                                                t.x1[2].as<T>(), c->keys.as<int>());
     k_bin_last<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->counts.as<int>());
+    k_recon_bins<T><<<fs_blocks(3 * dmax, 64), 64, 0, c->stream>>>(ga, r2, dmax, c->rbins.as<int2>());
     k_recon_splat<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, t.x1[0].as<T>(), t.x1[1].as<T>(),
                                                                t.x1[2].as<T>(), c->keys.as<int>(),
-                                                               c->counts.as<int>(), r2, smx, dmin);
+                                                               c->counts.as<int>(), r2, smx, c->rbins.as<int2>(),
+                                                               dmax, dmin);
   }
   k_recon_final<T><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(nc, dmin, rad, tp.sA.as<T>(), tp.sV[0].as<uint8_t>());
   FS_HIP(c, hipMemcpyAsync(tp.sB.p, t.sdfbuf0.p, cb, hipMemcpyDeviceToDevice, c->stream));
