@@ -1337,20 +1337,23 @@ __global__ __launch_bounds__(256) void k_update(int64_t ne, double* __restrict__
                                                 const S* __restrict__ z, PcgScalars* sc, int iter) {
   typedef S S4 __attribute__((ext_vector_type(4)));
   typedef double D4 __attribute__((ext_vector_type(4)));
+  const int64_t e = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * 4;
+  if (e >= ne) return;
+  // the three streams' loads go out before the stop state is read (a chain of scalar loads);
+  // x is next read by the next iteration's update, so it bypasses the caches (non-temporal),
+  // leaving them to the sweeps' and the SpMV's streams
+  const S4 sv = *reinterpret_cast<const S4*>(s + e);
+  D4 xv = __builtin_nontemporal_load(reinterpret_cast<const D4*>(x + e));
+  const S4 zv = *reinterpret_cast<const S4*>(z + e);
   const int st = sc->converged;
   if (st == STOP_ABORT) return;
   if (st == STOP_CONVERGED && sc->conv_iter != iter) return;
   const double alpha = sc->alpha;
   const S beta = (S)sc->beta;
-  const int64_t e = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * 4;
-  if (e >= ne) return;
-  const S4 sv = *reinterpret_cast<const S4*>(s + e);
-  D4 xv = *reinterpret_cast<const D4*>(x + e);
 #pragma unroll
   for (int q = 0; q < 4; q++) xv[q] = xv[q] + alpha * (double)sv[q];
-  *reinterpret_cast<D4*>(x + e) = xv;
+  __builtin_nontemporal_store(xv, reinterpret_cast<D4*>(x + e));
   if (st == STOP_RUN) {
-    const S4 zv = *reinterpret_cast<const S4*>(z + e);
     S4 nv;
 #pragma unroll
     for (int q = 0; q < 4; q++) nv[q] = zv[q] + beta * sv[q];
