@@ -796,7 +796,11 @@ struct RunBwd {
   __device__ size_t operator()(int c, int S) const { return (size_t)(S - (c + 1) * U) * 64; }
 };
 
-// Wave 3: issue the loads of chunk c (past the end: the last chunk again, unused).
+// Wave 3: issue the loads of chunk c (past the end: the last chunk again, unused). The loads are
+// non-temporal: each element is read once per sweep, and keeping these streams out of the
+// caches leaves them to the data that is reused soon (the SpMV's neighbour rows, the vectors
+// handed from one kernel to the next): SpMV 45 -> 40 us, backward sweep 122 -> 118 us, substep
+// 112 -> 108 ms at 256^3 mixed.
 template <int NARR, typename V, typename RUN>
 __device__ __forceinline__ void sw_load(SwSet<V>& st, const V* const* src, int nc, int c, int lane, RUN run, int S) {
   const int cs = min(c, nc - 1);
@@ -805,8 +809,8 @@ __device__ __forceinline__ void sw_load(SwSet<V>& st, const V* const* src, int n
   for (int k = 0; k < NARR; k++) {
 #pragma unroll
     for (int h = 0; h < SwCfg<V>::OPS; h++)
-      st.d[k][h] = *reinterpret_cast<const Vec16<V>*>(src[k] + o + h * (1024 / (int)sizeof(V)) +
-                                                     lane * SwCfg<V>::PER16);
+      st.d[k][h] = __builtin_nontemporal_load(
+          reinterpret_cast<const Vec16<V>*>(src[k] + o + h * (1024 / (int)sizeof(V)) + lane * SwCfg<V>::PER16));
   }
 }
 
