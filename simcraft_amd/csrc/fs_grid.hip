@@ -1027,6 +1027,69 @@ __global__ __launch_bounds__(TPB) void k_extrap_sdf(int nx, int ny, int nz, cons
   }
 }
 
+// Four consecutive cells per thread (nz a multiple of 4): the flags of four cells come in one
+// 4-byte load, the values of each x / y neighbour row in one 16-byte (FP32) or 32-byte (FP64)
+// load; per cell the same terms in the same order as k_extrap_sdf, and the same writes (a valid
+// cell and a cell with no valid neighbour leave the value buffer as it was).
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_extrap_sdf4(int nx, int ny, int nz, const T* __restrict__ cur,
+                                                     T* __restrict__ buf, const uint8_t* __restrict__ val,
+                                                     uint8_t* __restrict__ vbuf) {
+  const int k0 = 4 * ((int)blockIdx.x * G3X + (int)threadIdx.x);
+  const int j = (int)blockIdx.y * G3Y + (int)threadIdx.y;
+  const int i = (int)blockIdx.z;
+  if (k0 >= nz || j >= ny) return;
+  const int id0 = (i * ny + j) * nz + k0;
+  const int sx = ny * nz, sy = nz;
+  auto flags = [&](int id) {
+    const uchar4 f = *reinterpret_cast<const uchar4*>(val + id);
+    return (unsigned)f.x | ((unsigned)f.y << 8) | ((unsigned)f.z << 16) | ((unsigned)f.w << 24);
+  };
+  auto values = [&](int id, T* v) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = cur[id + q];
+  };
+  const unsigned fc = flags(id0);
+  if ((fc & 0x01010101u) == 0x01010101u) {  // four valid cells: flags only
+    *reinterpret_cast<uchar4*>(vbuf + id0) = make_uchar4(1, 1, 1, 1);
+    return;
+  }
+  const unsigned fxm = i > 0 ? flags(id0 - sx) : 0u, fxp = i < nx - 1 ? flags(id0 + sx) : 0u;
+  const unsigned fym = j > 0 ? flags(id0 - sy) : 0u, fyp = j < ny - 1 ? flags(id0 + sy) : 0u;
+  const bool km = k0 > 0, kp = k0 + 4 < nz;
+  const uint8_t fkm = km ? val[id0 - 1] : 0, fkp = kp ? val[id0 + 4] : 0;
+  T vxm[4] = {}, vxp[4] = {}, vym[4] = {}, vyp[4] = {}, vc[4] = {};
+  if (fxm) values(id0 - sx, vxm);
+  if (fxp) values(id0 + sx, vxp);
+  if (fym) values(id0 - sy, vym);
+  if (fyp) values(id0 + sy, vyp);
+  if (fc) values(id0, vc);
+  const T ckm = fkm ? cur[id0 - 1] : T(0), ckp = fkp ? cur[id0 + 4] : T(0);
+  unsigned fo = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const unsigned bit = 0xFFu << (8 * q);
+    if (fc & bit) {
+      fo |= 1u << (8 * q);
+      continue;
+    }
+    T sum = T(0);
+    int count = 0;
+    if (fxm & bit) { sum += vxm[q]; count++; }
+    if (fxp & bit) { sum += vxp[q]; count++; }
+    if (fym & bit) { sum += vym[q]; count++; }
+    if (fyp & bit) { sum += vyp[q]; count++; }
+    if (q > 0 ? (fc & (bit >> 8)) != 0 : fkm != 0) { sum += q > 0 ? vc[q - 1] : ckm; count++; }
+    if (q < 3 ? (fc & (bit << 8)) != 0 : fkp != 0) { sum += q < 3 ? vc[q + 1] : ckp; count++; }
+    if (count > 0) {
+      buf[id0 + q] = sum / (T)count;
+      fo |= 1u << (8 * q);
+    }
+  }
+  *reinterpret_cast<uchar4*>(vbuf + id0) =
+      make_uchar4((uint8_t)(fo & 1u), (uint8_t)((fo >> 8) & 1u), (uint8_t)((fo >> 16) & 1u), (uint8_t)(fo >> 24));
+}
+
 // smoothFluidSurface iteration (FluidSim/src/cpu/fluid-simulator.cc:40-71): min(avg6, self).
 // FLAG also records whether the cell took the average (the reverse pass needs the branch).
 template <typename T, bool FLAG = false>
@@ -1048,6 +1111,47 @@ __global__ __launch_bounds__(TPB) void k_smooth(int nx, int ny, int nz, const T*
   if (FLAG) took_avg[id] = !(c0 < b);
   if (c0 < b) b = c0;
   buf[id] = b;
+}
+
+// k_smooth for four consecutive cells per thread (nz a multiple of 4), the same terms in the
+// same order per cell.
+template <typename T, bool FLAG = false>
+__global__ __launch_bounds__(TPB) void k_smooth4(int nx, int ny, int nz, const T* __restrict__ cur,
+                                                 T* __restrict__ buf, uint8_t* __restrict__ took_avg = nullptr) {
+  const int k0 = 4 * ((int)blockIdx.x * G3X + (int)threadIdx.x);
+  const int j = (int)blockIdx.y * G3Y + (int)threadIdx.y;
+  const int i = (int)blockIdx.z;
+  if (k0 >= nz || j >= ny) return;
+  const int id0 = (i * ny + j) * nz + k0;
+  const int sx = ny * nz, sy = nz;
+  T vxm[4] = {}, vxp[4] = {}, vym[4] = {}, vyp[4] = {}, vc[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    vc[q] = cur[id0 + q];
+    if (i > 0) vxm[q] = cur[id0 - sx + q];
+    if (i < nx - 1) vxp[q] = cur[id0 + sx + q];
+    if (j > 0) vym[q] = cur[id0 - sy + q];
+    if (j < ny - 1) vyp[q] = cur[id0 + sy + q];
+  }
+  const T ckm = k0 > 0 ? cur[id0 - 1] : T(0), ckp = k0 + 4 < nz ? cur[id0 + 4] : T(0);
+  uint8_t fl[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    T sum = T(0);
+    int count = 0;
+    if (i > 0) { sum += vxm[q]; count++; }
+    if (i < nx - 1) { sum += vxp[q]; count++; }
+    if (j > 0) { sum += vym[q]; count++; }
+    if (j < ny - 1) { sum += vyp[q]; count++; }
+    if (k0 + q > 0) { sum += q > 0 ? vc[q - 1] : ckm; count++; }
+    if (k0 + q < nz - 1) { sum += q < 3 ? vc[q + 1] : ckp; count++; }
+    T b = sum / (T)count;
+    const T c0 = vc[q];
+    fl[q] = !(c0 < b);
+    if (c0 < b) b = c0;
+    buf[id0 + q] = b;
+  }
+  if (FLAG) *reinterpret_cast<uchar4*>(took_avg + id0) = make_uchar4(fl[0], fl[1], fl[2], fl[3]);
 }
 
 // applyDirichletBoundary (fluid-simulator.cc:113-138) for face grid `axis`.
@@ -1542,9 +1646,14 @@ int reconstruct_t(fs_ctx* c, double radius) {
 template <typename T>
 int extrap_sdf_t(fs_ctx* c, int iters) {
   for (int it = 0; it < iters; it++) {
-    k_extrap_sdf<T><<<g3_grid(c->nx, c->ny, c->nz), g3_block(), 0, c->stream>>>(c->nx, c->ny, c->nz, c->fsdf.as<T>(),
-                                                                      c->fsdfbuf.as<T>(), c->sdfvalid.as<uint8_t>(),
-                                                                      c->sdfvalidbuf.as<uint8_t>());
+    if (c->nz % 4 == 0)
+      k_extrap_sdf4<T><<<g3_grid(c->nx, c->ny, c->nz / 4), g3_block(), 0, c->stream>>>(
+          c->nx, c->ny, c->nz, c->fsdf.as<T>(), c->fsdfbuf.as<T>(), c->sdfvalid.as<uint8_t>(),
+          c->sdfvalidbuf.as<uint8_t>());
+    else
+      k_extrap_sdf<T><<<g3_grid(c->nx, c->ny, c->nz), g3_block(), 0, c->stream>>>(
+          c->nx, c->ny, c->nz, c->fsdf.as<T>(), c->fsdfbuf.as<T>(), c->sdfvalid.as<uint8_t>(),
+          c->sdfvalidbuf.as<uint8_t>());
     std::swap(c->fsdf, c->fsdfbuf);
     std::swap(c->sdfvalid, c->sdfvalidbuf);
   }
@@ -1555,8 +1664,12 @@ int extrap_sdf_t(fs_ctx* c, int iters) {
 template <typename T>
 int smooth_t(fs_ctx* c, int iters) {
   for (int it = 0; it < iters; it++) {
-    k_smooth<T><<<g3_grid(c->nx, c->ny, c->nz), g3_block(), 0, c->stream>>>(c->nx, c->ny, c->nz, c->fsdf.as<T>(),
-                                                                  c->fsdfbuf.as<T>());
+    if (c->nz % 4 == 0)
+      k_smooth4<T><<<g3_grid(c->nx, c->ny, c->nz / 4), g3_block(), 0, c->stream>>>(c->nx, c->ny, c->nz,
+                                                                                   c->fsdf.as<T>(), c->fsdfbuf.as<T>());
+    else
+      k_smooth<T><<<g3_grid(c->nx, c->ny, c->nz), g3_block(), 0, c->stream>>>(c->nx, c->ny, c->nz, c->fsdf.as<T>(),
+                                                                              c->fsdfbuf.as<T>());
     std::swap(c->fsdf, c->fsdfbuf);
   }
   FS_HIP(c, hipGetLastError());
@@ -2085,13 +2198,21 @@ int adj_sdf_t(fs_ctx* c, AdjStep& t, DBuf& phibar, DBuf& psibar, DBuf* xbar) {
   // forward: extrapolation (values and masks), then smoothing with its branch flags
   DBuf *cur = &tp.sA, *buf = &tp.sB;
   for (int it = 0; it < ie; it++) {
-    k_extrap_sdf<T><<<g3_grid(c->nx, c->ny, c->nz), g3_block(), 0, c->stream>>>(c->nx, c->ny, c->nz, cur->as<T>(), buf->as<T>(),
-                                                               tp.sV[it].as<uint8_t>(), tp.sV[it + 1].as<uint8_t>());
+    if (c->nz % 4 == 0)
+      k_extrap_sdf4<T><<<g3_grid(c->nx, c->ny, c->nz / 4), g3_block(), 0, c->stream>>>(
+          c->nx, c->ny, c->nz, cur->as<T>(), buf->as<T>(), tp.sV[it].as<uint8_t>(), tp.sV[it + 1].as<uint8_t>());
+    else
+      k_extrap_sdf<T><<<g3_grid(c->nx, c->ny, c->nz), g3_block(), 0, c->stream>>>(
+          c->nx, c->ny, c->nz, cur->as<T>(), buf->as<T>(), tp.sV[it].as<uint8_t>(), tp.sV[it + 1].as<uint8_t>());
     std::swap(cur, buf);
   }
   for (int it = 0; it < is; it++) {
-    k_smooth<T, true><<<g3_grid(c->nx, c->ny, c->nz), g3_block(), 0, c->stream>>>(c->nx, c->ny, c->nz, cur->as<T>(), buf->as<T>(),
-                                                                tp.sF[it].as<uint8_t>());
+    if (c->nz % 4 == 0)
+      k_smooth4<T, true><<<g3_grid(c->nx, c->ny, c->nz / 4), g3_block(), 0, c->stream>>>(
+          c->nx, c->ny, c->nz, cur->as<T>(), buf->as<T>(), tp.sF[it].as<uint8_t>());
+    else
+      k_smooth<T, true><<<g3_grid(c->nx, c->ny, c->nz), g3_block(), 0, c->stream>>>(
+          c->nx, c->ny, c->nz, cur->as<T>(), buf->as<T>(), tp.sF[it].as<uint8_t>());
     std::swap(cur, buf);
   }
   // reverse: smoothing (the buffer the step leaves behind is the last smoothing input)
