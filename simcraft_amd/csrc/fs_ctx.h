@@ -75,6 +75,11 @@ struct PcgState {
   DBuf tickets;                  // uint64[3]: brick tickets of the factor, forward and backward sweeps
   uint64_t ticket_base[3] = {0, 0, 0};
   PcgScalars* host_scal = nullptr;  // pinned
+  // snapshots of the scalars after a batch of iterations (pinned, two slots) and their events:
+  // the solve queues the next batch before it waits for the previous batch's snapshot
+  PcgScalars* snap = nullptr;
+  hipEvent_t sev[2];
+  bool sev_ok = false;
   bool built = false;
   // mixed-precision copies
   DBuf f_adiag, f_al, f_ad, f_ab, f_e, f_cl, f_cd, f_cb, f_dr, f_du, f_df;

@@ -1490,6 +1490,9 @@ int pcg_build(fs_ctx* c) {
     FS_TRY(fs_alloc(c, P.scal, sizeof(PcgScalars)));
     FS_HIP(c, hipMemsetAsync(P.scal.p, 0, sizeof(PcgScalars), c->stream));
     FS_HIP(c, hipHostMalloc((void**)&P.host_scal, sizeof(PcgScalars), hipHostMallocDefault));
+    FS_HIP(c, hipHostMalloc((void**)&P.snap, 2 * sizeof(PcgScalars), hipHostMallocDefault));
+    for (int q = 0; q < 2; q++) FS_HIP(c, hipEventCreateWithFlags(&P.sev[q], hipEventDisableTiming));
+    P.sev_ok = true;
     FS_TRY(fs_alloc(c, P.tickets, 4 * sizeof(uint64_t)));
     FS_HIP(c, hipMemsetAsync(P.tickets.p, 0, 4 * sizeof(uint64_t), c->stream));
     for (int q = 0; q < 3; q++) P.ticket_base[q] = 0;
@@ -1732,7 +1735,7 @@ int solve_t(fs_ctx* c, fs_stats* st) {
       double last_norm = std::sqrt(hs->rnorm2sq);
       int it = 1;
       bool stopped = false;
-      while (it < maxit && !stopped) {
+      auto batch = [&]() -> int {  // chk iterations (fewer at the cap)
         for (int m = 0; m < chk && it < maxit; m++, it++) {
           int pf = fs_prof_begin(c, FS_KERNEL_SPMV, 6.0 * sizeof(S) * c->n_active_last);
           k_spmv<true, S><<<P.nchunks, 256, 0, c->stream>>>(L, y.s, y.adiag, y.al, y.ad, y.ab, P.nbm.as<uint8_t>(),
@@ -1747,36 +1750,83 @@ int solve_t(fs_ctx* c, fs_stats* st) {
           fs_prof_end(c, pf);
         }
         FS_HIP(c, hipGetLastError());
-        FS_TRY(read_scalars(c));
-        fs_prof_flush(c);
-        if (hs->wait_timeouts != 0) break;
-        if (MIXED && hs->converged != STOP_ABORT) {
-          const bool claimed = hs->converged == STOP_CONVERGED;
-          if (claimed || std::sqrt(hs->rnorm2sq) < 0.1 * last_norm) {
-            const int at = claimed ? hs->conv_iter : it - 1;
-            const int pf = fs_prof_begin(c, FS_KERNEL_PCG_UPDATE, (8.0 * 7 + 4.0) * c->n_active_last);
-            k_resid<<<P.nchunks, 256, 0, c->stream>>>(L, P.rhs.as<double>(), P.x.as<double>(), P.adiag.as<double>(),
-                                                      P.al.as<double>(), P.ad.as<double>(), P.ab.as<double>(),
-                                                      P.nbm.as<uint8_t>(), P.f_r.as<float>(),
-                                                      P.partials.as<double>(), P.partials2.as<double>(), P.nchunks,
-                                                      P.scal.as<PcgScalars>(), at, tmode, tol);
-            fs_prof_end(c, pf);
-            FS_HIP(c, hipGetLastError());
+        return FS_OK;
+      };
+      // The host checks the scalars after every batch. Unless kernels are timed one by one, the
+      // next batch is queued before the host waits for the previous batch's snapshot, so the
+      // device does not idle while the host reads and decides; once the device has stopped (a
+      // converged or aborted solve), the queued iterations return at once.
+      const bool ahead = !(c->cfg.profile_stages >= 2 && c->ktime_on);
+      auto snapshot = [&](int b) -> int {
+        FS_HIP(c, hipMemcpyAsync(&P.snap[b], P.scal.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, c->stream));
+        FS_HIP(c, hipEventRecord(P.sev[b], c->stream));
+        return FS_OK;
+      };
+      int pend = -1;  // snapshot slot of the queued batch checked next (-1: none queued)
+      while (!stopped) {
+        if (pend < 0) {  // nothing queued (first batch, after a synchronous step, or not ahead)
+          if (it >= maxit) break;
+          FS_TRY(batch());
+          if (ahead) {
+            FS_TRY(snapshot(0));
+            pend = 0;
+          } else {
             FS_TRY(read_scalars(c));
-            refinements++;
-            last_norm = std::sqrt(hs->rnorm2sq);
-            if (hs->converged == STOP_CONVERGED) {
-              stopped = true;
-            } else if (claimed) {
-              it = at + 1;
-              FS_TRY(restart());
-            }
-            continue;
+            fs_prof_flush(c);
           }
         }
-        if (hs->converged != 0) stopped = true;
+        if (ahead) {
+          const bool more = it < maxit;
+          if (more) {
+            FS_TRY(batch());
+            FS_TRY(snapshot(1 - pend));
+          }
+          FS_HIP(c, hipEventSynchronize(P.sev[pend]));
+          *hs = P.snap[pend];
+          pend = more ? 1 - pend : -1;
+        }
+        // hs: the state after the checked batch (one more batch may be queued behind it)
+        if (hs->wait_timeouts != 0) break;
+        if (MIXED && hs->converged != STOP_ABORT &&
+            (hs->converged == STOP_CONVERGED || std::sqrt(hs->rnorm2sq) < 0.1 * last_norm)) {
+          if (pend >= 0) {  // decide on the state after the queued batch as well
+            FS_TRY(read_scalars(c));
+            pend = -1;
+            if (hs->wait_timeouts != 0) break;
+            if (hs->converged == STOP_ABORT) {
+              stopped = true;
+              break;
+            }
+          }
+          const bool claimed = hs->converged == STOP_CONVERGED;
+          const int at = claimed ? hs->conv_iter : it - 1;
+          const int pf = fs_prof_begin(c, FS_KERNEL_PCG_UPDATE, (8.0 * 7 + 4.0) * c->n_active_last);
+          k_resid<<<P.nchunks, 256, 0, c->stream>>>(L, P.rhs.as<double>(), P.x.as<double>(), P.adiag.as<double>(),
+                                                    P.al.as<double>(), P.ad.as<double>(), P.ab.as<double>(),
+                                                    P.nbm.as<uint8_t>(), P.f_r.as<float>(),
+                                                    P.partials.as<double>(), P.partials2.as<double>(), P.nchunks,
+                                                    P.scal.as<PcgScalars>(), at, tmode, tol);
+          fs_prof_end(c, pf);
+          FS_HIP(c, hipGetLastError());
+          FS_TRY(read_scalars(c));
+          refinements++;
+          last_norm = std::sqrt(hs->rnorm2sq);
+          if (hs->converged == STOP_CONVERGED) {
+            stopped = true;
+          } else if (claimed) {
+            it = at + 1;
+            FS_TRY(restart());
+          }
+          continue;
+        }
+        if (hs->converged != 0) {
+          stopped = true;
+          break;
+        }
+        if (ahead && pend < 0) break;  // the cap: the last batch has been checked
       }
       if (hs->wait_timeouts != 0) {
+        (void)hipStreamSynchronize(c->stream);  // a queued batch gives up too
         c->set_error("PCG sweep: a brick dependency wait exceeded its bound");
         return FS_DEVICE_ERROR;
       }
@@ -1895,4 +1945,9 @@ void fs_pcg_release(fs_ctx* c) {
   xbuf_put(c, P.xb);
   if (P.host_scal) (void)hipHostFree(P.host_scal);
   P.host_scal = nullptr;
+  if (P.snap) (void)hipHostFree(P.snap);
+  P.snap = nullptr;
+  if (P.sev_ok)
+    for (int q = 0; q < 2; q++) (void)hipEventDestroy(P.sev[q]);
+  P.sev_ok = false;
 }
