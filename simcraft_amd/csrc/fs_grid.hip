@@ -806,34 +806,46 @@ struct ReconCand {  // one particle's candidate cells (empty box when it is skip
   T p[3];
 };
 
-// Does cell i's bin range on axis d hold bin b? Cell c keeps bin b when
+// Cell i's bin range [mn, mx] on axis d: cell c keeps bin b when
 // floor((x_c - 2r)/h) <= b <= ceil((x_c + 2r)/h), clamped to the grid (ns-util.h:80-109).
 template <typename T>
-__device__ inline bool recon_in_range(int d, int i, int b, const GridArgs<T>& ga, T r2) {
+__global__ void k_recon_bounds(GridArgs<T> ga, T r2, int dmax, int2* __restrict__ bnd) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * dmax) return;
+  const int d = t / dmax, i = t % dmax;
   const int dims[3] = {ga.nx, ga.ny, ga.nz};
+  if (i >= dims[d]) return;
   const T xc = coord1(i, T(0.5), ga.h[d]);
-  const int mn = max(0, (int)floor((xc - r2) / ga.h[d]));
-  const int mx = min(dims[d] - 1, (int)ceil((xc + r2) / ga.h[d]));
-  return b >= mn && b <= mx;
+  bnd[t] = make_int2(max(0, (int)floor((xc - r2) / ga.h[d])), min(dims[d] - 1, (int)ceil((xc + r2) / ga.h[d])));
 }
 
 // Both bounds of a cell's bin range are non-decreasing in the cell index (each step - centre,
 // shift, quotient, floor or ceiling, clamp - is monotone under rounding to nearest), so the cells
-// that hold bin b form one interval [imin, imax]. One thread per axis and bin finds it with the
-// test above (empty: imin > imax).
-template <typename T>
-__global__ void k_recon_bins(GridArgs<T> ga, T r2, int dmax, int2* __restrict__ tab) {
+// that hold bin b form one interval [imin, imax]: imin is the first cell with mx >= b, imax the
+// last with mn <= b, both found by bisection over the bounds (empty: imin > imax).
+__global__ void k_recon_bins(int nx, int ny, int nz, int dmax, const int2* __restrict__ bnd, int2* __restrict__ tab) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 3 * dmax) return;
   const int d = t / dmax, b = t % dmax;
-  const int dims[3] = {ga.nx, ga.ny, ga.nz};
-  int lo = dims[d], hi = -1;
-  if (b < dims[d]) {
-    for (int i = 0; i < dims[d]; i++)
-      if (recon_in_range(d, i, b, ga, r2)) {
-        lo = min(lo, i);
-        hi = i;
-      }
+  const int n = d == 0 ? nx : (d == 1 ? ny : nz);
+  const int2* bd = bnd + d * dmax;
+  int lo = n, hi = -1;
+  if (b < n) {
+    int l = 0, r = n;  // first i in [0, n) with mx(i) >= b
+    while (l < r) {
+      const int m = (l + r) >> 1;
+      if (bd[m].y >= b) r = m;
+      else l = m + 1;
+    }
+    lo = l;
+    l = 0;
+    r = n;  // first i with mn(i) > b, minus one
+    while (l < r) {
+      const int m = (l + r) >> 1;
+      if (bd[m].x > b) r = m;
+      else l = m + 1;
+    }
+    hi = l - 1;
   }
   tab[t] = make_int2(lo, hi);
 }
@@ -1124,15 +1136,16 @@ __global__ __launch_bounds__(TPB) void k_smooth4(int nx, int ny, int nz, const T
   if (k0 >= nz || j >= ny) return;
   const int id0 = (i * ny + j) * nz + k0;
   const int sx = ny * nz, sy = nz;
-  T vxm[4] = {}, vxp[4] = {}, vym[4] = {}, vyp[4] = {}, vc[4];
+  auto values = [&](int id, T* v) {
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    vc[q] = cur[id0 + q];
-    if (i > 0) vxm[q] = cur[id0 - sx + q];
-    if (i < nx - 1) vxp[q] = cur[id0 + sx + q];
-    if (j > 0) vym[q] = cur[id0 - sy + q];
-    if (j < ny - 1) vyp[q] = cur[id0 + sy + q];
-  }
+    for (int q = 0; q < 4; q++) v[q] = cur[id + q];
+  };
+  T vxm[4] = {}, vxp[4] = {}, vym[4] = {}, vyp[4] = {}, vc[4];
+  values(id0, vc);
+  if (i > 0) values(id0 - sx, vxm);
+  if (i < nx - 1) values(id0 + sx, vxp);
+  if (j > 0) values(id0 - sy, vym);
+  if (j < ny - 1) values(id0 + sy, vyp);
   const T ckm = k0 > 0 ? cur[id0 - 1] : T(0), ckp = k0 + 4 < nz ? cur[id0 + 4] : T(0);
   uint8_t fl[4];
 #pragma unroll
@@ -1616,7 +1629,7 @@ int reconstruct_t(fs_ctx* c, double radius) {
   if (!c->rbinmax.p) FS_TRY(fs_alloc(c, c->rbinmax, (size_t)nb * sizeof(int)));
   if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)c->n_cell * sizeof(double)));
   const int dmax = std::max(c->nx, std::max(c->ny, c->nz));
-  if (!c->rbins.p) FS_TRY(fs_alloc(c, c->rbins, (size_t)3 * dmax * sizeof(int2)));
+  if (!c->rbins.p) FS_TRY(fs_alloc(c, c->rbins, (size_t)6 * dmax * sizeof(int2)));  // bounds, then intervals
   UB* dmin = c->recon_tmp.as<UB>();
   k_fill_bits<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->n_cell, dmin);
   T rad = (T)radius;
@@ -1630,11 +1643,13 @@ int reconstruct_t(fs_ctx* c, double radius) {
     k_keys_morton<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
                                                                c->pos[2].as<T>(), c->rkeys.as<int>());
     k_bin_last<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->rkeys.as<int>(), c->rbinmax.as<int>());
-    k_recon_bins<T><<<fs_blocks(3 * dmax, 64), 64, 0, c->stream>>>(ga, r2, dmax, c->rbins.as<int2>());
+    k_recon_bounds<T><<<fs_blocks(3 * dmax, 64), 64, 0, c->stream>>>(ga, r2, dmax, c->rbins.as<int2>());
+    k_recon_bins<<<fs_blocks(3 * dmax, 64), 64, 0, c->stream>>>(c->nx, c->ny, c->nz, dmax, c->rbins.as<int2>(),
+                                                                c->rbins.as<int2>() + 3 * dmax);
     k_recon_splat<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, c->pos[0].as<T>(), c->pos[1].as<T>(),
                                                                c->pos[2].as<T>(), c->rkeys.as<int>(),
                                                                c->rbinmax.as<int>(), r2, smax,
-                                                               c->rbins.as<int2>(), dmax, dmin);
+                                                               c->rbins.as<int2>() + 3 * dmax, dmax, dmin);
   }
   k_recon_final<T><<<fs_blocks(c->n_cell, TPB), TPB, 0, c->stream>>>(c->n_cell, dmin, rad, c->fsdf.as<T>(),
                                                                      c->sdfvalid.as<uint8_t>());
@@ -2177,7 +2192,7 @@ int adj_sdf_t(fs_ctx* c, AdjStep& t, DBuf& phibar, DBuf& psibar, DBuf* xbar) {
   FS_TRY(ensure_bins(c, (size_t)nb));
   if (!c->recon_tmp.p) FS_TRY(fs_alloc(c, c->recon_tmp, (size_t)nc * sizeof(double)));
   const int dmax = std::max(c->nx, std::max(c->ny, c->nz));
-  if (!c->rbins.p) FS_TRY(fs_alloc(c, c->rbins, (size_t)3 * dmax * sizeof(int2)));
+  if (!c->rbins.p) FS_TRY(fs_alloc(c, c->rbins, (size_t)6 * dmax * sizeof(int2)));
   UB* dmin = c->recon_tmp.as<UB>();
   const T rad = (T)(c->cfg.recon_radius_factor * c->h[0]), r2 = T(2) * rad;
   const T smx = sizeof(T) == 8 ? (T)smax_for((double)r2) : (T)smax_for_f((float)r2);
@@ -2187,11 +2202,13 @@ int adj_sdf_t(fs_ctx* c, AdjStep& t, DBuf& phibar, DBuf& psibar, DBuf* xbar) {
     k_keys_morton<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, t.x1[0].as<T>(), t.x1[1].as<T>(),
                                                                t.x1[2].as<T>(), c->keys.as<int>());
     k_bin_last<<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, c->keys.as<int>(), c->counts.as<int>());
-    k_recon_bins<T><<<fs_blocks(3 * dmax, 64), 64, 0, c->stream>>>(ga, r2, dmax, c->rbins.as<int2>());
+    k_recon_bounds<T><<<fs_blocks(3 * dmax, 64), 64, 0, c->stream>>>(ga, r2, dmax, c->rbins.as<int2>());
+    k_recon_bins<<<fs_blocks(3 * dmax, 64), 64, 0, c->stream>>>(c->nx, c->ny, c->nz, dmax, c->rbins.as<int2>(),
+                                                                c->rbins.as<int2>() + 3 * dmax);
     k_recon_splat<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, t.x1[0].as<T>(), t.x1[1].as<T>(),
                                                                t.x1[2].as<T>(), c->keys.as<int>(),
-                                                               c->counts.as<int>(), r2, smx, c->rbins.as<int2>(),
-                                                               dmax, dmin);
+                                                               c->counts.as<int>(), r2, smx,
+                                                               c->rbins.as<int2>() + 3 * dmax, dmax, dmin);
   }
   k_recon_final<T><<<fs_blocks(nc, TPB), TPB, 0, c->stream>>>(nc, dmin, rad, tp.sA.as<T>(), tp.sV[0].as<uint8_t>());
   FS_HIP(c, hipMemcpyAsync(tp.sB.p, t.sdfbuf0.p, cb, hipMemcpyDeviceToDevice, c->stream));
