@@ -38,6 +38,16 @@ def test_library_exports_every_declared_symbol():
         assert exported == declared_functions()
 
 
+def test_manual_build_line_lists_every_source():
+    """The hand-written hipcc line in INTEGRATION.md builds the same library as build.py."""
+    from simcraft_amd import build
+
+    text = (ROOT / "INTEGRATION.md").read_text()
+    start = text.index("```sh")
+    block = text[start:text.index("```", start + 5)]
+    assert [s for s in build.SOURCES if f"simcraft_amd/csrc/{s}" not in block] == []
+
+
 def test_struct_layout_matches_c(tmp_path):
     gcc = shutil.which("gcc")
     if gcc is None:
