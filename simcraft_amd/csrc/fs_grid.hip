@@ -193,20 +193,37 @@ __device__ inline void handle_collision(const GridArgs<T>& ga, const T* __restri
 }
 
 // FlipAdvectionSolver3D::advect (advect-solver.cc:170-187): Ralston RK3 + collision.
+// *flag = 1 if any face velocity of the three grids is nonzero or NaN (+0 and -0 count as zero).
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_any_nonzero(int64_t n0, int64_t n1, int64_t n2, const T* __restrict__ g0,
+                                                     const T* __restrict__ g1, const T* __restrict__ g2,
+                                                     int* __restrict__ flag) {
+  bool nz = false;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n0 + n1 + n2; e += stride) {
+    const T v = e < n0 ? g0[e] : (e < n0 + n1 ? g1[e - n0] : g2[e - n0 - n1]);
+    nz |= !(v == T(0));
+  }
+  if (__builtin_amdgcn_ballot_w64(nz) != 0 && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_advect(int n, GridArgs<T> ga, Grids3<T> G, const T* __restrict__ col, T* __restrict__ px,
                          T* __restrict__ py, T* __restrict__ pz, T* __restrict__ vx, T* __restrict__ vy,
-                         T* __restrict__ vz, T dt) {
+                         T* __restrict__ vz, T dt, const int* __restrict__ nonzero) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // a grid that holds only zeros (the reference's order clears it just before advecting) samples
+  // to exactly +0 at every point: the stage velocities are then taken as +0 without sampling
+  const bool sample = *nonzero != 0;
   V3<T> p = {px[i], py[i], pz[i]};
   V3<T> k1 = {vx[i], vy[i], vz[i]};
   T c1 = T(0.5) * dt;
   V3<T> p1 = {p.x + c1 * k1.x, p.y + c1 * k1.y, p.z + c1 * k1.z};
-  V3<T> k2 = sample_velocity(ga, G, p1.x, p1.y, p1.z);
+  V3<T> k2 = sample ? sample_velocity(ga, G, p1.x, p1.y, p1.z) : V3<T>{T(0), T(0), T(0)};
   T c2 = T(0.75) * dt;
   V3<T> p2 = {p.x + c2 * k2.x, p.y + c2 * k2.y, p.z + c2 * k2.z};
-  V3<T> k3 = sample_velocity(ga, G, p2.x, p2.y, p2.z);
+  V3<T> k3 = sample ? sample_velocity(ga, G, p2.x, p2.y, p2.z) : V3<T>{T(0), T(0), T(0)};
   T a1 = T(2.0 / 9.0) * dt, a2 = T(1.0 / 3.0) * dt, a3 = T(4.0 / 9.0) * dt;
   p.x = p.x + a1 * k1.x + a2 * k2.x + a3 * k3.x;
   p.y = p.y + a1 * k1.y + a2 * k2.y + a3 * k3.y;
@@ -1611,9 +1628,14 @@ int advect_t(fs_ctx* c, double dt) {
   GridArgs<T> ga = grid_args<T>(c);
   const int pf = fs_prof_begin(c, FS_KERNEL_ADVECT,
                                (double)n * 12 * sizeof(T) + (double)(c->n_face[0] + c->n_face[1] + c->n_face[2]) * sizeof(T));
+  int* nonzero = reinterpret_cast<int*>(c->red.as<char>() + 16);
+  FS_HIP(c, hipMemsetAsync(nonzero, 0, sizeof(int), c->stream));
+  k_any_nonzero<T><<<1024, TPB, 0, c->stream>>>(c->n_face[0], c->n_face[1], c->n_face[2], c->g[0].as<T>(),
+                                                  c->g[1].as<T>(), c->g[2].as<T>(), nonzero);
   k_advect<T><<<fs_blocks(n, TPB), TPB, 0, c->stream>>>(n, ga, grids_of<T>(c, c->g), c->collider.as<T>(),
                                                         c->pos[0].as<T>(), c->pos[1].as<T>(), c->pos[2].as<T>(),
-                                                        c->vel[0].as<T>(), c->vel[1].as<T>(), c->vel[2].as<T>(), (T)dt);
+                                                        c->vel[0].as<T>(), c->vel[1].as<T>(), c->vel[2].as<T>(), (T)dt,
+                                                        nonzero);
   fs_prof_end(c, pf);
   FS_HIP(c, hipGetLastError());
   return FS_OK;

@@ -59,6 +59,25 @@ def test_substeps_bitexact_32(hip, oracle, scene):
     assert_same(g, o, ["pg", "fluid_sdf", "active", "u", "v", "w"], "after 10 substeps")
 
 
+@pytest.mark.parametrize("cleared", [1, 0])
+def test_advect_grid_zero_and_live_bitexact(hip, oracle, cleared):
+    """Advection on the cleared grid (the reference's order: every RK3 stage velocity samples to +0,
+    which the device detects with a scan of the grids and takes without sampling) and on the live
+    grid of the previous substep (advect_on_cleared_grid = 0: the stages sample): both bit for bit
+    the oracle, which always samples."""
+    n, dt = 24, 0.004
+    announce(f"advect cleared={cleared} n={n}")
+    g, o = make_pair(hip, oracle, n, "overlap", flip_alpha=0.95, advect_on_cleared_grid=cleared)
+    for step in range(4):
+        sg, so = g.substep(dt), o.substep(dt)
+        assert sg["cg_iterations"] == so["cg_iterations"], (step, sg, so)
+        gp, gv = g.particles()
+        op, ov = o.particles()
+        assert np.array_equal(gp, op) and np.array_equal(gv, ov), (cleared, step)
+    g.close()
+    o.close()
+
+
 def test_two_contexts_interleaved_bitexact(hip, oracle):
     """Two live contexts of different sizes, substeps interleaved, then a third created after the
     first is destroyed: each keeps its own exchange entries (the sweeps' uncached buffers come
