@@ -1306,6 +1306,98 @@ __global__ __launch_bounds__(TPB) void k_extrap_vel4(int gx, int gy, int gz, con
       make_uchar4((uint8_t)(fo & 1u), (uint8_t)((fo >> 8) & 1u), (uint8_t)((fo >> 16) & 1u), (uint8_t)(fo >> 24));
 }
 
+// The same iteration for face grids whose z extent is not a multiple of 4 (the w grid, nz + 1
+// faces per row), over the flat array: a thread takes four consecutive faces e0 .. e0+3, which
+// may run from the end of one row into the next. Needs gz >= 4 and gy*gz a multiple of 4, so the
+// flags of a group and of its x neighbours are aligned 4-byte loads and a group never spans two
+// x-slabs. The y neighbours' flags are 4 bytes at e0 -/+ gz, read as two aligned words shifted
+// into place; each face masks every neighbour that is not on the grid (row and slab ends), so it
+// sums the same neighbours in the same order as the one-face kernel and writes what it writes.
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_extrap_vel4f(int gx, int gy, int gz, const T* __restrict__ g,
+                                                      T* __restrict__ gbuf, const uint8_t* __restrict__ val,
+                                                      uint8_t* __restrict__ vbuf) {
+  const int n = gx * gy * gz, sx = gy * gz, sy = gz;
+  const int e0 = 4 * (int)(blockIdx.x * TPB + threadIdx.x);
+  if (e0 >= n) return;
+  auto word = [&](int p) { return *reinterpret_cast<const unsigned*>(val + p); };
+  // flag bytes p .. p+3 (n is a multiple of 4: both words lie below n when p + 4 <= n)
+  auto flags = [&](int p) -> unsigned {
+    if (p >= 0 && p + 4 <= n) {
+      const int base = p & ~3, sh = p & 3;
+      const unsigned w0 = word(base);
+      if (sh == 0) return w0;
+      const unsigned long long w = ((unsigned long long)word(base + 4) << 32) | w0;
+      return (unsigned)(w >> (8 * sh));
+    }
+    unsigned f = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (p + q >= 0 && p + q < n) f |= (unsigned)val[p + q] << (8 * q);
+    return f;
+  };
+  const unsigned fc = word(e0);
+  if ((fc & 0x01010101u) == 0x01010101u) {  // four valid faces: flags only
+    *reinterpret_cast<unsigned*>(vbuf + e0) = 0x01010101u;
+    return;
+  }
+  // row of face e0 (j, i) and of the row after it, which faces past the row's end belong to
+  const int r0 = e0 / gz, k0 = e0 - r0 * gz;
+  const int ja = r0 % gy, ia = r0 / gy;  // a group stays in slab ia
+  const int jb = ja + 1 < gy ? ja + 1 : 0;
+  const unsigned fxm = ia > 0 ? word(e0 - sx) : 0u, fxp = ia < gx - 1 ? word(e0 + sx) : 0u;
+  const unsigned fym = flags(e0 - sy), fyp = flags(e0 + sy);
+  const uint8_t fkm = k0 > 0 ? val[e0 - 1] : 0;
+  const uint8_t fkp = e0 + 4 < n ? val[e0 + 4] : 0;
+  bool hxm[4], hxp[4], hym[4], hyp[4], hkm[4], hkp[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const unsigned bit = 0xFFu << (8 * q);
+    const bool wrap = k0 + q >= gz;
+    const int k = wrap ? k0 + q - gz : k0 + q, j = wrap ? jb : ja;
+    hxm[q] = (fxm & bit) != 0;
+    hxp[q] = (fxp & bit) != 0;
+    hym[q] = j > 0 && (fym & bit) != 0;
+    hyp[q] = j < gy - 1 && (fyp & bit) != 0;
+    hkm[q] = k > 0 && (q > 0 ? (fc & (bit >> 8)) != 0 : fkm != 0);
+    hkp[q] = k < gz - 1 && (q < 3 ? (fc & (bit << 8)) != 0 : fkp != 0);
+  }
+  T vxm[4], vxp[4], vym[4], vyp[4], vkm[4], vkp[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int e = e0 + q;
+    vxm[q] = hxm[q] ? g[e - sx] : T(0);
+    vxp[q] = hxp[q] ? g[e + sx] : T(0);
+    vym[q] = hym[q] ? g[e - sy] : T(0);
+    vyp[q] = hyp[q] ? g[e + sy] : T(0);
+    vkm[q] = hkm[q] ? g[e - 1] : T(0);
+    vkp[q] = hkp[q] ? g[e + 1] : T(0);
+  }
+  unsigned fo = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    if (fc & (0xFFu << (8 * q))) {
+      fo |= 1u << (8 * q);
+      continue;
+    }
+    T sum = T(0);
+    int count = 0;
+    if (hxm[q]) { sum += vxm[q]; count++; }
+    if (hxp[q]) { sum += vxp[q]; count++; }
+    if (hym[q]) { sum += vym[q]; count++; }
+    if (hyp[q]) { sum += vyp[q]; count++; }
+    if (hkm[q]) { sum += vkm[q]; count++; }
+    if (hkp[q]) { sum += vkp[q]; count++; }
+    if (count > 0) {
+      gbuf[e0 + q] = sum / (T)count;
+      fo |= 1u << (8 * q);
+    } else {
+      gbuf[e0 + q] = T(0);
+    }
+  }
+  *reinterpret_cast<unsigned*>(vbuf + e0) = fo;
+}
+
 // applyForce (fluid-simulator.cc:13-18)
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_force(int gx, int gy, int gz, T* __restrict__ V, const uint8_t* __restrict__ valid, T gdt) {
@@ -1959,6 +2051,9 @@ int extrap_vel_t(fs_ctx* c, int iters) {
       const int gx = c->fdim[a][0], gy = c->fdim[a][1], gz = c->fdim[a][2];
       if (gz % 4 == 0)
         k_extrap_vel4<T><<<g3_grid(gx, gy, gz / 4), g3_block(), 0, c->stream>>>(
+            gx, gy, gz, c->g[a].as<T>(), c->gbuf[a].as<T>(), c->gvalid[a].as<uint8_t>(), c->gvalidbuf[a].as<uint8_t>());
+      else if (gz >= 4 && (gy * gz) % 4 == 0)
+        k_extrap_vel4f<T><<<fs_blocks(c->n_face[a] / 4, TPB), TPB, 0, c->stream>>>(
             gx, gy, gz, c->g[a].as<T>(), c->gbuf[a].as<T>(), c->gvalid[a].as<uint8_t>(), c->gvalidbuf[a].as<uint8_t>());
       else
         k_extrap_vel<T><<<g3_grid(gx, gy, gz), g3_block(), 0, c->stream>>>(
