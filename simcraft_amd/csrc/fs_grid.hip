@@ -1243,8 +1243,9 @@ __global__ __launch_bounds__(TPB) void k_extrap_vel(int gx, int gy, int gz, cons
 // multiple of 4 (rows stay 4-byte aligned): the flags of a face and of its x and y neighbours come
 // as one 4-byte load per direction, the values as four loads per direction where a flag asks for
 // them. Every face sums its valid neighbours in the order above and divides as above; valid faces
-// write only their flag (the value buffer keeps its stale entry). (A byte-by-byte form for the
-// other grids ran slower than the one-face kernel, which they keep.)
+// write only their flag (the value buffer keeps its stale entry). Other grids take
+// k_extrap_vel4f below (a byte-by-byte form of this kernel for them ran slower than the one-face
+// kernel).
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_extrap_vel4(int gx, int gy, int gz, const T* __restrict__ g,
                                                      T* __restrict__ gbuf, const uint8_t* __restrict__ val,
@@ -1362,16 +1363,34 @@ __global__ __launch_bounds__(TPB) void k_extrap_vel4f(int gx, int gy, int gz, co
     hkm[q] = k > 0 && (q > 0 ? (fc & (bit >> 8)) != 0 : fkm != 0);
     hkp[q] = k < gz - 1 && (q < 3 ? (fc & (bit << 8)) != 0 : fkp != 0);
   }
-  T vxm[4], vxp[4], vym[4], vyp[4], vkm[4], vkp[4];
+  // the four values at p .. p+3 where a face asks for one: all four loads at once when they lie
+  // on the array, else one load per asking face
+  auto values = [&](int p, const bool* h, T* v) {
+    if (!(h[0] || h[1] || h[2] || h[3])) return;
+    if (p >= 0 && p + 4 <= n) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) v[q] = g[p + q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (h[q]) v[q] = g[p + q];
+    }
+  };
+  T vxm[4] = {}, vxp[4] = {}, vym[4] = {}, vyp[4] = {}, vc[4] = {};
+  values(e0 - sx, hxm, vxm);
+  values(e0 + sx, hxp, vxp);
+  values(e0 - sy, hym, vym);
+  values(e0 + sy, hyp, vyp);
+  if (fc) {  // the group's own values (z neighbours inside the group), always on the array
+#pragma unroll
+    for (int q = 0; q < 4; q++) vc[q] = g[e0 + q];
+  }
+  const T gkm = hkm[0] ? g[e0 - 1] : T(0), gkp = hkp[3] ? g[e0 + 4] : T(0);
+  T vkm[4], vkp[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    const int e = e0 + q;
-    vxm[q] = hxm[q] ? g[e - sx] : T(0);
-    vxp[q] = hxp[q] ? g[e + sx] : T(0);
-    vym[q] = hym[q] ? g[e - sy] : T(0);
-    vyp[q] = hyp[q] ? g[e + sy] : T(0);
-    vkm[q] = hkm[q] ? g[e - 1] : T(0);
-    vkp[q] = hkp[q] ? g[e + 1] : T(0);
+    vkm[q] = q > 0 ? vc[q - 1] : gkm;
+    vkp[q] = q < 3 ? vc[q + 1] : gkp;
   }
   unsigned fo = 0;
 #pragma unroll
