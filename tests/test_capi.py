@@ -100,6 +100,18 @@ def test_create_rejects_bad_config():
     st = lib.fs_create(C.byref(cfg), C.byref(ctx))
     assert st in (_capi.FS_UNSUPPORTED, _capi.FS_NO_DEVICE)
     assert not ctx.value
+    # the configuration is checked before any device is looked for, so these hold without one
+    bad = [(dict(nx=0, ny=0, nz=8), "positive"), (dict(nx=8, ny=8, nz=-1), "positive"),
+           (dict(nx=8, ny=8, nz=8, size_y=0.0), "positive"),
+           (dict(nx=1290, ny=1290, nz=1290), "32-bit"),  # 1291^3 nodes >= 2^31
+           (dict(nx=8, ny=8, nz=8, precision=7), "precision"),
+           (dict(nx=8, ny=8, nz=8, preconditioner=9), "preconditioner")]
+    for kw, msg in bad:
+        cfg = _capi.default_config(**kw)
+        st = lib.fs_create(C.byref(cfg), C.byref(ctx))
+        assert st == _capi.FS_BAD_ARG, kw
+        assert not ctx.value
+        assert msg in lib.fs_last_error(None).decode(), (kw, lib.fs_last_error(None))
 
 
 CALLER = ROOT / "tests" / "capi_caller.c"
